@@ -1,0 +1,373 @@
+// Dedispersion pipeline behind aa_pipeline_api<T> and aa_py_pipeline_api.
+// Sequencing follows the reference's generic runner
+// (include/aa_permitted_pipelines_generic.hpp:606-913): one (time chunk, DM
+// range) per run(status) call; range 0 of a chunk uploads t_processed[0] +
+// maxshift samples starting at sample `inc` and corner-turns them; every
+// range dedisperses on its binning level and, with copy_ddtr_data_to_host,
+// lands in output_buffer()[range][dm][inc/inBin + t]
+// (src/aa_device_save_data.cu:41-50).  When every chunk is done the next call
+// reports finished_component and frees device memory, and the call after
+// that returns false with status finished.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "aa_ddtr_hip.h"
+#include "aa_log.hpp"
+#include "aa_pipeline_api.hpp"
+#include "ddtr_engine.hpp"
+
+namespace astroaccelerate {
+
+class aa_ddtr_pipeline_impl {
+public:
+    aa_ddtr_pipeline_impl(const aa_pipeline::pipeline &components, const aa_pipeline::pipeline_option &options,
+                          const aa_filterbank_metadata &metadata, const void *input, int bytes_per_sample,
+                          aa_device_info &card)
+        : m_components(components), m_options(options), m_metadata(metadata), m_input(input),
+          m_bytes(bytes_per_sample), m_card(card) {}
+
+    ~aa_ddtr_pipeline_impl() {
+        release_device();
+        free_host_output();
+    }
+
+    bool bind(const aa_ddtr_plan &plan) {
+        m_ready = false;
+        if (m_components.find(aa_pipeline::component::dedispersion) == m_components.end()) {
+            LOG(log_level::error, "bind: the pipeline does not contain dedispersion.");
+            return false;
+        }
+        if (m_bound) {
+            LOG(log_level::error, "bind: a dedispersion plan is already bound.");
+            return false;
+        }
+        const bool analysis = m_components.find(aa_pipeline::component::analysis) != m_components.end();
+        aa_ddtr_strategy s(plan, m_metadata, m_card.free_memory(), analysis, &m_card);
+        if (!s.ready()) return false;
+        m_plan = plan;
+        m_strategy = s;
+        m_bound = true;
+        return true;
+    }
+
+    aa_ddtr_strategy ddtr_strategy() { return m_bound ? m_strategy : aa_ddtr_strategy(); }
+
+    bool ready() {
+        if (m_ready) return true;
+        for (auto c : m_components) {
+            if (c != aa_pipeline::component::dedispersion) {
+                LOG(log_level::error, aa_pipeline::component_name(c) +
+                                          " is outside this library (dedispersion only).");
+                return false;
+            }
+        }
+        if (!m_bound) {
+            LOG(log_level::error, "De-dispersion plan is not ok.");
+            return false;
+        }
+        using opt = aa_pipeline::component_option;
+        for (auto o : m_options) {
+            // Options that change the dedispersed values are refused; options
+            // that only concern output files or other components are ignored.
+            const bool changes_values = o == opt::zero_dm || o == opt::zero_dm_with_outliers ||
+                                        o == opt::input_DDTR_normalization ||
+                                        o == opt::output_DDTR_normalization || o == opt::old_rfi;
+            if (changes_values) {
+                LOG(log_level::error, "pipeline option " + std::to_string((int)o) +
+                                          " (zero-DM / normalisation / RFI) is not supported by this library.");
+                aa_hip::set_last_error("unsupported pipeline option");
+                return false;
+            }
+            if (o != opt::copy_ddtr_data_to_host && o != opt::empty)
+                LOG(log_level::notice, "pipeline option " + std::to_string((int)o) + " has no effect here (ignored).");
+        }
+        if (m_metadata.nbits() != 8) {
+            LOG(log_level::error, "only 8-bit filterbank data is supported (nbits=" +
+                                      std::to_string(m_metadata.nbits()) + ").");
+            aa_hip::set_last_error("unsupported nbits");
+            return false;
+        }
+        if (!m_input) {
+            LOG(log_level::error, "no input data.");
+            return false;
+        }
+        if (!m_strategy.setup()) return false;
+        m_ready = true;
+        return true;
+    }
+
+    bool run() {
+        aa_pipeline_runner::status st;
+        while (run(st)) {
+        }
+        return st != aa_pipeline_runner::status::error;
+    }
+
+    bool run(aa_pipeline_runner::status &status) {
+        if (!m_ready || !setup()) {
+            LOG(log_level::error, "Pipeline could not start/resume because either pipeline is not ready or runner is not setup.");
+            status = aa_pipeline_runner::status::finished;
+            return false;
+        }
+        const int nranges = (int)m_strategy.get_nRanges();
+        if (m_chunk >= m_strategy.num_tchunks()) {
+            if (!m_notified_finish) {
+                release_device();
+                m_notified_finish = true;
+                status = aa_pipeline_runner::status::finished_component;
+                return true;
+            }
+            status = aa_pipeline_runner::status::finished;
+            return false;
+        }
+        if (!step(m_chunk, m_range)) {
+            release_device();
+            status = aa_pipeline_runner::status::error;
+            return false;
+        }
+        ++m_range;
+        if (m_range >= nranges) {
+            m_inc += m_strategy.t_processed()[0][(size_t)m_chunk];
+            ++m_chunk;
+            m_range = 0;
+        }
+        status = aa_pipeline_runner::status::has_more;
+        return true;
+    }
+
+    size_t get_nRanges() { return m_strategy.get_nRanges(); }
+    const int *get_ndms_array() { return m_strategy.ndms_data(); }
+    int dm_low(int range) { return m_strategy.dm_low(range); }
+    int total_computed_samples() {
+        int total = 0;
+        if (!m_strategy.t_processed().empty())
+            for (int v : m_strategy.t_processed()[0]) total += v;
+        return total;
+    }
+    int samples_in_current(int i, int j) { return m_strategy.t_processed()[(size_t)i][(size_t)j]; }
+    aa_filterbank_metadata metadata() { return m_metadata; }
+    int current_range() const { return m_range; }
+    int current_chunk() const { return m_chunk; }
+    long current_inc() const { return m_inc; }
+
+    bool cleanup() {
+        release_device();
+        free_host_output();
+        return true;
+    }
+
+    float ***output_buffer() {
+        if (m_options.find(aa_pipeline::component_option::copy_ddtr_data_to_host) == m_options.end())
+            LOG(log_level::error, "Could not get data from DDTR. Enable option copy_ddtr_data_to_host.");
+        return m_host_out;
+    }
+
+private:
+    // Allocates device and host buffers on first use (the reference's
+    // runner setup, include/aa_permitted_pipelines_generic.hpp:1090-1106).
+    bool setup() {
+        if (m_engine) return true;
+        if (m_finished_setup_failed) return false;
+        const int nranges = (int)m_strategy.get_nRanges();
+        const int nchunks = m_strategy.num_tchunks();
+        const int nchans = m_metadata.nchans();
+        std::vector<float> low, step;
+        std::vector<int> ndms, inbin;
+        int max_tp0 = 0;
+        for (int c = 0; c < nchunks; ++c) max_tp0 = std::max(max_tp0, m_strategy.t_processed()[0][(size_t)c]);
+        for (int r = 0; r < nranges; ++r) {
+            low.push_back(m_strategy.dm((size_t)r).low);
+            step.push_back(m_strategy.dm((size_t)r).step);
+            ndms.push_back(m_strategy.ndms((size_t)r));
+            inbin.push_back(m_strategy.dm((size_t)r).inBin);
+        }
+        const std::vector<float> dmshifts = m_strategy.dmshifts();
+        aa_hip::ddtr_engine *e = nullptr;
+        int st = aa_hip::ddtr_engine::create(m_card.currently_selected_card_id(), nchans, dmshifts.data(), nranges,
+                                             low.data(), step.data(), ndms.data(), inbin.data(),
+                                             m_metadata.tsamp(), m_strategy.maxshift(), max_tp0, &e);
+        if (st != AA_HIP_SUCCESS) {
+            LOG(log_level::error, std::string("pipeline setup: ") + aa_hip::last_error());
+            m_finished_setup_failed = true;
+            return false;
+        }
+        m_engine.reset(e);
+        (void)hipSetDevice(m_engine->device());
+        const size_t in_bytes = ((size_t)max_tp0 + (size_t)m_strategy.maxshift()) * (size_t)nchans * (size_t)m_bytes;
+        hipError_t he = hipMalloc(&m_d_input, in_bytes);
+        size_t max_out = 0;
+        for (int r = 0; r < nranges; ++r) {
+            int tp = 0;
+            for (int c = 0; c < nchunks; ++c) tp = std::max(tp, m_strategy.t_processed()[(size_t)r][(size_t)c]);
+            max_out = std::max(max_out, (size_t)tp * (size_t)ndms[(size_t)r]);
+        }
+        if (he == hipSuccess) he = hipMalloc(&m_d_output, std::max<size_t>(max_out, 1) * sizeof(float));
+        if (he == hipSuccess) he = hipStreamCreate(&m_stream);
+        if (he != hipSuccess) {
+            LOG(log_level::error, std::string("pipeline setup: ") + hipGetErrorString(he));
+            m_finished_setup_failed = true;
+            release_device();
+            return false;
+        }
+        if (m_options.find(aa_pipeline::component_option::copy_ddtr_data_to_host) != m_options.end())
+            allocate_host_output();
+        return true;
+    }
+
+    // Host output as float***[range][dm] row pointers into one block per
+    // range, so a whole range is one strided device-to-host copy.
+    void allocate_host_output() {
+        free_host_output();
+        const int nranges = (int)m_strategy.get_nRanges();
+        m_host_out = (float ***)std::calloc((size_t)nranges, sizeof(float **));
+        m_host_blocks.assign((size_t)nranges, nullptr);
+        for (int r = 0; r < nranges; ++r) {
+            size_t total = 0;
+            for (int v : m_strategy.t_processed()[(size_t)r]) total += (size_t)v;
+            const int nd = m_strategy.ndms((size_t)r);
+            m_host_blocks[(size_t)r] = (float *)std::calloc(std::max<size_t>(total * (size_t)nd, 1), sizeof(float));
+            m_host_out[r] = (float **)std::calloc((size_t)std::max(nd, 1), sizeof(float *));
+            for (int d = 0; d < nd; ++d) m_host_out[r][d] = m_host_blocks[(size_t)r] + (size_t)d * total;
+            m_host_total.push_back(total);
+        }
+    }
+
+    void free_host_output() {
+        if (m_host_out) {
+            for (size_t r = 0; r < m_host_blocks.size(); ++r) {
+                std::free(m_host_out[r]);
+                std::free(m_host_blocks[r]);
+            }
+            std::free(m_host_out);
+        }
+        m_host_out = nullptr;
+        m_host_blocks.clear();
+        m_host_total.clear();
+    }
+
+    void release_device() {
+        if (m_engine) (void)hipSetDevice(m_engine->device());
+        if (m_stream) {
+            (void)hipStreamSynchronize(m_stream);
+            (void)hipStreamDestroy(m_stream);
+        }
+        if (m_d_input) (void)hipFree(m_d_input);
+        if (m_d_output) (void)hipFree(m_d_output);
+        m_stream = nullptr;
+        m_d_input = nullptr;
+        m_d_output = nullptr;
+        m_engine.reset();
+    }
+
+    bool step(int chunk, int range) {
+        const int nchans = m_metadata.nchans();
+        const int tp = m_strategy.t_processed()[(size_t)range][(size_t)chunk];
+        (void)hipSetDevice(m_engine->device());
+        if (range == 0) {
+            const int tp0 = m_strategy.t_processed()[0][(size_t)chunk];
+            const size_t rows = (size_t)tp0 + (size_t)m_strategy.maxshift();
+            const size_t bytes = rows * (size_t)nchans * (size_t)m_bytes;
+            const char *src = (const char *)m_input + (size_t)m_inc * (size_t)nchans * (size_t)m_bytes;
+            hipError_t he = hipMemcpyAsync(m_d_input, src, bytes, hipMemcpyHostToDevice, m_stream);
+            if (he != hipSuccess) {
+                LOG(log_level::error, std::string("H2D copy: ") + hipGetErrorString(he));
+                return false;
+            }
+            int st = m_engine->ingest(m_d_input, m_bytes == 1 ? 8 : 16, tp0, m_stream);
+            if (st != AA_HIP_SUCCESS) {
+                LOG(log_level::error, std::string("corner turn: ") + aa_hip::last_error());
+                return false;
+            }
+            if (m_bytes == 2) {
+                st = m_engine->take_input_overflow(m_stream);
+                if (st != 0) {
+                    LOG(log_level::error, st > 0 ? std::string("input sample above 255 in 8-bit data")
+                                                 : std::string(aa_hip::last_error()));
+                    aa_hip::set_last_error("input sample above 255 in 8-bit data");
+                    return false;
+                }
+            }
+        }
+        int st = m_engine->run_range(range, tp, m_d_output, (size_t)tp, m_stream);
+        if (st != AA_HIP_SUCCESS) {
+            LOG(log_level::error, std::string("dedispersion: ") + aa_hip::last_error());
+            return false;
+        }
+        if (m_host_out) {
+            const int ib = m_strategy.dm((size_t)range).inBin;
+            const size_t col = (size_t)(m_inc / ib);
+            const size_t total = m_host_total[(size_t)range];
+            const int nd = m_strategy.ndms((size_t)range);
+            if (col + (size_t)tp > total) {
+                LOG(log_level::error, "output column offset exceeds the range's total samples.");
+                return false;
+            }
+            hipError_t he = hipMemcpy2DAsync(m_host_blocks[(size_t)range] + col, total * sizeof(float), m_d_output,
+                                             (size_t)tp * sizeof(float), (size_t)tp * sizeof(float), (size_t)nd,
+                                             hipMemcpyDeviceToHost, m_stream);
+            if (he != hipSuccess) {
+                LOG(log_level::error, std::string("D2H copy: ") + hipGetErrorString(he));
+                return false;
+            }
+        }
+        hipError_t he = hipStreamSynchronize(m_stream);
+        if (he != hipSuccess) {
+            LOG(log_level::error, std::string("device error: ") + hipGetErrorString(he));
+            return false;
+        }
+        return true;
+    }
+
+    aa_pipeline::pipeline m_components;
+    aa_pipeline::pipeline_option m_options;
+    aa_filterbank_metadata m_metadata;
+    const void *m_input;
+    int m_bytes;
+    aa_device_info &m_card;
+    aa_ddtr_plan m_plan;
+    aa_ddtr_strategy m_strategy;
+    bool m_bound = false;
+    bool m_ready = false;
+    bool m_finished_setup_failed = false;
+    bool m_notified_finish = false;
+    int m_chunk = 0;
+    int m_range = 0;
+    long m_inc = 0;
+    std::unique_ptr<aa_hip::ddtr_engine> m_engine;
+    void *m_d_input = nullptr;
+    float *m_d_output = nullptr;
+    hipStream_t m_stream = nullptr;
+    float ***m_host_out = nullptr;
+    std::vector<float *> m_host_blocks;
+    std::vector<size_t> m_host_total;
+};
+
+// ------------------------------------------------------- aa_ddtr_pipeline
+aa_ddtr_pipeline::aa_ddtr_pipeline(const aa_pipeline::pipeline &requested_pipeline,
+                                   const aa_pipeline::pipeline_option &pipeline_options,
+                                   const aa_filterbank_metadata &filterbank_metadata, const void *input_data,
+                                   int input_bytes_per_sample, aa_device_info &card_info)
+    : m_impl(new aa_ddtr_pipeline_impl(requested_pipeline, pipeline_options, filterbank_metadata, input_data,
+                                       input_bytes_per_sample, card_info)) {}
+aa_ddtr_pipeline::~aa_ddtr_pipeline() = default;
+bool aa_ddtr_pipeline::bind(const aa_ddtr_plan &plan) { return m_impl->bind(plan); }
+aa_ddtr_strategy aa_ddtr_pipeline::ddtr_strategy() { return m_impl->ddtr_strategy(); }
+size_t aa_ddtr_pipeline::get_nRanges() { return m_impl->get_nRanges(); }
+const int *aa_ddtr_pipeline::get_ndms_array() { return m_impl->get_ndms_array(); }
+int aa_ddtr_pipeline::dm_low(const int range) { return m_impl->dm_low(range); }
+int aa_ddtr_pipeline::total_computed_samples() { return m_impl->total_computed_samples(); }
+int aa_ddtr_pipeline::samples_in_current(int i, int j) { return m_impl->samples_in_current(i, j); }
+aa_filterbank_metadata aa_ddtr_pipeline::metadata() { return m_impl->metadata(); }
+bool aa_ddtr_pipeline::ready() { return m_impl->ready(); }
+bool aa_ddtr_pipeline::run() { return m_impl->run(); }
+bool aa_ddtr_pipeline::run(aa_pipeline_runner::status &status_code) { return m_impl->run(status_code); }
+int aa_ddtr_pipeline::get_current_range() { return m_impl->current_range(); }
+int aa_ddtr_pipeline::get_current_tchunk() { return m_impl->current_chunk(); }
+long int aa_ddtr_pipeline::get_current_inc() { return m_impl->current_inc(); }
+bool aa_ddtr_pipeline::cleanup() { return m_impl->cleanup(); }
+float ***aa_ddtr_pipeline::output_buffer() { return m_impl->output_buffer(); }
+
+}  // namespace astroaccelerate

@@ -1,0 +1,130 @@
+// Drop-in device functions with the reference's signatures
+// (include/aa_dedisperse.hpp:20, include/aa_device_load_data.hpp,
+// include/aa_corner_turn.hpp, include/aa_bin_gpu.hpp), on the reference's
+// u16 [c][t] device layout and the legacy default stream.  The reference
+// carries i_nchans, i_nsamp, i_t_processed_s and dm_shifts between these
+// calls in __constant__ memory; here they are one mutex-guarded host record
+// plus a library-owned device copy of the dmshift table.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "aa_bin_gpu.hpp"
+#include "aa_corner_turn.hpp"
+#include "aa_dedisperse.hpp"
+#include "aa_device_load_data.hpp"
+#include "aa_log.hpp"
+#include "ddtr_engine.hpp"
+
+namespace astroaccelerate {
+
+namespace {
+struct shim_constants {
+    int nchans = 0;
+    long i_nsamp = 0;
+    int t_processed = 0;
+    std::vector<float> dmshifts;
+    float *d_dmshifts = nullptr;
+    int d_capacity = 0;
+    int device = -1;
+};
+std::mutex g_mutex;
+shim_constants g_const;
+}  // namespace
+
+// src/aa_device_load_data.cu:39-56
+void load_chunk_data(unsigned short *device_pointer, unsigned short const *const host_pointer, int t_processed,
+                     int maxshift, int nchans, float *dmshifts, float *d_dm_shifts, int nbits) {
+    (void)d_dm_shifts;
+    (void)nbits;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    const long length = (long)t_processed + maxshift;
+    const size_t size = (size_t)nchans * (size_t)length * sizeof(unsigned short);
+    hipError_t e = hipMemcpy(device_pointer, host_pointer, size, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        LOG(log_level::error, std::string("load_chunk_data: ") + hipGetErrorString(e));
+        return;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (g_const.d_dmshifts && (g_const.d_capacity < nchans || g_const.device != dev)) {
+        (void)hipFree(g_const.d_dmshifts);
+        g_const.d_dmshifts = nullptr;
+    }
+    if (!g_const.d_dmshifts) {
+        if (hipMalloc(&g_const.d_dmshifts, sizeof(float) * (size_t)nchans) != hipSuccess) {
+            LOG(log_level::error, "load_chunk_data: cannot allocate the dmshift table");
+            g_const.d_dmshifts = nullptr;
+            return;
+        }
+        g_const.d_capacity = nchans;
+        g_const.device = dev;
+    }
+    (void)hipMemcpy(g_const.d_dmshifts, dmshifts, sizeof(float) * (size_t)nchans, hipMemcpyHostToDevice);
+    g_const.dmshifts.assign(dmshifts, dmshifts + nchans);
+    g_const.nchans = nchans;
+    g_const.i_nsamp = length;
+    g_const.t_processed = t_processed;
+}
+
+// src/aa_device_load_data.cu:58-61
+void set_dedispersion_constants(int t_processed, int maxshift) {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    g_const.i_nsamp = (long)t_processed + maxshift;
+    g_const.t_processed = t_processed;
+}
+
+// src/aa_dedisperse.cpp:5-163.  The failsafe kernel's semantics for every
+// geometry: its grid covers t < 8*(t_processed/8) and d < 64*(ndms/64), the
+// output row stride is the current i_t_processed_s.
+int dedisperse(int i, int t_processed, int *inBin, float *dmshifts, unsigned short *d_input, float *d_output,
+               float *d_dm_shifts, int nchans, float *tsamp, float *dm_low, float *dm_step, int const *const ndms,
+               int nbits, int failsafe) {
+    (void)dmshifts;
+    (void)d_dm_shifts;
+    (void)nchans;
+    (void)failsafe;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (nbits != 8 && nbits != 16) {
+        LOG(log_level::error, "dedisperse: only 8-bit samples (u16 containers) are supported");
+        return 1;
+    }
+    if (!g_const.d_dmshifts || g_const.nchans <= 0) {
+        LOG(log_level::error, "dedisperse: load_chunk_data has not been called");
+        return 1;
+    }
+    const float mstartdm = (float)(dm_low[i] / (*tsamp));
+    const float mdmstep = (float)(dm_step[i] / (*tsamp));
+    const int t_cov = (t_processed / 8) * 8;
+    const int d_cov = (ndms[i] / 64) * 64;
+    if (aa_hip::check_delays(g_const.dmshifts.data(), g_const.nchans, d_cov, mstartdm, mdmstep, t_cov,
+                             g_const.i_nsamp) != 0) {
+        LOG(log_level::error, std::string("dedisperse: ") + aa_hip::last_error());
+        return 1;
+    }
+    hipError_t e = aa_hip::launch_dedisperse_direct_u16(d_input, (size_t)g_const.i_nsamp, g_const.nchans,
+                                                        g_const.d_dmshifts, d_cov, t_cov, mstartdm, mdmstep,
+                                                        inBin[i], d_output, (size_t)g_const.t_processed, nullptr);
+    if (e == hipSuccess) e = hipGetLastError();
+    return e == hipSuccess ? 0 : 1;
+}
+
+// src/aa_corner_turn.cpp:118-121: [secondary][primary] -> [primary][secondary].
+void corner_turn_SM_inplace(unsigned short *const d_input, unsigned short *const d_temporary,
+                            const size_t primary_size, const size_t secondary_size) {
+    hipError_t e = aa_hip::launch_transpose_u16(d_input, (int)secondary_size, (int)primary_size, d_temporary, nullptr);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_input, d_temporary, primary_size * secondary_size * sizeof(unsigned short),
+                           hipMemcpyDeviceToDevice, nullptr);
+    if (e != hipSuccess) LOG(log_level::error, std::string("corner_turn_SM_inplace: ") + hipGetErrorString(e));
+}
+
+// src/aa_bin_gpu.cpp:8-44
+void bin_gpu(unsigned short *const d_input, float *const d_output, const int nchans, const int nsamp) {
+    hipError_t e = aa_hip::launch_bin2_u16_packed(d_input, nchans, nsamp, d_output, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) LOG(log_level::error, std::string("bin_gpu: ") + hipGetErrorString(e));
+}
+
+}  // namespace astroaccelerate

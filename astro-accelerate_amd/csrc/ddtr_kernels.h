@@ -1,0 +1,68 @@
+// Host-side launchers for the DDTR HIP kernels (gfx950).  All launchers are
+// asynchronous on the given stream, check their arguments on the host before
+// launching, and return hipSuccess or the first error.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace aa_hip {
+
+// Dedispersion kernel variants selectable by callers and the autotuner.
+enum class ddtr_kernel : int {
+    automatic = 0,  // pick the fastest variant valid for the geometry
+    direct = 1,     // one thread per (DM, sample); global-memory gathers
+    lds_swar = 2,   // LDS-staged channel windows, packed 16-bit integer sums
+};
+
+// Tile shape of the LDS kernel.  dms_per_block DMs x (64 * samples_per_lane)
+// samples per work-group, channels staged chunk_chans at a time.
+struct lds_tile {
+    int dms_per_wave;      // DM trials handled by one wave (each lane: all of them)
+    int waves;             // waves per work-group
+    int chunk_chans;       // channels staged in LDS per pass
+};
+
+// [t][c] -> [c][t].  in_tc holds nsamp rows of nchans samples.
+hipError_t launch_corner_turn_u8(const uint8_t* in_tc, int nsamp, int nchans,
+                                 uint8_t* out_ct, size_t out_pitch, hipStream_t s);
+// u16 input carrying 8-bit samples; a value above 255 sets *d_err to 1.
+hipError_t launch_corner_turn_u16(const uint16_t* in_tc, int nsamp, int nchans,
+                                  uint8_t* out_ct, size_t out_pitch, int* d_err,
+                                  hipStream_t s);
+// u16 [t][c] -> u16 [c][t] (the reference's in-place corner turn, via temp).
+hipError_t launch_transpose_u16(const uint16_t* in, int rows, int cols,
+                                uint16_t* out, hipStream_t s);
+
+// out[c][t] = (in[c][2t] + in[c][2t+1]) >> 1 for t < n_out.
+hipError_t launch_bin2_u8(const uint8_t* in, size_t in_pitch, int nchans, int n_out,
+                          uint8_t* out, size_t out_pitch, hipStream_t s);
+// Reference bin_gpu semantics on u16 rows of length nsamp packed into rows
+// of nsamp/2 (src/aa_bin_gpu.cpp:8-44), in place through a temp buffer.
+hipError_t launch_bin2_u16_packed(uint16_t* data, int nchans, int nsamp, float* temp,
+                                  hipStream_t s);
+
+// One DM range.  L: nchans rows of u8 (or u16) samples with row pitch
+// `pitch` elements; out: ndms rows of t_processed floats, row pitch out_pitch.
+hipError_t launch_dedisperse_direct_u8(const uint8_t* L, size_t pitch, int nchans,
+                                       const float* d_dmshifts, int ndms, int t_processed,
+                                       float mstartdm, float mdmstep, int inBin,
+                                       float* out, size_t out_pitch, hipStream_t s);
+hipError_t launch_dedisperse_direct_u16(const uint16_t* L, size_t pitch, int nchans,
+                                        const float* d_dmshifts, int ndms, int t_processed,
+                                        float mstartdm, float mdmstep, int inBin,
+                                        float* out, size_t out_pitch, hipStream_t s);
+
+// Synthetic 8-bit filterbank noise: N(mean, sigma^2) rounded and clipped to
+// [0,255], SplitMix64 counter stream keyed by seed (value depends only on
+// seed and the flat [t][c] index).
+hipError_t launch_synth_noise_u8(uint64_t seed, int64_t n, double mean, double sigma,
+                                 uint8_t* out_tc, hipStream_t s);
+// Adds `amplitude` (saturating) to `width` samples at t0 + floor(dmshift[c]*
+// (float)(dm/tsamp)) in each channel of a [t][c] u8 buffer.
+hipError_t launch_inject_pulse_u8(uint8_t* data_tc, int nsamp, int nchans,
+                                  const float* d_dmshifts, float dm_over_tsamp,
+                                  int t0, int width, int amplitude, hipStream_t s);
+
+}  // namespace aa_hip

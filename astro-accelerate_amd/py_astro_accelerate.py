@@ -1,0 +1,573 @@
+"""ctypes front end of libaa_ddtr_hip.so (MI355X dedispersion).
+
+Two parts:
+
+* The reference's Python interface for the dedispersion path
+  (cmake/py_astro_accelerate.py.in): ``aa_py_dm``, ``aa_py_ddtr_plan``,
+  ``aa_py_filterbank_metadata``, ``filterbank_metadata_struct``,
+  ``aa_py_pipeline_components``, ``aa_py_pipeline_component_options`` and
+  ``aa_py_pipeline`` keep their names, constructor arguments and methods, so a
+  script written for the reference runs against this library.  Plus
+  ``aa_py_ddtr_strategy`` (host-only strategy for explicit free memory).
+
+* Device-level helpers over torch tensors (``DdtrPlan``, ``corner_turn_u8``,
+  ``bin2_u8``, ``dedisperse_u8``, ``synth_noise_u8``, ``inject_pulse_u8``)
+  calling the aa_hip_* entry points with raw device pointers and the current
+  torch stream.  torch only provides device memory and streams.
+
+The library must be built (``make -C astro-accelerate_amd``); loading fails
+loudly when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("AA_DDTR_HIP_LIB", os.path.join(_HERE, "lib", "libaa_ddtr_hip.so"))
+
+_lib = None
+
+
+def load_library(path: Optional[str] = None) -> ctypes.CDLL:
+    """Load (once) and return the HIP dedispersion library; raise if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"libaa_ddtr_hip.so not found at {p}; build it with `make -C astro-accelerate_amd`")
+    lib = ctypes.CDLL(p)
+    _declare(lib)
+    if path is None:
+        _lib = lib
+    return lib
+
+
+PFLOAT = ctypes.POINTER(ctypes.c_float)
+PPFLOAT = ctypes.POINTER(PFLOAT)
+PPPFLOAT = ctypes.POINTER(PPFLOAT)
+
+
+class filterbank_metadata_struct(ctypes.Structure):
+    """Same fields as the reference's filterbank_metadata_struct."""
+    _fields_ = [
+        ("m_tstart", ctypes.c_double),
+        ("m_tsamp", ctypes.c_double),
+        ("m_fch1", ctypes.c_double),
+        ("m_foff", ctypes.c_double),
+        ("m_nbits", ctypes.c_int),
+        ("m_nsamples", ctypes.c_int),
+        ("m_nchans", ctypes.c_int),
+        ("m_src_raj", ctypes.c_double),
+    ]
+
+
+class aa_py_pipeline_components(ctypes.Structure):
+    _fields_ = [("dedispersion", ctypes.c_bool), ("analysis", ctypes.c_bool),
+                ("periodicity", ctypes.c_bool), ("fdas", ctypes.c_bool)]
+
+
+class aa_py_pipeline_component_options(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_bool) for n in (
+        "zero_dm", "zero_dm_with_outliers", "input_DDTR_normalization", "output_DDTR_normalization",
+        "set_bandpass_average", "old_rfi", "copy_ddtr_data_to_host", "msd_baseline_noise", "output_dmt",
+        "output_ffdot_plan", "output_fdas_list", "candidate_algorithm", "fdas_custom_fft", "fdas_inbin",
+        "fdas_norm")]
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    vp, i, f, d, sz, b = (ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double,
+                          ctypes.c_size_t, ctypes.c_bool)
+    u64, i64 = ctypes.c_uint64, ctypes.c_int64
+    pi = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "aa_hip_last_error": ([], ctypes.c_char_p),
+        "aa_hip_version": ([], ctypes.c_char_p),
+        "aa_py_ddtr_plan": ([], vp),
+        "aa_py_ddtr_plan_delete": ([vp], None),
+        "aa_py_ddtr_plan_add_dm": ([vp, f, f, f, i, i], b),
+        "aa_py_ddtr_plan_range": ([vp], sz),
+        "aa_py_ddtr_plan_set_power": ([vp, f], b),
+        "aa_py_ddtr_plan_power": ([vp], f),
+        "aa_py_ddtr_plan_set_enable_msd_baseline": ([vp, b], b),
+        "aa_py_ddtr_plan_enable_msd_baseline_noise": ([vp], b),
+        "aa_py_ddtr_plan_bind_bandpass_normalization": ([vp, PFLOAT, i], b),
+        "aa_py_filterbank_metadata": ([d, d, i, i, d, d, i, d], vp),
+        "aa_py_filterbank_metadata_delete": ([vp], None),
+        "aa_py_filterbank_metadata_tstart": ([vp], d),
+        "aa_py_pipeline_api": ([aa_py_pipeline_components, aa_py_pipeline_component_options,
+                                filterbank_metadata_struct, ctypes.POINTER(ctypes.c_ushort), i], vp),
+        "aa_py_pipeline_api_u8": ([aa_py_pipeline_components, aa_py_pipeline_component_options,
+                                   filterbank_metadata_struct, ctypes.POINTER(ctypes.c_ubyte), i], vp),
+        "aa_py_pipeline_api_delete": ([vp], None),
+        "aa_py_pipeline_api_bind_ddtr_plan": ([vp, vp], b),
+        "aa_py_pipeline_api_ddtr_strategy": ([vp], vp),
+        "aa_py_ddtr_strategy_delete": ([vp], None),
+        "aa_py_get_ddtr_nRanges": ([vp], sz),
+        "aa_py_get_ndms_array": ([vp], pi),
+        "aa_py_get_t_proc": ([vp, i, i], i),
+        "aa_py_dm_low": ([vp, i], i),
+        "aa_py_total_computed_samples": ([vp], i),
+        "aa_py_pipeline_api_run": ([vp, pi], b),
+        "aa_py_buffer": ([vp], PPPFLOAT),
+        "aa_py_current_range": ([vp], i),
+        "aa_py_current_time_chunk": ([vp], i),
+        "aa_py_current_inc": ([vp], ctypes.c_long),
+        "aa_py_cleanup": ([vp], b),
+        "aa_py_ddtr_strategy_create": ([vp, filterbank_metadata_struct, sz, b], vp),
+        "aa_py_ddtr_strategy_ready": ([vp], b),
+        "aa_py_ddtr_strategy_maxshift": ([vp], i),
+        "aa_py_ddtr_strategy_num_tchunks": ([vp], i),
+        "aa_py_ddtr_strategy_max_ndms": ([vp], i),
+        "aa_py_ddtr_strategy_nranges": ([vp], sz),
+        "aa_py_ddtr_strategy_ndms": ([vp, i], i),
+        "aa_py_ddtr_strategy_t_processed": ([vp, i, i], i),
+        "aa_py_ddtr_strategy_dm": ([vp, i, PFLOAT, PFLOAT, PFLOAT, pi, pi], b),
+        "aa_py_ddtr_strategy_dmshifts": ([vp, PFLOAT, i], b),
+        "aa_hip_ddtr_create": ([i, i, PFLOAT, i, PFLOAT, PFLOAT, pi, pi, d, i, i,
+                                ctypes.POINTER(vp)], i),
+        "aa_hip_ddtr_destroy": ([vp], i),
+        "aa_hip_ddtr_set_kernel": ([vp, i], i),
+        "aa_hip_ddtr_device_bytes": ([vp], sz),
+        "aa_hip_ddtr_execute_chunk": ([vp, vp, i, pi, ctypes.POINTER(vp), ctypes.POINTER(sz), vp], i),
+        "aa_hip_ddtr_ingest": ([vp, vp, i, i, vp], i),
+        "aa_hip_ddtr_run_range": ([vp, i, i, vp, sz, vp], i),
+        "aa_hip_corner_turn_u8": ([vp, i, i, vp, sz, vp], i),
+        "aa_hip_corner_turn_u16": ([vp, i, i, vp, sz, vp, vp], i),
+        "aa_hip_bin2_u8": ([vp, sz, i, i, vp, sz, vp], i),
+        "aa_hip_dedisperse_u8": ([vp, sz, i, i, PFLOAT, vp, i, i, f, f, i, vp, sz, i, vp], i),
+        "aa_hip_synth_noise_u8": ([u64, i64, d, d, vp, vp], i),
+        "aa_hip_inject_pulse_u8": ([vp, i, i, vp, d, d, i, i, i, vp], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def _lib_or_raise() -> ctypes.CDLL:
+    return load_library()
+
+
+class AAHipError(RuntimeError):
+    """A negative aa_hip_status, with the library's message."""
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = _lib_or_raise().aa_hip_last_error().decode(errors="replace")
+        raise AAHipError(f"{what} failed with status {status}: {msg}")
+
+
+# ---------------------------------------------------------------- FFI mirror
+class aa_py_dm:
+    def __init__(self, low: float, high: float, step: float, inBin: int, outBin: int):
+        self.m_low, self.m_high, self.m_step = low, high, step
+        self.m_inBin, self.m_outBin = inBin, outBin
+
+    def low(self):
+        return self.m_low
+
+    def high(self):
+        return self.m_high
+
+    def step(self):
+        return self.m_step
+
+    def inBin(self):
+        return self.m_inBin
+
+    def outBin(self):
+        return self.m_outBin
+
+
+class aa_py_ddtr_plan:
+    """Same constructor as the reference: an array of aa_py_dm."""
+
+    def __init__(self, dm):
+        lib = _lib_or_raise()
+        self.m_obj = lib.aa_py_ddtr_plan()
+        self.m_dm = list(dm)
+        for r in self.m_dm:
+            if not isinstance(r, aa_py_dm):
+                raise TypeError(f"Supplied dm is the wrong type, {type(r).__name__}")
+            lib.aa_py_ddtr_plan_add_dm(self.m_obj, r.low(), r.high(), r.step(), int(r.inBin()), int(r.outBin()))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def close(self):
+        if self.m_obj:
+            _lib_or_raise().aa_py_ddtr_plan_delete(self.m_obj)
+            self.m_obj = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pointer(self):
+        return self.m_obj
+
+    def set_power(self, power: float):
+        return _lib_or_raise().aa_py_ddtr_plan_set_power(self.m_obj, power)
+
+    def power(self):
+        return _lib_or_raise().aa_py_ddtr_plan_power(self.m_obj)
+
+    def set_enable_msd_baseline_noise(self, flag: bool):
+        return _lib_or_raise().aa_py_ddtr_plan_set_enable_msd_baseline(self.m_obj, bool(flag))
+
+    def enable_msd_baseline_noise(self):
+        return _lib_or_raise().aa_py_ddtr_plan_enable_msd_baseline_noise(self.m_obj)
+
+    def bind_bandpass_normalization(self, data, data_size: int):
+        arr = np.ascontiguousarray(data, dtype=np.float32)
+        return _lib_or_raise().aa_py_ddtr_plan_bind_bandpass_normalization(
+            self.m_obj, arr.ctypes.data_as(PFLOAT), int(data_size))
+
+    def range(self):
+        return _lib_or_raise().aa_py_ddtr_plan_range(self.m_obj)
+
+    def print_info(self):
+        print("AA_PY_DDTR_PLAN INFORMATION:")
+        for r in self.m_dm:
+            print(f"     aa_py_ddtr_plan range: low {r.low()}, high {r.high()}, step {r.step()}, "
+                  f"inBin {r.inBin()}, outBin {r.outBin()}")
+
+
+class aa_py_filterbank_metadata:
+    def __init__(self, tstart: float, tsamp: float, nbits: int, nsamples: int, fch1: float, foff: float,
+                 nchans: int, src_raj: float = 0.0):
+        self.m_obj = _lib_or_raise().aa_py_filterbank_metadata(tstart, tsamp, nbits, nsamples, fch1, foff,
+                                                               nchans, src_raj)
+        self.m_struct = filterbank_metadata_struct(tstart, tsamp, fch1, foff, nbits, nsamples, nchans, src_raj)
+
+    def pointer(self):
+        return self.m_obj
+
+    def tstart(self):
+        return _lib_or_raise().aa_py_filterbank_metadata_tstart(self.m_obj)
+
+    def struct(self) -> filterbank_metadata_struct:
+        return self.m_struct
+
+    def __del__(self):
+        try:
+            if self.m_obj:
+                _lib_or_raise().aa_py_filterbank_metadata_delete(self.m_obj)
+                self.m_obj = None
+        except Exception:
+            pass
+
+
+class aa_py_pipeline:
+    """The reference's aa_py_pipeline for the dedispersion component.
+
+    ``input_data`` is a ctypes pointer (or numpy array) of u16 samples in
+    [t][c] order, as the reference's SIGPROC reader produces; a numpy uint8
+    array selects the u8 entry point.
+    """
+
+    def __init__(self, pipeline: aa_py_pipeline_components, pipeline_options: aa_py_pipeline_component_options,
+                 metadata: filterbank_metadata_struct, input_data, card_number: int):
+        lib = _lib_or_raise()
+        self._keepalive = input_data
+        if isinstance(input_data, np.ndarray):
+            if input_data.dtype == np.uint8:
+                ptr = input_data.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+                self.m_obj = lib.aa_py_pipeline_api_u8(pipeline, pipeline_options, metadata, ptr, int(card_number))
+                return
+            if input_data.dtype != np.uint16:
+                raise TypeError("input_data must be uint16 or uint8")
+            input_data = input_data.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort))
+        self.m_obj = lib.aa_py_pipeline_api(pipeline, pipeline_options, metadata, input_data, int(card_number))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def close(self):
+        if self.m_obj:
+            _lib_or_raise().aa_py_pipeline_api_delete(self.m_obj)
+            self.m_obj = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bind_ddtr_plan(self, plan: aa_py_ddtr_plan):
+        return _lib_or_raise().aa_py_pipeline_api_bind_ddtr_plan(self.m_obj, plan.pointer())
+
+    def ddtr_strategy(self):
+        return _lib_or_raise().aa_py_pipeline_api_ddtr_strategy(self.m_obj)
+
+    def ddtr_range(self):
+        return _lib_or_raise().aa_py_get_ddtr_nRanges(self.m_obj)
+
+    def ddtr_ndms(self):
+        lib = _lib_or_raise()
+        n = self.ddtr_range()
+        p = lib.aa_py_get_ndms_array(self.m_obj)
+        return [p[k] for k in range(n)]
+
+    def run(self):
+        self.m_status_code_c_int = ctypes.c_int()
+        return bool(_lib_or_raise().aa_py_pipeline_api_run(self.m_obj, ctypes.byref(self.m_status_code_c_int)))
+
+    def status_code(self):
+        return self.m_status_code_c_int.value
+
+    def get_buffer(self):
+        lib = _lib_or_raise()
+        return lib.aa_py_total_computed_samples(self.m_obj), lib.aa_py_buffer(self.m_obj)
+
+    def dm_low(self, pos: int):
+        return _lib_or_raise().aa_py_dm_low(self.m_obj, int(pos))
+
+    def t_processed(self, r: int, chunk: int):
+        return _lib_or_raise().aa_py_get_t_proc(self.m_obj, int(r), int(chunk))
+
+    def current_range(self):
+        return _lib_or_raise().aa_py_current_range(self.m_obj)
+
+    def current_time_chunk(self):
+        return _lib_or_raise().aa_py_current_time_chunk(self.m_obj)
+
+    def current_inc(self):
+        return _lib_or_raise().aa_py_current_inc(self.m_obj)
+
+    def cleanUp(self):
+        return bool(_lib_or_raise().aa_py_cleanup(self.m_obj))
+
+    def output_planes(self, strategy: "aa_py_ddtr_strategy") -> List[np.ndarray]:
+        """Copies of the host output as numpy [ndms][total_t] arrays per range."""
+        _, buf = self.get_buffer()
+        if not buf:
+            raise AAHipError("no host output (enable copy_ddtr_data_to_host)")
+        planes = []
+        for r in range(strategy.nranges):
+            total = sum(strategy.t_processed(r, k) for k in range(strategy.num_tchunks))
+            nd = strategy.ndms(r)
+            # rows of one range are contiguous in this library
+            base = np.ctypeslib.as_array(buf[r][0], shape=(nd * total,))
+            planes.append(base.reshape(nd, total).copy())
+        return planes
+
+
+class aa_py_ddtr_strategy:
+    """Strategy of a plan for explicit metadata and free device memory (host only)."""
+
+    def __init__(self, plan: aa_py_ddtr_plan, metadata: filterbank_metadata_struct, free_memory: int,
+                 enable_analysis: bool = False):
+        lib = _lib_or_raise()
+        self.m_obj = lib.aa_py_ddtr_strategy_create(plan.pointer(), metadata, int(free_memory), bool(enable_analysis))
+        self.nchans = metadata.m_nchans
+        self.tsamp = metadata.m_tsamp
+        if not self.m_obj:
+            raise AAHipError("strategy allocation failed")
+
+    def __del__(self):
+        try:
+            if self.m_obj:
+                _lib_or_raise().aa_py_ddtr_strategy_delete(self.m_obj)
+                self.m_obj = None
+        except Exception:
+            pass
+
+    @property
+    def ready(self) -> bool:
+        return bool(_lib_or_raise().aa_py_ddtr_strategy_ready(self.m_obj))
+
+    @property
+    def maxshift(self) -> int:
+        return _lib_or_raise().aa_py_ddtr_strategy_maxshift(self.m_obj)
+
+    @property
+    def num_tchunks(self) -> int:
+        return _lib_or_raise().aa_py_ddtr_strategy_num_tchunks(self.m_obj)
+
+    @property
+    def max_ndms(self) -> int:
+        return _lib_or_raise().aa_py_ddtr_strategy_max_ndms(self.m_obj)
+
+    @property
+    def nranges(self) -> int:
+        return _lib_or_raise().aa_py_ddtr_strategy_nranges(self.m_obj)
+
+    def ndms(self, r: int) -> int:
+        return _lib_or_raise().aa_py_ddtr_strategy_ndms(self.m_obj, int(r))
+
+    def t_processed(self, r: int, chunk: int) -> int:
+        return _lib_or_raise().aa_py_ddtr_strategy_t_processed(self.m_obj, int(r), int(chunk))
+
+    def dm(self, r: int):
+        low, high, step = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        ib, ob = ctypes.c_int(), ctypes.c_int()
+        ok = _lib_or_raise().aa_py_ddtr_strategy_dm(self.m_obj, int(r), ctypes.byref(low), ctypes.byref(high),
+                                                    ctypes.byref(step), ctypes.byref(ib), ctypes.byref(ob))
+        if not ok:
+            raise IndexError(r)
+        return low.value, high.value, step.value, ib.value, ob.value
+
+    def dmshifts(self) -> np.ndarray:
+        out = np.zeros(self.nchans, dtype=np.float32)
+        if not _lib_or_raise().aa_py_ddtr_strategy_dmshifts(self.m_obj, out.ctypes.data_as(PFLOAT), self.nchans):
+            raise AAHipError("dmshifts unavailable")
+        return out
+
+    def range_scalars(self, r: int):
+        """(mstartdm, mdmstep) of range r as float32, derived as the reference does."""
+        low, _, step, ib, _ = self.dm(r)
+        tsamp_r = np.float32(np.float32(self.tsamp) * np.float32(ib))
+        return np.float32(np.float32(low) / tsamp_r), np.float32(np.float32(step) / tsamp_r)
+
+
+def make_plan(ranges: Sequence[Sequence[float]]) -> aa_py_ddtr_plan:
+    """Plan from (low, high, step, inBin[, outBin]) tuples."""
+    return aa_py_ddtr_plan([aa_py_dm(r[0], r[1], r[2], int(r[3]), int(r[4]) if len(r) > 4 else int(r[3]))
+                            for r in ranges])
+
+
+def make_metadata(nchans: int, nsamples: int, fch1: float, foff: float, tsamp: float, nbits: int = 8,
+                  tstart: float = 0.0) -> filterbank_metadata_struct:
+    return filterbank_metadata_struct(tstart, tsamp, fch1, foff, nbits, nsamples, nchans, 0.0)
+
+
+# ------------------------------------------------------ device-level helpers
+def _stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class DdtrPlan:
+    """Device plan (aa_hip_ddtr handle) for a strategy's geometry.
+
+    ``execute_chunk`` runs corner turn + binning cascade + every range on
+    torch device tensors: the input is [t][c] uint8 (or int16/uint16-holding
+    8-bit samples), outputs are float32 [ndms_r][>= t_processed_r] tensors.
+    """
+
+    def __init__(self, dmshifts: np.ndarray, ranges: Sequence[tuple], tsamp: float, maxshift: int,
+                 max_t_processed0: int, device: int = 0):
+        lib = _lib_or_raise()
+        self.dmshifts = np.ascontiguousarray(dmshifts, dtype=np.float32)
+        self.nchans = int(self.dmshifts.size)
+        self.ranges = [tuple(r) for r in ranges]  # (dm_low, dm_step, ndms, inBin)
+        n = len(self.ranges)
+        low = (ctypes.c_float * n)(*[r[0] for r in self.ranges])
+        step = (ctypes.c_float * n)(*[r[1] for r in self.ranges])
+        ndms = (ctypes.c_int * n)(*[int(r[2]) for r in self.ranges])
+        inbin = (ctypes.c_int * n)(*[int(r[3]) for r in self.ranges])
+        h = ctypes.c_void_p()
+        check(lib.aa_hip_ddtr_create(int(device), self.nchans, self.dmshifts.ctypes.data_as(PFLOAT), n, low, step,
+                                     ndms, inbin, float(tsamp), int(maxshift), int(max_t_processed0),
+                                     ctypes.byref(h)), "aa_hip_ddtr_create")
+        self.handle = h
+        self.tsamp = tsamp
+        self.maxshift = int(maxshift)
+
+    @classmethod
+    def from_strategy(cls, s: aa_py_ddtr_strategy, device: int = 0) -> "DdtrPlan":
+        ranges = []
+        for r in range(s.nranges):
+            low, _, step, ib, _ = s.dm(r)
+            ranges.append((low, step, s.ndms(r), ib))
+        max_tp0 = max(s.t_processed(0, k) for k in range(s.num_tchunks))
+        return cls(s.dmshifts(), ranges, s.tsamp, s.maxshift, max_tp0, device)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib_or_raise().aa_hip_ddtr_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_kernel(self, kernel: int):
+        check(_lib_or_raise().aa_hip_ddtr_set_kernel(self.handle, int(kernel)), "set_kernel")
+
+    def device_bytes(self) -> int:
+        return _lib_or_raise().aa_hip_ddtr_device_bytes(self.handle)
+
+    def execute_chunk(self, d_input, t_processed: Sequence[int], outs, stream=None):
+        lib = _lib_or_raise()
+        n = len(self.ranges)
+        bits = 8 if d_input.element_size() == 1 else 16
+        tp = (ctypes.c_int * n)(*[int(v) for v in t_processed])
+        ptrs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        pitches = (ctypes.c_size_t * n)(*[o.stride(0) for o in outs])
+        check(lib.aa_hip_ddtr_execute_chunk(self.handle, _ptr(d_input), bits, tp, ptrs, pitches,
+                                            _stream_ptr(stream)), "execute_chunk")
+
+    def ingest(self, d_input, t_processed0: int, stream=None):
+        bits = 8 if d_input.element_size() == 1 else 16
+        check(_lib_or_raise().aa_hip_ddtr_ingest(self.handle, _ptr(d_input), bits, int(t_processed0),
+                                                 _stream_ptr(stream)), "ingest")
+
+    def run_range(self, r: int, t_processed: int, out, stream=None):
+        check(_lib_or_raise().aa_hip_ddtr_run_range(self.handle, int(r), int(t_processed), _ptr(out),
+                                                    out.stride(0), _stream_ptr(stream)), "run_range")
+
+
+def corner_turn_u8(d_in_tc, nsamp: int, nchans: int, d_out, out_pitch: int, stream=None):
+    check(_lib_or_raise().aa_hip_corner_turn_u8(_ptr(d_in_tc), int(nsamp), int(nchans), _ptr(d_out),
+                                                int(out_pitch), _stream_ptr(stream)), "corner_turn_u8")
+
+
+def corner_turn_u16(d_in_tc, nsamp: int, nchans: int, d_out, out_pitch: int, d_err, stream=None):
+    check(_lib_or_raise().aa_hip_corner_turn_u16(_ptr(d_in_tc), int(nsamp), int(nchans), _ptr(d_out),
+                                                 int(out_pitch), _ptr(d_err), _stream_ptr(stream)),
+          "corner_turn_u16")
+
+
+def bin2_u8(d_in, in_pitch: int, nchans: int, n_out: int, d_out, out_pitch: int, stream=None):
+    check(_lib_or_raise().aa_hip_bin2_u8(_ptr(d_in), int(in_pitch), int(nchans), int(n_out), _ptr(d_out),
+                                         int(out_pitch), _stream_ptr(stream)), "bin2_u8")
+
+
+def dedisperse_u8(d_in, pitch: int, row_len: int, dmshifts: np.ndarray, d_dmshifts, ndms: int,
+                  t_processed: int, mstartdm: float, mdmstep: float, inBin: int, d_out, out_pitch: int,
+                  kernel: int = 0, stream=None):
+    h = np.ascontiguousarray(dmshifts, dtype=np.float32)
+    check(_lib_or_raise().aa_hip_dedisperse_u8(_ptr(d_in), int(pitch), int(row_len), int(h.size),
+                                               h.ctypes.data_as(PFLOAT), _ptr(d_dmshifts), int(ndms),
+                                               int(t_processed), float(mstartdm), float(mdmstep), int(inBin),
+                                               _ptr(d_out), int(out_pitch), int(kernel), _stream_ptr(stream)),
+          "dedisperse_u8")
+
+
+def synth_noise_u8(seed: int, d_out, mean: float = 128.0, sigma: float = 16.0, stream=None):
+    check(_lib_or_raise().aa_hip_synth_noise_u8(int(seed) & 0xFFFFFFFFFFFFFFFF, int(d_out.numel()), float(mean),
+                                                float(sigma), _ptr(d_out), _stream_ptr(stream)), "synth_noise_u8")
+
+
+def inject_pulse_u8(d_data_tc, nsamp: int, nchans: int, d_dmshifts, dm: float, tsamp: float, t0: int,
+                    width: int, amplitude: int, stream=None):
+    check(_lib_or_raise().aa_hip_inject_pulse_u8(_ptr(d_data_tc), int(nsamp), int(nchans), _ptr(d_dmshifts),
+                                                 float(dm), float(tsamp), int(t0), int(width), int(amplitude),
+                                                 _stream_ptr(stream)), "inject_pulse_u8")
+
+
+KERNEL_AUTO, KERNEL_DIRECT, KERNEL_LDS = 0, 1, 2

@@ -1,0 +1,277 @@
+/*
+ * C ABI of the MI355X (gfx950) dedispersion library libaa_ddtr_hip.so.
+ *
+ * Two layers, both plain C (no HIP, torch or C++ types in any signature):
+ *
+ *  1. aa_py_* -- the entry points the reference's Python FFI binds with
+ *     ctypes (cmake/py_astro_accelerate.py.in), restricted to the
+ *     dedispersion path.  Same names, argument meaning and return
+ *     conventions; each declaration cites the reference definition it
+ *     replaces.  Host buffers in, host float*** out.
+ *
+ *  2. aa_hip_* -- device-level entry points of the hot path: the pieces
+ *     the reference's runner calls on device memory (corner turn, binning,
+ *     dedisperse, chunk execution), re-cut as explicit-stream calls on
+ *     caller-owned device buffers with no process-global state.  Streams are
+ *     passed as `void*` holding a hipStream_t (NULL = legacy default stream).
+ *
+ * Return convention of the aa_hip_* layer: 0 (AA_HIP_SUCCESS) on success,
+ * a negative aa_hip_status on error; aa_hip_last_error() returns a message
+ * for the calling thread's most recent failure.
+ */
+#ifndef AA_DDTR_HIP_H
+#define AA_DDTR_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+typedef enum {
+    AA_HIP_SUCCESS = 0,
+    AA_HIP_ERROR_INVALID_ARGUMENT = -1,
+    AA_HIP_ERROR_DEVICE = -2,          /* a HIP runtime call failed */
+    AA_HIP_ERROR_OUT_OF_MEMORY = -3,
+    AA_HIP_ERROR_UNSUPPORTED = -4,     /* geometry/option outside this library's scope */
+    AA_HIP_ERROR_INPUT_RANGE = -5,     /* a u16 input sample exceeded 255 */
+    AA_HIP_ERROR_NO_DEVICE = -6
+} aa_hip_status;
+
+/* Kernel variants for aa_hip_dedisperse_u8 / aa_hip_ddtr_set_kernel. */
+typedef enum {
+    AA_HIP_KERNEL_AUTO = 0,     /* fastest valid variant (autotuned tiles) */
+    AA_HIP_KERNEL_DIRECT = 1,   /* one thread per (DM, sample), global gathers */
+    AA_HIP_KERNEL_LDS = 2       /* LDS-staged windows, packed integer sums */
+} aa_hip_kernel;
+
+/* Message for the calling thread's most recent aa_hip_* failure ("" if none). */
+const char *aa_hip_last_error(void);
+/* Library version string. */
+const char *aa_hip_version(void);
+
+/* ======================================================================
+ * Layer 1: reference FFI (aa_py_*), dedispersion path only.
+ * ====================================================================== */
+
+/* Replaces src/aa_py_ddtr_plan.cpp:9-11 (aa_py_ddtr_plan). */
+void *aa_py_ddtr_plan(void);
+/* Replaces src/aa_py_ddtr_plan.cpp:13-15. */
+void aa_py_ddtr_plan_delete(void const *obj);
+/* Replaces src/aa_py_ddtr_plan.cpp:17-24. */
+bool aa_py_ddtr_plan_add_dm(void *obj, float low, float high, float step, int inBin, int outBin);
+/* Replaces src/aa_py_ddtr_plan.cpp:26-28. */
+size_t aa_py_ddtr_plan_range(void const *obj);
+/* Replaces src/aa_py_ddtr_plan.cpp:30-32. */
+bool aa_py_ddtr_plan_set_power(void *obj, float power);
+/* Replaces src/aa_py_ddtr_plan.cpp:34-36. */
+float aa_py_ddtr_plan_power(void const *obj);
+/* Replaces src/aa_py_ddtr_plan.cpp:38-40. */
+bool aa_py_ddtr_plan_set_enable_msd_baseline(void *obj, bool flag);
+/* Replaces src/aa_py_ddtr_plan.cpp:42-44. */
+bool aa_py_ddtr_plan_enable_msd_baseline_noise(void const *obj);
+/* Replaces src/aa_py_ddtr_plan.cpp:46-48. */
+bool aa_py_ddtr_plan_bind_bandpass_normalization(void *obj, float const *data, int data_size);
+
+/* Same field order as the reference's aa_py_filterbank_metadata_struct
+ * (include/aa_py_filterbank_metadata.hpp:15-25). */
+typedef struct {
+    double m_tstart;
+    double m_tsamp;
+    double m_fch1;
+    double m_foff;
+    int m_nbits;
+    int m_nsamples;
+    int m_nchans;
+    double m_src_raj;
+} aa_py_filterbank_metadata_struct;
+
+/* Replaces src/aa_py_filterbank_metadata.cpp:8-27. */
+aa_py_filterbank_metadata_struct *aa_py_filterbank_metadata(double tstart, double tsamp, int nbits,
+                                                            int nsamples, double fch1, double foff,
+                                                            int nchans, double src_raj);
+/* Replaces src/aa_py_filterbank_metadata.cpp:29-31. */
+void aa_py_filterbank_metadata_delete(aa_py_filterbank_metadata_struct const *obj);
+/* Replaces src/aa_py_filterbank_metadata.cpp:33-35. */
+double aa_py_filterbank_metadata_tstart(aa_py_filterbank_metadata_struct const *obj);
+
+/* Same layout as include/aa_py_pipeline_api.hpp:20-25. */
+typedef struct {
+    bool dedispersion;
+    bool analysis;
+    bool periodicity;
+    bool fdas;
+} pipeline_components_struct;
+
+/* Same layout as include/aa_py_pipeline_api.hpp:32-48.  Only
+ * copy_ddtr_data_to_host is honoured; every other option makes bind/ready
+ * fail with AA_HIP_ERROR_UNSUPPORTED recorded in aa_hip_last_error(). */
+typedef struct {
+    bool zero_dm;
+    bool zero_dm_with_outliers;
+    bool input_DDTR_normalization;
+    bool output_DDTR_normalization;
+    bool set_bandpass_average;
+    bool old_rfi;
+    bool copy_ddtr_data_to_host;
+    bool msd_baseline_noise;
+    bool output_dmt;
+    bool output_ffdot_plan;
+    bool output_fdas_list;
+    bool candidate_algorithm;
+    bool fdas_custom_fft;
+    bool fdas_inbin;
+    bool fdas_norm;
+} pipeline_component_option_struct;
+
+/* Replaces src/aa_py_pipeline_api.cpp:9-44: a pipeline over a borrowed host
+ * buffer of u16 samples in [t][c] order (8-bit values, as the reference's
+ * SIGPROC reader widens them). */
+void *aa_py_pipeline_api(pipeline_components_struct pipeline,
+                         pipeline_component_option_struct options,
+                         aa_py_filterbank_metadata_struct metadata,
+                         unsigned short const *input_data, int card_number);
+/* No reference counterpart: the same pipeline over u8 [t][c] host samples,
+ * which halves host memory and PCIe traffic. */
+void *aa_py_pipeline_api_u8(pipeline_components_struct pipeline,
+                            pipeline_component_option_struct options,
+                            aa_py_filterbank_metadata_struct metadata,
+                            unsigned char const *input_data, int card_number);
+/* Replaces src/aa_py_pipeline_api.cpp:46-48. */
+void aa_py_pipeline_api_delete(void const *obj);
+/* Replaces src/aa_py_pipeline_api.cpp:50-52. */
+bool aa_py_pipeline_api_bind_ddtr_plan(void *obj, void const *plan);
+/* Replaces src/aa_py_pipeline_api.cpp:54-56 (returns a new strategy copy,
+ * released with aa_py_ddtr_strategy_delete). */
+void *aa_py_pipeline_api_ddtr_strategy(void *obj);
+/* Replaces src/aa_py_ddtr_strategy.cpp:6-8. */
+void aa_py_ddtr_strategy_delete(void const *obj);
+/* Replaces src/aa_py_pipeline_api.cpp:58-60. */
+size_t aa_py_get_ddtr_nRanges(void *obj);
+/* Replaces src/aa_py_pipeline_api.cpp:62-64. */
+const int *aa_py_get_ndms_array(void *obj);
+/* Replaces src/aa_py_pipeline_api.cpp:66-68. */
+int aa_py_get_t_proc(void *obj, int i, int j);
+/* Replaces src/aa_py_pipeline_api.cpp:70-72 (returns int, truncating a
+ * fractional range start exactly as the reference does). */
+int aa_py_dm_low(void *obj, int range);
+/* Replaces src/aa_py_pipeline_api.cpp:74-76. */
+int aa_py_total_computed_samples(void *obj);
+/* Replaces src/aa_py_pipeline_api.cpp:92-103; the reference's `int&` is a
+ * pointer at the ABI level. */
+bool aa_py_pipeline_api_run(void *obj, int *status_code_int);
+/* Replaces src/aa_py_pipeline_api.cpp:104-106: float***[range][dm][t]. */
+float ***aa_py_buffer(void *obj);
+/* Replace src/aa_py_pipeline_api.cpp:140-151. */
+int aa_py_current_range(void *obj);
+int aa_py_current_time_chunk(void *obj);
+long aa_py_current_inc(void *obj);
+/* Replaces src/aa_py_pipeline_api.cpp:152-154. */
+bool aa_py_cleanup(void *obj);
+
+/* No reference counterpart (the reference builds strategies only inside a
+ * pipeline): strategy of a plan for explicit metadata and free device
+ * memory, computed on the host only.  Accessors mirror
+ * include/aa_ddtr_strategy.hpp:55-151. */
+void *aa_py_ddtr_strategy_create(void const *plan, aa_py_filterbank_metadata_struct metadata,
+                                 size_t free_memory, bool enable_analysis);
+bool aa_py_ddtr_strategy_ready(void const *obj);
+int aa_py_ddtr_strategy_maxshift(void const *obj);
+int aa_py_ddtr_strategy_num_tchunks(void const *obj);
+int aa_py_ddtr_strategy_max_ndms(void const *obj);
+size_t aa_py_ddtr_strategy_nranges(void const *obj);
+int aa_py_ddtr_strategy_ndms(void const *obj, int range);
+int aa_py_ddtr_strategy_t_processed(void const *obj, int range, int chunk);
+/* Strategy-redefined range: low/high/step (floats) and inBin/outBin. */
+bool aa_py_ddtr_strategy_dm(void const *obj, int range, float *low, float *high, float *step,
+                            int *inBin, int *outBin);
+/* Copies nchans dmshift values into out. */
+bool aa_py_ddtr_strategy_dmshifts(void const *obj, float *out, int nchans);
+
+/* ======================================================================
+ * Layer 2: device-level hot path (aa_hip_*).
+ * ====================================================================== */
+
+typedef struct aa_hip_ddtr aa_hip_ddtr;
+
+/* Creates a dedispersion plan bound to `device`.  Replaces the state that
+ * src/aa_device_load_data.cu:39-61 stores in __constant__ memory
+ * (dm_shifts, i_nchans, i_nsamp, i_t_processed_s), made per handle.
+ * dm_low/dm_step are the strategy-redefined ranges (aa_ddtr_strategy::dm),
+ * tsamp is the metadata tsamp (narrowed to float as the reference does),
+ * maxshift the strategy maxshift, max_t_processed0 the largest
+ * t_processed[0][chunk] this handle will execute. */
+int aa_hip_ddtr_create(int device, int nchans, const float *dmshifts, int nranges,
+                       const float *dm_low, const float *dm_step, const int *ndms,
+                       const int *inBin, double tsamp, int maxshift, int max_t_processed0,
+                       aa_hip_ddtr **out);
+int aa_hip_ddtr_destroy(aa_hip_ddtr *h);
+/* Selects the dedispersion kernel variant (aa_hip_kernel) for this handle. */
+int aa_hip_ddtr_set_kernel(aa_hip_ddtr *h, int kernel);
+/* Device bytes held by the handle (dmshift table, cascade levels). */
+size_t aa_hip_ddtr_device_bytes(const aa_hip_ddtr *h);
+
+/* One time chunk, end to end on the device: corner turn of
+ * t_processed[0] + maxshift rows of d_input_tc (u8 when input_bits == 8,
+ * u16 carrying 8-bit values when input_bits == 16), the x2 binning cascade,
+ * and the dedispersion of every range into d_out[r] (ndms_r rows of
+ * t_processed[r] floats, row pitch out_pitch[r] floats).  Replaces the
+ * per-chunk DDTR block of include/aa_permitted_pipelines_generic.hpp:687-840
+ * (load_chunk_data is the caller's H2D; D2H is the caller's). */
+int aa_hip_ddtr_execute_chunk(aa_hip_ddtr *h, const void *d_input_tc, int input_bits,
+                              const int *t_processed, float *const *d_out,
+                              const size_t *out_pitch, void *stream);
+/* The same chunk split in two steps, for callers that interleave work:
+ * ingest = corner turn into the handle's level-0 buffer; run_range =
+ * binning (when the range's bin exceeds the resident level) + one range. */
+int aa_hip_ddtr_ingest(aa_hip_ddtr *h, const void *d_input_tc, int input_bits, int t_processed0,
+                       void *stream);
+int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_out,
+                          size_t out_pitch, void *stream);
+
+/* Replaces corner_turn_SM_inplace (src/aa_corner_turn.cpp:118-121) for the
+ * 8-bit path: [t][c] -> [c][t] into a separate u8 buffer with row pitch
+ * out_pitch bytes (>= nsamp). */
+int aa_hip_corner_turn_u8(const uint8_t *d_in_tc, int nsamp, int nchans, uint8_t *d_out_ct,
+                          size_t out_pitch, void *stream);
+/* As above for u16 input holding 8-bit samples.  A sample above 255 sets
+ * *d_err (device int) to 1; the samples are narrowed regardless. */
+int aa_hip_corner_turn_u16(const uint16_t *d_in_tc, int nsamp, int nchans, uint8_t *d_out_ct,
+                           size_t out_pitch, int *d_err, void *stream);
+/* Replaces bin_gpu (src/aa_bin_gpu.cpp:8-44) for the 8-bit path, one x2
+ * level: out[c][t] = floor((in[c][2t] + in[c][2t+1]) / 2), t < n_out. */
+int aa_hip_bin2_u8(const uint8_t *d_in, size_t in_pitch, int nchans, int n_out, uint8_t *d_out,
+                   size_t out_pitch, void *stream);
+/* Replaces dedisperse() (src/aa_dedisperse.cpp:5-163) + the failsafe kernel
+ * (src/aa_device_dedispersion_kernel.cu:422-450) for one range on u8 rows:
+ * out[d*out_pitch + t] = ((float)sum_c L[c*pitch + t + delay(c,d)] / nchans) / inBin,
+ * delay(c,d) = trunc(dmshift[c] * fma(d, mdmstep, mstartdm)).  The dmshift
+ * table is passed twice: h_dmshifts (host) is used to check, before the
+ * launch, that every delay lies in [0, row_len - t_processed]; d_dmshifts
+ * (device) is what the kernel reads.  row_len is the number of valid
+ * samples per row (<= pitch); rows must also hold at least 4096 readable
+ * bytes past row_len (the LDS kernel reads whole windows). */
+int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nchans,
+                         const float *h_dmshifts, const float *d_dmshifts, int ndms, int t_processed,
+                         float mstartdm, float mdmstep, int inBin, float *d_out, size_t out_pitch,
+                         int kernel, void *stream);
+
+/* Synthetic input on the device (no reference counterpart; benchmarks and
+ * tests).  Noise: N(mean, sigma^2) rounded to nearest and clipped to
+ * [0,255], a SplitMix64 counter stream keyed by `seed` over the flat [t][c]
+ * index (Box-Muller in double). */
+int aa_hip_synth_noise_u8(uint64_t seed, int64_t n, double mean, double sigma, uint8_t *d_out_tc,
+                          void *stream);
+/* Adds `amplitude` (saturating at 255) to `width` samples starting at
+ * t0 + floor(dmshift[c] * (float)(dm / tsamp)) of each channel c
+ * (the injector convention of src/aa_host_fake_signal_generator.cpp:75-78). */
+int aa_hip_inject_pulse_u8(uint8_t *d_data_tc, int nsamp, int nchans, const float *d_dmshifts,
+                           double dm, double tsamp, int t0, int width, int amplitude, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AA_DDTR_HIP_H */

@@ -1,0 +1,259 @@
+"""GPU parity: every HIP path bit-exact against the CPU oracle.
+
+All tests call the library through its C ABI (ctypes), on torch-allocated
+device memory.  Sizes are chosen so the oracle finishes in seconds; the
+full-size config-1 plane is compared whole.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = [1, 2]  # AA_HIP_KERNEL_DIRECT, AA_HIP_KERNEL_LDS
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(x, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+@pytest.mark.parametrize("nsamp,nchans", [(1000, 77), (4109, 130), (64, 64), (5, 3)])
+def test_corner_turn_u8(aa, oracle, cuda, nsamp, nchans):
+    import torch
+    say(f"corner_turn_u8 nsamp={nsamp} nchans={nchans}")
+    rng = np.random.default_rng(nsamp * 7 + nchans)
+    x = rng.integers(0, 256, size=(nsamp, nchans), dtype=np.uint8)
+    pitch = nsamp + 37
+    out = torch.full((nchans, pitch), 0xAB, dtype=torch.uint8, device=cuda)
+    aa.corner_turn_u8(to_dev(x, cuda), nsamp, nchans, out, pitch)
+    got = out.cpu().numpy()
+    np.testing.assert_array_equal(got[:, :nsamp], oracle.corner_turn(x, nsamp, nchans))
+    assert np.all(got[:, nsamp:] == 0xAB)  # nothing written past the row
+
+
+def test_corner_turn_u16_and_flag(aa, oracle, cuda):
+    import torch
+    say("corner_turn_u16")
+    rng = np.random.default_rng(4)
+    x = rng.integers(0, 256, size=(3000, 100), dtype=np.uint16)
+    out = torch.zeros((100, 3000), dtype=torch.uint8, device=cuda)
+    err = torch.zeros(1, dtype=torch.int32, device=cuda)
+    xd = torch.from_numpy(x.view(np.int16)).to(cuda)
+    aa.corner_turn_u16(xd, 3000, 100, out, 3000, err)
+    np.testing.assert_array_equal(out.cpu().numpy(), oracle.corner_turn(x, 3000, 100))
+    assert int(err.item()) == 0
+    x[1234, 56] = 300
+    xd = torch.from_numpy(x.view(np.int16)).to(cuda)
+    aa.corner_turn_u16(xd, 3000, 100, out, 3000, err)
+    assert int(err.item()) == 1
+
+
+@pytest.mark.parametrize("n_out,in_pitch,out_pitch", [(1000, 2048, 1024), (1001, 2003, 1001), (3, 8, 4)])
+def test_bin2(aa, oracle, cuda, n_out, in_pitch, out_pitch):
+    import torch
+    say(f"bin2 n_out={n_out}")
+    rng = np.random.default_rng(n_out)
+    L = rng.integers(0, 256, size=(33, in_pitch), dtype=np.uint8)
+    out = torch.zeros((33, out_pitch), dtype=torch.uint8, device=cuda)
+    aa.bin2_u8(to_dev(L, cuda), in_pitch, 33, n_out, out, out_pitch)
+    np.testing.assert_array_equal(out.cpu().numpy()[:, :n_out], oracle.bin2(L[:, :2 * n_out], n_out))
+
+
+GEOMS = [
+    # nchans, fch1, foff, tsamp, dm_low, dm_step, ndms, tp, inBin
+    (64, 1400.0, -4.0, 64e-6, 0.0, 1.0, 64, 3000, 1),
+    (200, 1550.0, -1.5, 64e-6, 153.6, 0.2, 128, 2240, 1),
+    (512, 1550.0, -0.5859, 64e-6, 0.0, 0.5, 192, 4000, 1),
+    (96, 900.0, -2.0, 1e-4, 300.0, 2.0, 70, 1001, 4),
+]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("geom", GEOMS, ids=[f"{g[0]}ch_{g[6]}dm_{g[7]}t" for g in GEOMS])
+def test_dedisperse_range(aa, oracle, cuda, geom, kernel):
+    import torch
+    nchans, fch1, foff, tsamp, low, step, ndms, tp, ib = geom
+    say(f"dedisperse kernel={kernel} geom={geom}")
+    ds = oracle.dmshifts(nchans, fch1, foff)
+    tsr = np.float32(np.float32(tsamp) * np.float32(ib))
+    ms, st = np.float32(np.float32(low) / tsr), np.float32(np.float32(step) / tsr)
+    maxd = int(np.trunc(np.float32(ds[-1]) * np.float32(np.float64(ndms - 1) * st + ms))) + 1
+    row_len = tp + maxd
+    pitch = ((row_len + 4096 + 255) // 256) * 256
+    rng = np.random.default_rng(nchans + ndms)
+    L = np.zeros((nchans, pitch), dtype=np.uint8)
+    L[:, :row_len] = rng.integers(0, 256, size=(nchans, row_len), dtype=np.uint8)
+    want = oracle.dedisperse(L[:, :row_len], ds, ndms, tp, float(ms), float(st), ib)
+    d_ds = to_dev(ds, cuda)
+    out_pitch = tp + 13
+    out = torch.full((ndms, out_pitch), -1.0, dtype=torch.float32, device=cuda)
+    aa.dedisperse_u8(to_dev(L, cuda), pitch, row_len, ds, d_ds, ndms, tp, float(ms), float(st), ib, out,
+                     out_pitch, kernel=kernel)
+    got = out.cpu().numpy()
+    np.testing.assert_array_equal(got[:, :tp], want)
+    assert np.all(got[:, tp:] == -1.0)
+
+
+def test_dedisperse_rejects_short_rows(aa, oracle, cuda):
+    import torch
+    say("dedisperse short rows rejected on the host")
+    ds = oracle.dmshifts(64, 1400.0, -4.0)
+    L = torch.zeros((64, 4096), dtype=torch.uint8, device=cuda)
+    out = torch.zeros((64, 2000), dtype=torch.float32, device=cuda)
+    with pytest.raises(aa.AAHipError):
+        aa.dedisperse_u8(L, 4096, 2001, ds, to_dev(ds, cuda), 64, 2000, 0.0, 500.0, 1, out, 2000)
+
+
+def _chunked_reference(oracle, ranges, nchans, nsamp, fch1, foff, tsamp, free, seed, pulses):
+    s = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp, free_memory=free)
+    assert s.ok
+    data = oracle.gen_noise(seed, nsamp, nchans)
+    for dm, t0, w in pulses:
+        oracle.inject_pulse(data, s.dmshifts, dm, tsamp, t0, w, 32)
+    return s, data, oracle.run(s, data)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_execute_chunk_config1_full(aa, oracle, cuda, kernel):
+    """512 ch x 131072 samples, DM 0-500/0.5: the whole plane, bit-exact."""
+    import torch
+    say(f"config-1 full plane kernel={kernel}")
+    tsamp = 64e-6
+    s, data, want = _chunked_reference(oracle, [(0, 500, 0.5, 1, 1)], 512, 131072, 1550.0, -0.5859, tsamp,
+                                       280 * 2**30, 1, [(250.0, 60000, 8)])
+    plan = aa.DdtrPlan(s.dmshifts, [(s.dm[0][0], s.dm[0][2], s.ndms[0], 1)], tsamp, s.maxshift, s.t_processed[0][0])
+    plan.set_kernel(kernel)
+    out = torch.empty((s.ndms[0], s.t_processed[0][0]), dtype=torch.float32, device=cuda)
+    plan.execute_chunk(to_dev(data, cuda), [s.t_processed[0][0]], [out])
+    got = out.cpu().numpy()
+    np.testing.assert_array_equal(got, want[0])
+    # the injected pulse (DM 250 = trial 500) peaks at its arrival sample
+    d, t = np.unravel_index(np.argmax(got), got.shape)
+    assert d == 500 and 60000 <= t < 60008
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_execute_chunks_multi_range_multi_chunk(aa, oracle, cuda, kernel):
+    """Several chunks (small free memory), 4 ranges with x2/x4/x8 binning."""
+    import torch
+    say(f"multi-chunk multi-range kernel={kernel}")
+    nchans, nsamp, tsamp = 96, 200000, 1.28e-4
+    ranges = [(0, 40, 0.5, 1, 1), (40, 80, 1.0, 2, 2), (80, 160, 2.0, 4, 4), (160, 400, 4.0, 8, 8)]
+    free = None
+    for extra in range(200_000_000, 1_000_000, -250_000):  # largest budget that gives >= 3 chunks
+        s0 = oracle.strategy(ranges, nchans, nsamp, 1200.0, -2.5, tsamp, free_memory=104857600 + extra)
+        if s0.ok and s0.num_tchunks >= 3:
+            free = 104857600 + extra
+            break
+    assert free is not None
+    s, data, want = _chunked_reference(oracle, ranges, nchans, nsamp, 1200.0, -2.5, tsamp, free, 3,
+                                       [(30.0, 50000, 4), (120.0, 90000, 16), (300.0, 20000, 40)])
+    assert s.num_tchunks >= 3
+    plan = aa.DdtrPlan(s.dmshifts, [(s.dm[r][0], s.dm[r][2], s.ndms[r], s.dm[r][3]) for r in range(s.nranges)],
+                       tsamp, s.maxshift, max(s.t_processed[0]))
+    plan.set_kernel(kernel)
+    d_all = to_dev(data, cuda)
+    inc = 0
+    for k in range(s.num_tchunks):
+        tps = [s.t_processed[r][k] for r in range(s.nranges)]
+        outs = [torch.empty((s.ndms[r], tps[r]), dtype=torch.float32, device=cuda) for r in range(s.nranges)]
+        plan.execute_chunk(d_all[inc:], tps, outs)
+        for r in range(s.nranges):
+            col = inc // s.dm[r][3]
+            np.testing.assert_array_equal(outs[r].cpu().numpy(), want[r][:, col:col + tps[r]])
+        inc += tps[0]
+
+
+def _metadata(aa, nchans, nsamp, fch1, foff, tsamp):
+    return aa.make_metadata(nchans, nsamp, fch1, foff, tsamp)
+
+
+def test_ffi_pipeline_u16_matches_oracle(aa, oracle, cuda):
+    """The reference FFI path: plan -> pipeline -> run() loop -> float*** buffer."""
+    say("FFI pipeline u16")
+    nchans, nsamp, tsamp, fch1, foff = 128, 120000, 64e-6, 1500.0, -2.0
+    ranges = [(0, 100, 0.5, 1, 1), (100, 300, 1.0, 2, 2)]
+    s = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp)
+    data = oracle.gen_noise(77, nsamp, nchans)
+    oracle.inject_pulse(data, s.dmshifts, 170.0, tsamp, 30000, 12, 40)
+    want = oracle.run(s, data)
+    data16 = data.astype(np.uint16)
+
+    comps = aa.aa_py_pipeline_components()
+    comps.dedispersion = True
+    opts = aa.aa_py_pipeline_component_options()
+    opts.copy_ddtr_data_to_host = True
+    plan = aa.make_plan(ranges)
+    pipe = aa.aa_py_pipeline(comps, opts, _metadata(aa, nchans, nsamp, fch1, foff, tsamp), data16, 0)
+    assert pipe.bind_ddtr_plan(plan)
+    statuses = []
+    while pipe.run():
+        statuses.append(pipe.status_code())
+    statuses.append(pipe.status_code())
+    assert statuses[-2:] == [2, 0] and all(x == 1 for x in statuses[:-2])
+    strat = aa.aa_py_ddtr_strategy(plan, _metadata(aa, nchans, nsamp, fch1, foff, tsamp), 280 * 2**30)
+    planes = pipe.output_planes(strat)
+    for r in range(s.nranges):
+        np.testing.assert_array_equal(planes[r], want[r])
+    assert pipe.ddtr_ndms() == s.ndms
+    pipe.cleanUp()
+
+
+def test_ffi_pipeline_rejects_out_of_range_u16(aa, cuda):
+    say("FFI pipeline rejects samples > 255")
+    nchans, nsamp, tsamp = 64, 40000, 64e-6
+    data = np.full((nsamp, nchans), 100, dtype=np.uint16)
+    data[5, 5] = 4000
+    comps = aa.aa_py_pipeline_components()
+    comps.dedispersion = True
+    opts = aa.aa_py_pipeline_component_options()
+    pipe = aa.aa_py_pipeline(comps, opts, _metadata(aa, nchans, nsamp, 1400.0, -4.0, tsamp), data, 0)
+    assert pipe.bind_ddtr_plan(aa.make_plan([(0, 50, 1, 1, 1)]))
+    assert not pipe.run()
+    assert pipe.status_code() == -1
+
+
+def test_ffi_pipeline_rejects_unsupported_option(aa, cuda):
+    say("FFI pipeline rejects zero_dm")
+    data = np.zeros((40000, 64), dtype=np.uint16)
+    comps = aa.aa_py_pipeline_components()
+    comps.dedispersion = True
+    opts = aa.aa_py_pipeline_component_options()
+    opts.zero_dm = True
+    pipe = aa.aa_py_pipeline(comps, opts, _metadata(aa, 64, 40000, 1400.0, -4.0, 64e-6), data, 0)
+    assert pipe.bind_ddtr_plan(aa.make_plan([(0, 50, 1, 1, 1)]))
+    assert not pipe.run()
+
+
+def test_gpu_known_answer_255(aa, oracle, cuda):
+    """The reference's fake-signal KAT through the FFI pipeline on the GPU."""
+    say("GPU KAT 255")
+    tsamp, nchans, fch1 = 6.4e-5, 128, 1550.0
+    foff = -300.0 / nchans
+    nsamples = int(2 / tsamp)
+    s = oracle.strategy([(0, 250.0, 1, 1, 1)], nchans, nsamples, fch1, foff, tsamp)
+    width = int(1 / (tsamp * 25))
+    rng = np.random.default_rng(4242)
+    for _ in range(3):
+        dm_pos = float(rng.integers(0, 200))
+        start = int(rng.integers(1, 10)) * 1000 + int(rng.integers(0, 10)) * 100 + int(rng.integers(0, 10)) * 10
+        sig = oracle.aa_fake_signal(s.dmshifts, nsamples, tsamp, s.maxshift, dm_pos, start, width, 0.5)
+        comps = aa.aa_py_pipeline_components()
+        comps.dedispersion = True
+        opts = aa.aa_py_pipeline_component_options()
+        opts.copy_ddtr_data_to_host = True
+        plan = aa.make_plan([(0, 250.0, 1, 1, 1)])
+        pipe = aa.aa_py_pipeline(comps, opts, _metadata(aa, nchans, nsamples, fch1, foff, tsamp), sig, 0)
+        assert pipe.bind_ddtr_plan(plan)
+        while pipe.run():
+            pass
+        _, buf = pipe.get_buffer()
+        assert buf[0][int(dm_pos)][start] == 255.0, (dm_pos, start)
+        pipe.cleanUp()
