@@ -107,12 +107,17 @@ public:
     }
 
     bool run(aa_pipeline_runner::status &status) {
-        if (!m_ready || !setup()) {
-            LOG(log_level::error, "Pipeline could not start/resume because either pipeline is not ready or runner is not setup.");
+        if (!m_ready) {
+            LOG(log_level::error, "Pipeline could not start/resume because the pipeline is not ready.");
             status = aa_pipeline_runner::status::finished;
             return false;
         }
         const int nranges = (int)m_strategy.get_nRanges();
+        if (m_chunk < m_strategy.num_tchunks() && !setup()) {
+            LOG(log_level::error, "Pipeline could not start/resume because the runner could not be set up.");
+            status = aa_pipeline_runner::status::error;
+            return false;
+        }
         if (m_chunk >= m_strategy.num_tchunks()) {
             if (!m_notified_finish) {
                 release_device();
@@ -212,7 +217,8 @@ private:
             release_device();
             return false;
         }
-        if (m_options.find(aa_pipeline::component_option::copy_ddtr_data_to_host) != m_options.end())
+        if (!m_host_out &&
+            m_options.find(aa_pipeline::component_option::copy_ddtr_data_to_host) != m_options.end())
             allocate_host_output();
         return true;
     }

@@ -33,7 +33,14 @@ _lib = None
 
 
 def load_library(path: Optional[str] = None) -> ctypes.CDLL:
-    """Load (once) and return the HIP dedispersion library; raise if absent."""
+    """Load (once) and return the HIP dedispersion library; raise if absent.
+
+    A process must hold a single HIP runtime.  torch ships its own
+    libamdhip64.so.7; when torch is importable it is imported first, so that
+    the library's libamdhip64.so.7 dependency binds to the runtime torch
+    already mapped (a second runtime in the same process does not see the
+    device).
+    """
     global _lib
     if _lib is not None and path is None:
         return _lib
@@ -41,6 +48,10 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     if not os.path.exists(p):
         raise RuntimeError(
             f"libaa_ddtr_hip.so not found at {p}; build it with `make -C astro-accelerate_amd`")
+    try:
+        import torch  # noqa: F401  (binds the HIP runtime before the library loads)
+    except ImportError:
+        pass
     lib = ctypes.CDLL(p)
     _declare(lib)
     if path is None:

@@ -3,6 +3,11 @@ import sys
 
 import pytest
 
+try:  # one HIP runtime per process: torch binds it before the library loads
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "astro-accelerate_amd"), os.path.join(ROOT, "oracle")):
     if p not in sys.path:
