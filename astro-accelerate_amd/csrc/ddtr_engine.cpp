@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "aa_ddtr_hip.h"
+#include "ddtr_lds.h"
 
 namespace aa_hip {
 
@@ -73,9 +74,29 @@ int check_delays(const float *dmshifts, int nchans, int ndms, float mstartdm, fl
 hipError_t dispatch_dedisperse(int kernel, const uint8_t *L, size_t pitch, int nchans,
                                const float *d_dmshifts, const float *h_dmshifts, int ndms,
                                int t_processed, float mstartdm, float mdmstep, int inBin,
-                               float *out, size_t out_pitch, hipStream_t s) {
-    (void)h_dmshifts;
-    (void)kernel;
+                               float *out, size_t out_pitch, hipStream_t s, int row_quads) {
+    if (kernel != AA_HIP_KERNEL_DIRECT && (pitch % 16) == 0 && ((uintptr_t)L % 16) == 0) {
+        if (row_quads == 0) row_quads = lds_row_quads(h_dmshifts, nchans, ndms, mstartdm, mdmstep);
+        if (row_quads > 0) {
+            lds_params p{};
+            p.L = L;
+            p.pitch = pitch;
+            p.nchans = nchans;
+            p.dmshifts = d_dmshifts;
+            p.ndms = ndms;
+            p.t_processed = t_processed;
+            p.mstartdm = mstartdm;
+            p.mdmstep = mdmstep;
+            p.fnchans = (float)nchans;
+            p.fbin = (float)inBin;
+            p.out = out;
+            p.out_pitch = out_pitch;
+            p.row_quads = row_quads;
+            return launch_dedisperse_lds(p, s);
+        }
+        if (kernel == AA_HIP_KERNEL_LDS) return hipErrorInvalidValue;  // asked for LDS, geometry does not fit
+    }
+    if (kernel == AA_HIP_KERNEL_LDS) return hipErrorInvalidValue;
     return launch_dedisperse_direct_u8(L, pitch, nchans, d_dmshifts, ndms, t_processed, mstartdm,
                                        mdmstep, inBin, out, out_pitch, s);
 }
@@ -237,8 +258,12 @@ int ddtr_engine::run_range(int r, int t_processed, float *d_out, size_t out_pitc
     range_scalars(r, &mstartdm, &mdmstep);
     st = check_delays(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, t_processed, i_nsamp);
     if (st != AA_HIP_SUCCESS) return st;
+    if (m_row_quads.size() != m_ranges.size()) m_row_quads.assign(m_ranges.size(), 0);
+    if (m_row_quads[(size_t)r] == 0)
+        m_row_quads[(size_t)r] = lds_row_quads(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep);
     hipError_t e = dispatch_dedisperse(m_kernel, lv.ptr, lv.pitch, m_nchans, m_d_dmshifts, m_dmshifts.data(),
-                                       rd.ndms, t_processed, mstartdm, mdmstep, rd.inBin, d_out, out_pitch, s);
+                                       rd.ndms, t_processed, mstartdm, mdmstep, rd.inBin, d_out, out_pitch, s,
+                                       m_row_quads[(size_t)r]);
     if (e != hipSuccess) return hip_fail(e, "dedisperse");
     return AA_HIP_SUCCESS;
 }

@@ -94,6 +94,7 @@ private:
     int *m_d_err = nullptr;
     std::vector<level> m_levels;
     int m_resident = -1;  // highest level valid for the current chunk
+    std::vector<int> m_row_quads;  // per range: LDS row width, -1 = does not fit, 0 = not computed
     size_t m_device_bytes = 0;
 };
 
@@ -113,6 +114,6 @@ inline int host_delay(float dmshift, int d, float mstartdm, float mdmstep) {
 hipError_t dispatch_dedisperse(int kernel, const uint8_t *L, size_t pitch, int nchans,
                                const float *d_dmshifts, const float *h_dmshifts, int ndms,
                                int t_processed, float mstartdm, float mdmstep, int inBin,
-                               float *out, size_t out_pitch, hipStream_t s);
+                               float *out, size_t out_pitch, hipStream_t s, int row_quads = 0);
 
 }  // namespace aa_hip

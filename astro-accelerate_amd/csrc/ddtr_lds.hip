@@ -1,0 +1,320 @@
+// LDS-tiled dedispersion kernel for gfx950 (MI355X).
+//
+// Work-group tile: kDms DM trials (one per wave) x kT output samples (64
+// lanes x kR samples each).  Channels are streamed kC at a time: the rows of
+// a channel chunk (u8 samples, the window the tile's DMs need) are copied
+// into LDS with global_load_lds_dwordx4 (straight from global memory to LDS,
+// no registers), double-buffered, so the copy of chunk k+1 is in flight while
+// chunk k is summed.  One barrier per chunk.
+//
+// Inner loop, per wave (one DM), per channel pair: each lane reads its
+// 32-sample run from the two LDS rows with three aligned ds_read_b128 each,
+// then v_perm_b32 with scalar selectors does the byte realignment and the
+// u8 -> 16-bit unpack in one instruction, and v_add3_u32 adds both channels
+// into 16 packed accumulators (two samples per dword, 16-bit fields).  The
+// packed fields are flushed into 32-bit accumulators every 256 channels
+// (256 * 255 < 2^16).  The misalignment of a lane run is the same for all
+// lanes of the wave (lane runs are multiples of 16 samples apart), so it is
+// resolved by a scalar branch over the 16 (dword shift a, dword shift b)
+// combinations; the byte shift inside a dword goes into the selectors.
+//
+// Every sum is an exact integer; the output is ((float)sum / nchans) / inBin
+// as in the reference's failsafe kernel (src/aa_device_dedispersion_kernel.cu:448),
+// with delays trunc(dmshift[c] * fma(d, mdmstep, mstartdm)) (:430,440).
+#include "ddtr_lds.h"
+
+namespace aa_hip {
+
+namespace {
+
+constexpr int kR = 32;              // samples per lane
+constexpr int kT = 64 * kR;         // samples per tile
+constexpr int kDms = 16;            // DM trials per work-group (= waves)
+constexpr int kC = 32;              // channels per LDS chunk
+constexpr int kThreads = 64 * kDms; // 1024
+constexpr int kFlushChunks = 256 / kC;
+
+// Two separate LDS arrays (not one [2][...] array), so that the compiler can
+// tell the buffer being summed from the buffer being filled by the
+// direct-to-LDS copies and does not wait for those copies before each read.
+__shared__ uint4 g_rows0[kC * kLdsRowQuadsMax];
+__shared__ uint4 g_rows1[kC * kLdsRowQuadsMax];
+__shared__ int g_offs0[kDms * kC];
+__shared__ int g_offs1[kDms * kC];
+
+__device__ __forceinline__ int dev_delay(float dmshift, int d, float mstartdm, float mdmstep)
+{
+    const float shift_temp = __fmaf_rn((float)d, mdmstep, mstartdm);
+    return (int)__fmul_rn(dmshift, shift_temp);
+}
+
+// One channel pair: wa / wb are the 12 dwords read from each channel's row
+// (aligned to 16 bytes), QA / QB the dword shift of the run start, sel*0 /
+// sel*1 the byte selectors of even / odd sample pairs.
+template <int QA, int QB>
+__device__ __forceinline__ void accumulate_pair(uint32_t (&acc)[kR / 2], const uint32_t (&wa)[12],
+                                                const uint32_t (&wb)[12], uint32_t sela0, uint32_t sela1,
+                                                uint32_t selb0, uint32_t selb1)
+{
+#pragma unroll
+    for (int j = 0; j < kR / 2; ++j) {
+        const int ja = QA + (j >> 1);
+        const int jb = QB + (j >> 1);
+        const uint32_t xa = __builtin_amdgcn_perm(wa[ja + 1], wa[ja], (j & 1) ? sela1 : sela0);
+        const uint32_t xb = __builtin_amdgcn_perm(wb[jb + 1], wb[jb], (j & 1) ? selb1 : selb0);
+        // one v_add3_u32 (two channels, two samples), written as a single
+        // instruction so the add cannot be split and moved out of the case
+        asm("v_add3_u32 %0, %1, %2, %3" : "=v"(acc[j]) : "v"(acc[j]), "v"(xa), "v"(xb));
+    }
+}
+
+template <int QA>
+__device__ __forceinline__ void accumulate_one(uint32_t (&acc)[kR / 2], const uint32_t (&wa)[12], uint32_t sela0,
+                                               uint32_t sela1)
+{
+#pragma unroll
+    for (int j = 0; j < kR / 2; ++j) {
+        const int ja = QA + (j >> 1);
+        const uint32_t xa = __builtin_amdgcn_perm(wa[ja + 1], wa[ja], (j & 1) ? sela1 : sela0);
+        asm("v_add_u32 %0, %1, %2" : "=v"(acc[j]) : "v"(acc[j]), "v"(xa));
+    }
+}
+
+__device__ __forceinline__ void load_run(const uint4* p, uint32_t (&w)[12])
+{
+    const uint4 v0 = p[0], v1 = p[1], v2 = p[2];
+    w[0] = v0.x; w[1] = v0.y; w[2] = v0.z; w[3] = v0.w;
+    w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
+    w[8] = v2.x; w[9] = v2.y; w[10] = v2.z; w[11] = v2.w;
+}
+
+// Selector for the bytes at offsets r, r+1 of an 8-byte window, placed in
+// bytes 0 and 2 of the result (selector 0x0c yields 0x00 in bytes 1 and 3).
+__device__ __forceinline__ uint32_t pair_selector(int r)
+{
+    return 0x0c000c00u | ((uint32_t)(r + 1) << 16) | (uint32_t)r;
+}
+
+struct tile_ctx {
+    const uint8_t* __restrict__ L;
+    const float* __restrict__ dmshifts;
+    int d0, lane, wave, tid, Wq;
+    long t0;
+};
+
+// Issues the copies of chunk `chunk` into `rows`: wave w copies rows w and
+// w + 16 of the chunk, 64 uint4 per instruction.  The row base comes from
+// a scalar load of the row's dmshift, so no vector load precedes the copies
+// (vmcnt is in order: waiting on such a load would wait on the copies too).
+__device__ __forceinline__ void stage_chunk(const lds_params& p, const tile_ctx& t, int chunk, uint4* rows)
+{
+    const int c0 = chunk * kC;
+    const int nrows = (p.nchans - c0) < kC ? (p.nchans - c0) : kC;
+    for (int row = t.wave; row < nrows; row += kDms) {
+        const int c = c0 + row;
+        const float ds = t.dmshifts[c];
+        const int b16 = __builtin_amdgcn_readfirstlane(dev_delay(ds, t.d0, p.mstartdm, p.mdmstep) & ~15);
+        const uint8_t* src = t.L + (size_t)c * p.pitch + (size_t)t.t0 + (size_t)b16;
+        uint4* dst = rows + row * t.Wq;
+        for (int col = 0; col < t.Wq; col += 64) {
+            if (col + t.lane < t.Wq)
+                __builtin_amdgcn_global_load_lds((const void*)(src + 16 * (size_t)(col + t.lane)),
+                                                 (void __attribute__((address_space(3)))*)(dst + col), 16, 0, 0);
+        }
+    }
+}
+
+// Offset e = delay(c, d0 + wd) - floor16(delay(c, d0)) of (wd, ci) = (tid / kC, tid % kC).
+__device__ __forceinline__ int chunk_offset(const lds_params& p, const tile_ctx& t, int chunk)
+{
+    int e = 0;
+    if (t.tid < kDms * kC) {
+        const int wd = t.tid / kC, ci = t.tid % kC;
+        const int c = chunk * kC + ci;
+        const int dd = t.d0 + wd;
+        if (c < p.nchans && dd < p.ndms) {
+            const float ds = t.dmshifts[c];
+            e = dev_delay(ds, dd, p.mstartdm, p.mdmstep) - (dev_delay(ds, t.d0, p.mstartdm, p.mdmstep) & ~15);
+        }
+    }
+    return e;
+}
+
+// Sums one staged chunk into the wave's packed accumulators.
+__device__ __forceinline__ void sum_chunk(const lds_params& p, const tile_ctx& t, int chunk, const uint4* rows,
+                                          const int* offs, uint32_t (&acc16)[kR / 2])
+{
+    const int wave_off = t.wave * kC;
+    const uint4* base = rows + 2 * t.lane;
+    const int nrows = (p.nchans - chunk * kC) < kC ? (p.nchans - chunk * kC) : kC;
+#pragma unroll 1
+    for (int ci = 0; ci + 1 < nrows; ci += 2) {
+        const int ea = __builtin_amdgcn_readfirstlane(offs[wave_off + ci]);
+        const int eb = __builtin_amdgcn_readfirstlane(offs[wave_off + ci + 1]);
+        uint32_t wa[12], wb[12];
+        load_run(base + ci * t.Wq + (ea >> 4), wa);
+        load_run(base + (ci + 1) * t.Wq + (eb >> 4), wb);
+        const uint32_t sa0 = pair_selector(ea & 3), sa1 = sa0 + 0x00020002u;
+        const uint32_t sb0 = pair_selector(eb & 3), sb1 = sb0 + 0x00020002u;
+        switch (((ea >> 2) & 3) * 4 + ((eb >> 2) & 3)) {
+            case 0: accumulate_pair<0, 0>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 1: accumulate_pair<0, 1>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 2: accumulate_pair<0, 2>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 3: accumulate_pair<0, 3>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 4: accumulate_pair<1, 0>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 5: accumulate_pair<1, 1>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 6: accumulate_pair<1, 2>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 7: accumulate_pair<1, 3>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 8: accumulate_pair<2, 0>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 9: accumulate_pair<2, 1>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 10: accumulate_pair<2, 2>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 11: accumulate_pair<2, 3>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 12: accumulate_pair<3, 0>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 13: accumulate_pair<3, 1>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            case 14: accumulate_pair<3, 2>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+            default: accumulate_pair<3, 3>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
+        }
+    }
+    if (nrows & 1) {  // odd channel count in the last chunk: its last channel alone
+        const int ea = __builtin_amdgcn_readfirstlane(offs[wave_off + nrows - 1]);
+        uint32_t wa[12];
+        load_run(base + (nrows - 1) * t.Wq + (ea >> 4), wa);
+        const uint32_t sa0 = pair_selector(ea & 3), sa1 = sa0 + 0x00020002u;
+        switch ((ea >> 2) & 3) {
+            case 0: accumulate_one<0>(acc16, wa, sa0, sa1); break;
+            case 1: accumulate_one<1>(acc16, wa, sa0, sa1); break;
+            case 2: accumulate_one<2>(acc16, wa, sa0, sa1); break;
+            default: accumulate_one<3>(acc16, wa, sa0, sa1); break;
+        }
+    }
+}
+
+__device__ __forceinline__ void flush(uint32_t (&acc16)[kR / 2], uint32_t (&acc32)[kR])
+{
+#pragma unroll
+    for (int j = 0; j < kR / 2; ++j) {
+        acc32[2 * j] += acc16[j] & 0xFFFFu;
+        acc32[2 * j + 1] += acc16[j] >> 16;
+        acc16[j] = 0;
+    }
+}
+
+// One pipeline step: sum chunk `chunk` from (rows_cur, offs_cur) while the
+// copy of chunk + 1 into rows_next is in flight; publish its offsets, wait
+// for the copies, barrier.
+__device__ __forceinline__ void pipeline_step(const lds_params& p, const tile_ctx& t, int chunk, int nchunks,
+                                              bool active, const uint4* rows_cur, const int* offs_cur,
+                                              uint4* rows_next, int* offs_next, uint32_t (&acc16)[kR / 2],
+                                              uint32_t (&acc32)[kR])
+{
+    const bool more = chunk + 1 < nchunks;
+    if (more) stage_chunk(p, t, chunk + 1, rows_next);
+    if (active) {
+        sum_chunk(p, t, chunk, rows_cur, offs_cur, acc16);
+        if ((chunk + 1) % kFlushChunks == 0 || !more) flush(acc16, acc32);
+    }
+    if (more) {
+        const int e = chunk_offset(p, t, chunk + 1);
+        if (t.tid < kDms * kC) offs_next[t.tid] = e;
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's copies have landed
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kThreads) void dedisperse_lds_kernel(const uint8_t* __restrict__ L,
+                                                                  const float* __restrict__ dmshifts,
+                                                                  float* __restrict__ out, lds_params p)
+{
+    // XCD-aware tile order: work-groups are dispatched round-robin over the
+    // 8 XCDs, so each XCD takes a contiguous block of tiles (consecutive DM
+    // tiles of a time tile share most of their input window in its L2).
+    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (w >= ntiles) return;  // whole work-group leaves before any barrier
+
+    tile_ctx t;
+    t.L = L;
+    t.dmshifts = dmshifts;
+    t.d0 = (w % p.n_dm_tiles) * kDms;
+    t.t0 = (long)(w / p.n_dm_tiles) * kT;
+    t.tid = threadIdx.x;
+    t.lane = t.tid & 63;
+    t.wave = __builtin_amdgcn_readfirstlane(t.tid >> 6);
+    t.Wq = p.row_quads;
+    const int d = t.d0 + t.wave;
+    const bool active = d < p.ndms;
+    const int nchunks = (p.nchans + kC - 1) / kC;
+
+    uint32_t acc16[kR / 2];
+    uint32_t acc32[kR];
+#pragma unroll
+    for (int j = 0; j < kR / 2; ++j) acc16[j] = 0;
+#pragma unroll
+    for (int j = 0; j < kR; ++j) acc32[j] = 0;
+
+    stage_chunk(p, t, 0, g_rows0);
+    {
+        const int e = chunk_offset(p, t, 0);
+        if (t.tid < kDms * kC) g_offs0[t.tid] = e;
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+
+    for (int chunk = 0; chunk < nchunks; chunk += 2) {
+        pipeline_step(p, t, chunk, nchunks, active, g_rows0, g_offs0, g_rows1, g_offs1, acc16, acc32);
+        if (chunk + 1 >= nchunks) break;
+        pipeline_step(p, t, chunk + 1, nchunks, active, g_rows1, g_offs1, g_rows0, g_offs0, acc16, acc32);
+    }
+
+    if (!active) return;
+    const long tl = t.t0 + (long)kR * t.lane;
+    float* orow = out + (size_t)d * p.out_pitch;
+#pragma unroll
+    for (int j = 0; j < kR; ++j) {
+        const long tt = tl + j;
+        if (tt < p.t_processed) orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[j], p.fnchans), p.fbin);
+    }
+}
+
+}  // namespace
+
+int lds_row_quads(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
+{
+    // e_max over every tile and channel: delay(c, last DM of the tile) -
+    // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
+    // 16*floor(e/16) + 32*lane.
+    long e_max = 0;
+    for (int dt = 0; dt * kDms < ndms; ++dt) {
+        const int da = dt * kDms;
+        const int db = (da + kDms - 1 < ndms) ? da + kDms - 1 : ndms - 1;
+        const float s0 = __builtin_fmaf((float)da, mdmstep, mstartdm);
+        const float s1 = __builtin_fmaf((float)db, mdmstep, mstartdm);
+        for (int c = 0; c < nchans; ++c) {
+            const long a = (long)(int)(h_dmshifts[c] * s0);
+            const long b = (long)(int)(h_dmshifts[c] * s1);
+            const long e = b - (a & ~15L);
+            if (e > e_max) e_max = e;
+        }
+    }
+    const long quads = (e_max >> 4) + 2 * 63 + 3;  // last lane's third quad
+    return quads > kLdsRowQuadsMax ? -1 : (int)quads;
+}
+
+long lds_tile_samples() { return kT; }
+
+hipError_t launch_dedisperse_lds(const lds_params& p, hipStream_t s)
+{
+    if (p.row_quads <= 0 || p.row_quads > kLdsRowQuadsMax) return hipErrorInvalidValue;
+    if (p.ndms <= 0 || p.t_processed <= 0) return hipSuccess;
+    lds_params q = p;
+    q.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
+    q.n_t_tiles = (int)((p.t_processed + kT - 1) / kT);
+    const long ntiles = (long)q.n_dm_tiles * q.n_t_tiles;
+    const long grid = 8 * ((ntiles + 7) / 8);
+    if (grid > 0x7fffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(dedisperse_lds_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, p.L, p.dmshifts, p.out, q);
+    return hipGetLastError();
+}
+
+}  // namespace aa_hip

@@ -72,6 +72,8 @@ GEOMS = [
     (200, 1550.0, -1.5, 64e-6, 153.6, 0.2, 128, 2240, 1),
     (512, 1550.0, -0.5859, 64e-6, 0.0, 0.5, 192, 4000, 1),
     (96, 900.0, -2.0, 1e-4, 300.0, 2.0, 70, 1001, 4),
+    (77, 1200.0, -3.0, 64e-6, 0.0, 1.0, 48, 2500, 1),      # odd channel count, partial DM tile
+    (33, 1400.0, -8.0, 64e-6, 20.0, 0.5, 16, 7000, 2),     # 33 channels: one full chunk + 1
 ]
 
 
