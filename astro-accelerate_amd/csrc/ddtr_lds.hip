@@ -33,6 +33,12 @@ constexpr int kDms = 16;            // DM trials per work-group (= waves)
 constexpr int kC = 32;              // channels per LDS chunk
 constexpr int kThreads = 64 * kDms; // 1024
 constexpr int kFlushChunks = 256 / kC;
+// A staged row is kLdsRowQuadsMax uint4: the even quads of the row's window
+// in the first half, the odd quads in the second half (kHalf apart).  A lane
+// run starts at quad E + 2*lane; every lane has the parity of E, so each of
+// the run's three reads is a stride-1 access across the wave (no LDS bank
+// conflicts), where a plain layout would be stride 2.
+constexpr int kHalf = kLdsRowQuadsMax / 2;
 
 // Two separate LDS arrays (not one [2][...] array), so that the compiler can
 // tell the buffer being summed from the buffer being filled by the
@@ -64,7 +70,7 @@ __device__ __forceinline__ void accumulate_pair(uint32_t (&acc)[kR / 2], const u
         const uint32_t xb = __builtin_amdgcn_perm(wb[jb + 1], wb[jb], (j & 1) ? selb1 : selb0);
         // one v_add3_u32 (two channels, two samples), written as a single
         // instruction so the add cannot be split and moved out of the case
-        asm("v_add3_u32 %0, %1, %2, %3" : "=v"(acc[j]) : "v"(acc[j]), "v"(xa), "v"(xb));
+        asm("v_add3_u32 %0, %0, %1, %2" : "+v"(acc[j]) : "v"(xa), "v"(xb));
     }
 }
 
@@ -76,13 +82,29 @@ __device__ __forceinline__ void accumulate_one(uint32_t (&acc)[kR / 2], const ui
     for (int j = 0; j < kR / 2; ++j) {
         const int ja = QA + (j >> 1);
         const uint32_t xa = __builtin_amdgcn_perm(wa[ja + 1], wa[ja], (j & 1) ? sela1 : sela0);
-        asm("v_add_u32 %0, %1, %2" : "=v"(acc[j]) : "v"(acc[j]), "v"(xa));
+        asm("v_add_u32 %0, %0, %1" : "+v"(acc[j]) : "v"(xa));
     }
 }
 
-__device__ __forceinline__ void load_run(const uint4* p, uint32_t (&w)[12])
+// Reads the 3 quads of a lane run from a split row: `row` is the row base,
+// E the run's first quad for lane 0 (uniform).  Quad 2k is at row[k], quad
+// 2k+1 at row[kHalf + k].
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void load_run(const uint4* row, int E, int lane, uint32_t (&w)[12])
 {
-    const uint4 v0 = p[0], v1 = p[1], v2 = p[2];
+    // Offsets as integer arithmetic on the parity p (a select between two
+    // pointers would hide the LDS array from the compiler's alias analysis
+    // and make it wait for the in-flight copies before these reads).
+    const int p = E & 1;
+    const int m = (E >> 1) + lane;
+    const u32x4* lo = reinterpret_cast<const u32x4*>(row + m + p * kHalf);              // quads E, E+2
+    const u32x4* mid = reinterpret_cast<const u32x4*>(row + m + kHalf + p * (1 - kHalf));  // quad E+1
+    u32x4 v0 = lo[0], v1 = mid[0], v2 = lo[1];
+    // Every variant uses a different subset of the 12 dwords; marking all of
+    // them as used keeps each read a single ds_read_b128 (the compiler would
+    // otherwise split a partly used quad into narrower LDS reads).
+    asm("" : "+v"(v0), "+v"(v1), "+v"(v2));
     w[0] = v0.x; w[1] = v0.y; w[2] = v0.z; w[3] = v0.w;
     w[4] = v1.x; w[5] = v1.y; w[6] = v1.z; w[7] = v1.w;
     w[8] = v2.x; w[9] = v2.y; w[10] = v2.z; w[11] = v2.w;
@@ -93,6 +115,30 @@ __device__ __forceinline__ void load_run(const uint4* p, uint32_t (&w)[12])
 __device__ __forceinline__ uint32_t pair_selector(int r)
 {
     return 0x0c000c00u | ((uint32_t)(r + 1) << 16) | (uint32_t)r;
+}
+
+// Direct global -> LDS copy of 16 bytes per lane (global_load_lds_dwordx4):
+// lane l's 16 bytes land at LDS byte address lds_base + 16*l.  Issued from
+// inline asm so that the compiler does not track it: its wait-count pass
+// would otherwise stall LDS reads of the other buffer on these copies.  The
+// kernel waits for them explicitly (vmcnt(0)) before each barrier; M0 is
+// saved and restored around the copy.
+__device__ __forceinline__ void lds_copy16(const void* src, uint32_t lds_base)
+{
+    uint32_t saved;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %2\n\t"
+        "global_load_lds_dwordx4 %1, off\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(saved)
+        : "v"(src), "s"(lds_base)
+        : "memory");
+}
+
+__device__ __forceinline__ uint32_t lds_addr(const uint4* p)
+{
+    return (uint32_t)(uintptr_t)p;
 }
 
 struct tile_ctx {
@@ -115,11 +161,16 @@ __device__ __forceinline__ void stage_chunk(const lds_params& p, const tile_ctx&
         const float ds = t.dmshifts[c];
         const int b16 = __builtin_amdgcn_readfirstlane(dev_delay(ds, t.d0, p.mstartdm, p.mdmstep) & ~15);
         const uint8_t* src = t.L + (size_t)c * p.pitch + (size_t)t.t0 + (size_t)b16;
-        uint4* dst = rows + row * t.Wq;
-        for (int col = 0; col < t.Wq; col += 64) {
-            if (col + t.lane < t.Wq)
-                __builtin_amdgcn_global_load_lds((const void*)(src + 16 * (size_t)(col + t.lane)),
-                                                 (void __attribute__((address_space(3)))*)(dst + col), 16, 0, 0);
+        uint4* dst = rows + row * kLdsRowQuadsMax;
+        const int n_even = (t.Wq + 1) >> 1, n_odd = t.Wq >> 1;
+        for (int j = 0; j < n_even; j += 64) {
+            if (j + t.lane < n_even)
+                lds_copy16(src + 32 * (size_t)(j + t.lane), __builtin_amdgcn_readfirstlane(lds_addr(dst + j)));
+        }
+        for (int j = 0; j < n_odd; j += 64) {
+            if (j + t.lane < n_odd)
+                lds_copy16(src + 16 + 32 * (size_t)(j + t.lane),
+                           __builtin_amdgcn_readfirstlane(lds_addr(dst + kHalf + j)));
         }
     }
 }
@@ -140,45 +191,51 @@ __device__ __forceinline__ int chunk_offset(const lds_params& p, const tile_ctx&
     return e;
 }
 
-// Sums one staged chunk into the wave's packed accumulators.
+// All channel pairs of the chunk whose two runs have dword shifts (QA, QB):
+// bit 2p of `m` set means pair (2p, 2p+1), whose offsets sit in lanes 2p and
+// 2p+1 of e_v.  A plain loop, so the accumulators stay in their registers.
+template <int QA, int QB>
+__device__ __forceinline__ void sum_group(uint64_t m, int e_v, const uint4* rows, int lane, uint32_t (&acc16)[kR / 2])
+{
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        const int ea = __builtin_amdgcn_readlane(e_v, l);
+        const int eb = __builtin_amdgcn_readlane(e_v, l + 1);
+        uint32_t wa[12], wb[12];
+        load_run(rows + l * kLdsRowQuadsMax, ea >> 4, lane, wa);
+        load_run(rows + (l + 1) * kLdsRowQuadsMax, eb >> 4, lane, wb);
+        const uint32_t sa0 = pair_selector(ea & 3), sa1 = sa0 + 0x00020002u;
+        const uint32_t sb0 = pair_selector(eb & 3), sb1 = sb0 + 0x00020002u;
+        accumulate_pair<QA, QB>(acc16, wa, wb, sa0, sa1, sb0, sb1);
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void sum_groups(int variant, bool voter, int e_v, const uint4* rows, int lane,
+                                           uint32_t (&acc16)[kR / 2])
+{
+    const uint64_t m = __builtin_amdgcn_ballot_w64(voter && variant == V);
+    sum_group<V / 4, V % 4>(m, e_v, rows, lane, acc16);
+    if constexpr (V + 1 < 16) sum_groups<V + 1>(variant, voter, e_v, rows, lane, acc16);
+}
+
+// Sums one staged chunk into the wave's packed accumulators.  Lane l holds
+// the offset of chunk channel l; pairs are grouped by their (dword shift a,
+// dword shift b) variant with one ballot per variant.
 __device__ __forceinline__ void sum_chunk(const lds_params& p, const tile_ctx& t, int chunk, const uint4* rows,
                                           const int* offs, uint32_t (&acc16)[kR / 2])
 {
-    const int wave_off = t.wave * kC;
-    const uint4* base = rows + 2 * t.lane;
     const int nrows = (p.nchans - chunk * kC) < kC ? (p.nchans - chunk * kC) : kC;
-#pragma unroll 1
-    for (int ci = 0; ci + 1 < nrows; ci += 2) {
-        const int ea = __builtin_amdgcn_readfirstlane(offs[wave_off + ci]);
-        const int eb = __builtin_amdgcn_readfirstlane(offs[wave_off + ci + 1]);
-        uint32_t wa[12], wb[12];
-        load_run(base + ci * t.Wq + (ea >> 4), wa);
-        load_run(base + (ci + 1) * t.Wq + (eb >> 4), wb);
-        const uint32_t sa0 = pair_selector(ea & 3), sa1 = sa0 + 0x00020002u;
-        const uint32_t sb0 = pair_selector(eb & 3), sb1 = sb0 + 0x00020002u;
-        switch (((ea >> 2) & 3) * 4 + ((eb >> 2) & 3)) {
-            case 0: accumulate_pair<0, 0>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 1: accumulate_pair<0, 1>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 2: accumulate_pair<0, 2>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 3: accumulate_pair<0, 3>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 4: accumulate_pair<1, 0>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 5: accumulate_pair<1, 1>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 6: accumulate_pair<1, 2>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 7: accumulate_pair<1, 3>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 8: accumulate_pair<2, 0>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 9: accumulate_pair<2, 1>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 10: accumulate_pair<2, 2>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 11: accumulate_pair<2, 3>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 12: accumulate_pair<3, 0>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 13: accumulate_pair<3, 1>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            case 14: accumulate_pair<3, 2>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-            default: accumulate_pair<3, 3>(acc16, wa, wb, sa0, sa1, sb0, sb1); break;
-        }
-    }
+    const int e_v = t.lane < nrows ? offs[t.wave * kC + t.lane] : 0;
+    const int q = (e_v >> 2) & 3;
+    const int q_next = __shfl_down(q, 1, 64);
+    const bool voter = ((t.lane & 1) == 0) && (t.lane + 1 < nrows);
+    sum_groups<0>(q * 4 + q_next, voter, e_v, rows, t.lane, acc16);
     if (nrows & 1) {  // odd channel count in the last chunk: its last channel alone
-        const int ea = __builtin_amdgcn_readfirstlane(offs[wave_off + nrows - 1]);
+        const int ea = __builtin_amdgcn_readlane(e_v, nrows - 1);
         uint32_t wa[12];
-        load_run(base + (nrows - 1) * t.Wq + (ea >> 4), wa);
+        load_run(rows + (nrows - 1) * kLdsRowQuadsMax, ea >> 4, t.lane, wa);
         const uint32_t sa0 = pair_selector(ea & 3), sa1 = sa0 + 0x00020002u;
         switch ((ea >> 2) & 3) {
             case 0: accumulate_one<0>(acc16, wa, sa0, sa1); break;

@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "direct", "lds"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=16, help="DM rows per range in the CPU sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -77,6 +79,16 @@ def main():
     torch.cuda.set_device(dev)
     lib = aa.load_library()
     kernel = {"auto": aa.KERNEL_AUTO, "direct": aa.KERNEL_DIRECT, "lds": aa.KERNEL_LDS}[args.kernel]
+
+    if args.copy_calibration:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 1 GiB read + 1 GiB written")
+        del src, dst
 
     free_b, _ = torch.cuda.mem_get_info()
     plan = aa.make_plan(cfg["ranges"])
