@@ -165,14 +165,28 @@ def _lib_or_raise() -> ctypes.CDLL:
     return load_library()
 
 
+# aa_hip_status (include/aa_ddtr_hip.h)
+AA_HIP_SUCCESS = 0
+AA_HIP_ERROR_INVALID_ARGUMENT = -1
+AA_HIP_ERROR_DEVICE = -2
+AA_HIP_ERROR_OUT_OF_MEMORY = -3
+AA_HIP_ERROR_UNSUPPORTED = -4
+AA_HIP_ERROR_INPUT_RANGE = -5
+AA_HIP_ERROR_NO_DEVICE = -6
+
+
 class AAHipError(RuntimeError):
-    """A negative aa_hip_status, with the library's message."""
+    """A negative aa_hip_status (``.status``), with the library's message."""
+
+    def __init__(self, message: str, status: Optional[int] = None):
+        super().__init__(message)
+        self.status = status
 
 
 def check(status: int, what: str = "") -> None:
     if status != 0:
         msg = _lib_or_raise().aa_hip_last_error().decode(errors="replace")
-        raise AAHipError(f"{what} failed with status {status}: {msg}")
+        raise AAHipError(f"{what} failed with status {status}: {msg}", status)
 
 
 # ---------------------------------------------------------------- FFI mirror

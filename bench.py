@@ -46,6 +46,13 @@ CONFIGS = {
                     (2000, 3000, 2.0, 8, 8)], pulses="config2"),
 }
 ADD_ROOFLINE_T = 78.64   # T adds/s, packed FP32 VALU adds
+# HBM bytes moved by the dedisperse launches of one step, from rocprofv3
+# FETCH_SIZE (x2.000, calibrated on a 1 GiB device copy) + WRITE_SIZE (x1.000)
+# summed over the 8 launches; recorded in
+# profiles/dedisperse_lds_8range_4096ch_1048576_hbm_traffic.txt. Counters
+# cannot be read live, so the figure is tied to the kernel it was taken on.
+MEASURED_TRAFFIC = {2: {"bytes": 78.70e9, "kernel": "lds",
+                        "profile": "profiles/dedisperse_lds_8range_4096ch_1048576_hbm_traffic.txt"}}
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 
@@ -171,6 +178,8 @@ def main():
         f"{achieved:.2f} T adds/s ({achieved / ADD_ROOFLINE_T * 100:.1f}% of {ADD_ROOFLINE_T})")
 
     value = dm_x_samples / (ms_step * 1e-3)
+    meas = MEASURED_TRAFFIC.get(args.config)
+    traffic = meas["bytes"] if meas and args.kernel in ("auto", meas["kernel"]) else None
     result = {
         "metric": "DM×sample/s bit-exact, % FP32 add roofline; batched vs per-beam speedup",
         "value": value,
@@ -189,10 +198,12 @@ def main():
                    "kernel": args.kernel, "parallelism": "single GPU"},
         "channel_adds_per_s": adds / (ms_step * 1e-3),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": ADD_ROOFLINE_T, "unit": "Tadd/s",
-                     "frac": achieved / ADD_ROOFLINE_T, "traffic": None,
+                     "frac": achieved / ADD_ROOFLINE_T, "traffic": traffic,
                      "kernel": "dedisperse (all ranges)", "kernel_ms": k_total_ms,
                      "hbm_algorithmic_GBs": hbm_bytes / (k_total_ms * 1e-3) / 1e9,
-                     "hbm_frac": hbm_bytes / (k_total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                     "hbm_frac": hbm_bytes / (k_total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "hbm_algorithmic_bytes": hbm_bytes,
+                     "traffic_source": meas["profile"] if traffic else None},
     }
 
     if not args.no_cpu_baseline:
