@@ -1,0 +1,214 @@
+// extern "C" surface of the fractional-delay (differentiable) dedispersion:
+// a plan holding the delay table and trial offsets on the device, forward
+// and adjoint calls on caller-owned device buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "aa_ddtr_hip.h"
+#include "ddtr_engine.hpp"
+#include "frac_ddtr.h"
+
+struct aa_hip_frac {
+    int device = 0;
+    int nchans = 0, ntrials = 0, nsamp = 0, max_batch = 0;
+    int trials_per_wave = 0;     // forward / DM adjoint grouping
+    int chans_per_wave = 0;      // input adjoint grouping
+    float* d_ds = nullptr;
+    float* d_trials = nullptr;
+    double* d_part = nullptr;
+    size_t part_entries = 0;
+    size_t bytes = 0;
+};
+
+namespace {
+
+int fail(int status, const std::string& msg) {
+    aa_hip::set_last_error(msg);
+    return status;
+}
+
+int from_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return AA_HIP_SUCCESS;
+    aa_hip::set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? AA_HIP_ERROR_OUT_OF_MEMORY : AA_HIP_ERROR_DEVICE;
+}
+
+void release(aa_hip_frac* p) {
+    if (!p) return;
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess && cur != p->device) (void)hipSetDevice(p->device);
+    if (p->d_ds) (void)hipFree(p->d_ds);
+    if (p->d_trials) (void)hipFree(p->d_trials);
+    if (p->d_part) (void)hipFree(p->d_part);
+    if (cur != p->device) (void)hipSetDevice(cur);
+    delete p;
+}
+
+// Channel groups (16 * q channels) for the input adjoint: the largest q whose
+// delay spread over a group stays inside a staged row for |trial + dm| up to
+// u_absmax.  Outside that, the kernel reads the window from global memory.
+int chans_per_wave(const std::vector<float>& ds, float u_absmax) {
+    const int C = (int)ds.size();
+    for (int q = aa_hip::kFracMaxPerWave; q >= 1; q >>= 1) {
+        bool ok = true;
+        for (int c0 = 0; c0 < C && ok; c0 += 16 * q) {
+            float lo = ds[(size_t)c0], hi = lo;
+            for (int c = c0; c < std::min(C, c0 + 16 * q); ++c) {
+                lo = std::min(lo, ds[(size_t)c]);
+                hi = std::max(hi, ds[(size_t)c]);
+            }
+            ok = ((double)hi - lo) * u_absmax + 8.0 <= (double)aa_hip::kFracMaxOffset;
+        }
+        if (ok) return q;
+    }
+    return 1;
+}
+
+aa_hip::frac_args base_args(const aa_hip_frac* p, int nbatch, const float* d_dm) {
+    aa_hip::frac_args a{};
+    a.ds = p->d_ds;
+    a.trials = p->d_trials;
+    a.dm = d_dm;
+    a.B = nbatch;
+    a.C = p->nchans;
+    a.T = p->nsamp;
+    a.K = p->ntrials;
+    a.n_ttiles = (p->nsamp + aa_hip::kFracTile - 1) / aa_hip::kFracTile;
+    return a;
+}
+
+bool aligned16(const void* ptr) { return ((uintptr_t)ptr & 15u) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int aa_hip_frac_create(int device, int nchans, const float* delay_per_dm, int ntrials, const float* trials,
+                       int nsamp, int max_batch, float dm_absmax, aa_hip_frac** out) {
+    if (!out) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL output handle");
+    *out = nullptr;
+    if (!delay_per_dm || !trials) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL table");
+    if (nchans <= 0 || ntrials <= 0 || nsamp <= 0 || max_batch <= 0 || !(dm_absmax >= 0.0f))
+        return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "invalid geometry");
+    if (nsamp % 4 != 0) return fail(AA_HIP_ERROR_UNSUPPORTED, "nsamp must be a multiple of 4");
+    std::vector<float> ds(delay_per_dm, delay_per_dm + nchans);
+    std::vector<float> tr(trials, trials + ntrials);
+    float ds_absmax = 0.0f, tr_absmax = 0.0f;
+    for (float d : ds) {
+        if (!std::isfinite(d)) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "non-finite delay table");
+        ds_absmax = std::max(ds_absmax, std::fabs(d));
+    }
+    for (float t : tr) {
+        if (!std::isfinite(t)) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "non-finite trial offset");
+        tr_absmax = std::max(tr_absmax, std::fabs(t));
+    }
+    // every delay must stay far inside int range: |tau| < 2^24 samples
+    if ((double)ds_absmax * ((double)tr_absmax + dm_absmax) >= 16777216.0)
+        return fail(AA_HIP_ERROR_UNSUPPORTED, "delays beyond 2^24 samples");
+    const int tq = aa_hip::frac_trials_per_wave(tr.data(), ntrials, ds_absmax);
+    if (tq == 0)
+        return fail(AA_HIP_ERROR_UNSUPPORTED,
+                    "trial offsets too far apart: 16 consecutive trials must span at most " +
+                        std::to_string(aa_hip::kFracMaxOffset - 8) + " samples of delay");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(AA_HIP_ERROR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "device out of range");
+
+    auto* p = new (std::nothrow) aa_hip_frac();
+    if (!p) return fail(AA_HIP_ERROR_OUT_OF_MEMORY, "host allocation");
+    p->device = device;
+    p->nchans = nchans;
+    p->ntrials = ntrials;
+    p->nsamp = nsamp;
+    p->max_batch = max_batch;
+    p->trials_per_wave = tq;
+    p->chans_per_wave = chans_per_wave(ds, tr_absmax + dm_absmax);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    hipError_t e = hipSetDevice(device);
+    const int ttiles = (nsamp + aa_hip::kFracTile - 1) / aa_hip::kFracTile;
+    const int groups = (ntrials + 16 * tq - 1) / (16 * tq);
+    p->part_entries = (size_t)max_batch * ttiles * groups;
+    if (e == hipSuccess) e = hipMalloc(&p->d_ds, sizeof(float) * nchans);
+    if (e == hipSuccess) e = hipMalloc(&p->d_trials, sizeof(float) * ntrials);
+    if (e == hipSuccess) e = hipMalloc(&p->d_part, sizeof(double) * p->part_entries);
+    if (e == hipSuccess) e = hipMemcpy(p->d_ds, ds.data(), sizeof(float) * nchans, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_trials, tr.data(), sizeof(float) * ntrials, hipMemcpyHostToDevice);
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) {
+        const int st = from_hip(e, "aa_hip_frac_create");
+        release(p);
+        return st;
+    }
+    p->bytes = sizeof(float) * (nchans + ntrials) + sizeof(double) * p->part_entries;
+    *out = p;
+    return AA_HIP_SUCCESS;
+}
+
+int aa_hip_frac_destroy(aa_hip_frac* h) {
+    release(h);
+    return AA_HIP_SUCCESS;
+}
+
+int aa_hip_frac_grouping(const aa_hip_frac* h, int* trials_per_wave, int* chans_per_wave) {
+    if (!h) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL handle");
+    if (trials_per_wave) *trials_per_wave = h->trials_per_wave;
+    if (chans_per_wave) *chans_per_wave = h->chans_per_wave;
+    return AA_HIP_SUCCESS;
+}
+
+int aa_hip_frac_forward(aa_hip_frac* h, const float* d_x, int nbatch, const float* d_dm, float* d_y,
+                        void* stream) {
+    if (!h || !d_x || !d_dm || !d_y) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL argument");
+    if (nbatch < 0 || nbatch > h->max_batch) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "batch exceeds the plan's");
+    if (!aligned16(d_x) || !aligned16(d_y)) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "buffers must be 16-byte aligned");
+    if (nbatch == 0) return AA_HIP_SUCCESS;
+    aa_hip::frac_args a = base_args(h, nbatch, d_dm);
+    a.x = d_x;
+    a.y = d_y;
+    a.group_q = h->trials_per_wave;
+    a.n_groups = (h->ntrials + 16 * a.group_q - 1) / (16 * a.group_q);
+    return from_hip(aa_hip::launch_frac_forward(a, (hipStream_t)stream), "frac forward");
+}
+
+int aa_hip_frac_adjoint(aa_hip_frac* h, const float* d_x, const float* d_gy, int nbatch, const float* d_dm,
+                        float* d_gx, double* d_gdm, void* stream) {
+    if (!h || !d_gy || !d_dm) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL argument");
+    if (d_gdm && !d_x) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "the DM adjoint needs the input");
+    if (nbatch < 0 || nbatch > h->max_batch) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "batch exceeds the plan's");
+    if ((d_x && !aligned16(d_x)) || !aligned16(d_gy) || (d_gx && !aligned16(d_gx)))
+        return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "buffers must be 16-byte aligned");
+    if (nbatch == 0) return AA_HIP_SUCCESS;
+    const hipStream_t s = (hipStream_t)stream;
+    if (d_gx) {
+        aa_hip::frac_args a = base_args(h, nbatch, d_dm);
+        a.gy = d_gy;
+        a.gx = d_gx;
+        a.group_q = h->chans_per_wave;
+        a.n_groups = (h->nchans + 16 * a.group_q - 1) / (16 * a.group_q);
+        const int st = from_hip(aa_hip::launch_frac_grad_x(a, s), "frac input adjoint");
+        if (st != AA_HIP_SUCCESS) return st;
+    }
+    if (d_gdm) {
+        aa_hip::frac_args a = base_args(h, nbatch, d_dm);
+        a.x = d_x;
+        a.gy = d_gy;
+        a.gdm = d_gdm;
+        a.gdm_part = h->d_part;
+        a.group_q = h->trials_per_wave;
+        a.n_groups = (h->ntrials + 16 * a.group_q - 1) / (16 * a.group_q);
+        if (aa_hip::frac_grad_dm_parts(a) > h->part_entries)
+            return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "workspace too small for the batch");
+        const int st = from_hip(aa_hip::launch_frac_grad_dm(a, s), "frac DM adjoint");
+        if (st != AA_HIP_SUCCESS) return st;
+    }
+    return AA_HIP_SUCCESS;
+}
+
+}  // extern "C"

@@ -154,6 +154,11 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_dedisperse_u8": ([vp, sz, i, i, PFLOAT, vp, i, i, f, f, i, vp, sz, i, vp], i),
         "aa_hip_synth_noise_u8": ([u64, i64, d, d, vp, vp], i),
         "aa_hip_inject_pulse_u8": ([vp, i, i, vp, d, d, i, i, i, vp], i),
+        "aa_hip_frac_create": ([i, i, PFLOAT, i, PFLOAT, i, i, f, ctypes.POINTER(vp)], i),
+        "aa_hip_frac_destroy": ([vp], i),
+        "aa_hip_frac_grouping": ([vp, pi, pi], i),
+        "aa_hip_frac_forward": ([vp, vp, i, vp, vp, vp], i),
+        "aa_hip_frac_adjoint": ([vp, vp, vp, i, vp, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -271,6 +271,40 @@ int aa_hip_synth_noise_u8(uint64_t seed, int64_t n, double mean, double sigma, u
 int aa_hip_inject_pulse_u8(uint8_t *d_data_tc, int nsamp, int nchans, const float *d_dmshifts,
                            double dm, double tsamp, int t0, int width, int amplitude, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Fractional-delay (differentiable) dedispersion of FP32 candidate windows.
+ * No counterpart in the reference (it has no autograd, fractional delays or
+ * batching); it uses the reference's delay table, as samples per unit DM
+ * (dmshift[c] / tsamp, src/aa_ddtr_strategy.cpp:81-92).
+ *
+ * x is [B][C][T] float (T = nsamp, a multiple of 4), dm[B] the candidates'
+ * continuous DM offsets, trials[K] the trial DM offsets.  For each b, k, t:
+ *   u = trials[k] + dm[b]; tau = delay_per_dm[c] * u (in double);
+ *   y[b][k][t] = sum_c (1 - f) x[b][c][t + i0] + f x[b][c][t + i0 + 1],
+ *   i0 = floor(tau), f = (float)(tau - i0), x = 0 outside [0, T).
+ * The adjoint gives gx = (dy/dx)^T gy and gdm[b] = sum_{k,t} gy * dy/d(dm[b])
+ * (accumulated in double).  Device buffers are caller-owned, 16-byte
+ * aligned; calls on one handle must not overlap.
+ * ---------------------------------------------------------------------- */
+typedef struct aa_hip_frac aa_hip_frac;
+/* dm_absmax bounds |dm[b]| for sizing the input adjoint's channel groups
+ * (larger values stay correct, through a slower path). */
+int aa_hip_frac_create(int device, int nchans, const float *delay_per_dm, int ntrials,
+                       const float *trials, int nsamp, int max_batch, float dm_absmax,
+                       aa_hip_frac **out);
+int aa_hip_frac_destroy(aa_hip_frac *h);
+/* Trials (forward, DM adjoint) and channels (input adjoint) per wave chosen
+ * for this plan (1 or 2). */
+int aa_hip_frac_grouping(const aa_hip_frac *h, int *trials_per_wave, int *chans_per_wave);
+/* y: [nbatch][K][T] float. */
+int aa_hip_frac_forward(aa_hip_frac *h, const float *d_x, int nbatch, const float *d_dm, float *d_y,
+                        void *stream);
+/* gy: [nbatch][K][T]; writes gx ([nbatch][C][T], may be NULL) and gdm
+ * ([nbatch] double, may be NULL; needs d_x).  Recomputes the interpolation
+ * from x: nothing from the forward pass is stored. */
+int aa_hip_frac_adjoint(aa_hip_frac *h, const float *d_x, const float *d_gy, int nbatch,
+                        const float *d_dm, float *d_gx, double *d_gdm, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,7 +2,7 @@
  * CPU oracle for AstroAccelerate DDTR -- TEST INFRASTRUCTURE ONLY.
  * See ddtr_oracle.h for scope and for who may use this file.
  *
- * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ * Build: oracle/Makefile (gcc -O3 -fopenmp -ffp-contract=off).
  * Every function cites the reference file:line it restates.
  */
 #include "ddtr_oracle.h"

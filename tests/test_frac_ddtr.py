@@ -1,0 +1,280 @@
+"""Fractional-delay (differentiable) dedispersion: reference pins and HIP parity.
+
+CPU tests pin the reference (oracle/frac_ddtr_ref.py) against scalar loops,
+closed forms (integer and half-sample delays), the adjoint identity and
+float64 finite differences.  GPU tests compare the HIP forward and adjoint
+kernels (through the C ABI, via frac_ddtr.FracPlan) with the reference, and
+the DM gradient with float64 finite differences (relative 1e-4).
+"""
+import numpy as np
+import pytest
+
+import frac_ddtr_ref as ref
+
+BAND = dict(fch1=1760.0, foff_total=-810.0, tsamp=64e-6)
+
+
+def band_delays(nchans):
+    import frac_ddtr
+    return frac_ddtr.delay_per_dm(nchans, BAND["fch1"], BAND["foff_total"] / nchans, BAND["tsamp"])
+
+
+# ----------------------------------------------------------------- CPU: pins
+def test_reference_vectorised_matches_loops():
+    import torch
+    rng = np.random.default_rng(1)
+    B, C, T, K = 2, 5, 12, 3
+    x = rng.standard_normal((B, C, T))
+    ds = rng.uniform(0.0, 3.0, C)
+    trials = rng.uniform(-2.0, 2.0, K)
+    dm = rng.uniform(-1.0, 1.0, B)
+    want = ref.frac_ddtr_loops(x, dm, trials, ds)
+    got = ref.frac_ddtr_torch(torch.tensor(x), torch.tensor(dm), trials, ds).numpy()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+def test_reference_integer_delays_are_a_shifted_sum():
+    """f = 0: y[k, t] = sum_c x[c, t + ds_c * u_k] with zero padding."""
+    import torch
+    rng = np.random.default_rng(2)
+    B, C, T = 1, 6, 40
+    x = rng.integers(0, 255, (B, C, T)).astype(np.float64)
+    ds = np.array([0, 1, 2, 3, 5, 8], dtype=np.float64)
+    trials = np.array([-2.0, 0.0, 1.0, 3.0])
+    dm = np.zeros(B)
+    got = ref.frac_ddtr_torch(torch.tensor(x), torch.tensor(dm), trials, ds).numpy()
+    for k, u in enumerate(trials):
+        want = np.zeros(T)
+        for c in range(C):
+            s = int(ds[c] * u)
+            for t in range(T):
+                if 0 <= t + s < T:
+                    want[t] += x[0, c, t + s]
+        np.testing.assert_array_equal(got[0, k], want)
+
+
+def test_reference_half_sample_delay_is_the_mean_of_neighbours():
+    import torch
+    x = np.arange(10, dtype=np.float64)[None, None, :] ** 2
+    got = ref.frac_ddtr_torch(torch.tensor(x), torch.zeros(1, dtype=torch.float64), [0.5], [3.0]).numpy()
+    # tau = 1.5: y[t] = (x[t+1] + x[t+2]) / 2, with x = 0 past the end
+    xp = np.concatenate([x[0, 0], [0.0, 0.0]])
+    np.testing.assert_allclose(got[0, 0], 0.5 * (xp[1:11] + xp[2:12]), rtol=0, atol=1e-12)
+
+
+def test_reference_adjoint_identity():
+    """<y(x), g> == <x, J^T g>: the autograd adjoint in x is the transpose."""
+    import torch
+    rng = np.random.default_rng(3)
+    B, C, T, K = 2, 7, 33, 4
+    x = torch.tensor(rng.standard_normal((B, C, T)), requires_grad=True)
+    g = torch.tensor(rng.standard_normal((B, K, T)))
+    ds = rng.uniform(0.0, 4.0, C)
+    trials = rng.uniform(-2, 2, K)
+    dm = torch.tensor(rng.uniform(-1, 1, B))
+    y = ref.frac_ddtr_torch(x, dm, trials, ds)
+    (gx,) = torch.autograd.grad((y * g).sum(), x)
+    lhs = float((y * g).sum().detach())
+    rhs = float((x * gx).sum().detach())
+    assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs))
+
+
+def test_reference_dm_gradient_matches_finite_differences():
+    import torch
+    rng = np.random.default_rng(4)
+    B, C, T, K = 3, 9, 64, 5
+    x = torch.tensor(rng.standard_normal((B, C, T)))
+    ds = rng.uniform(0.0, 6.0, C)
+    trials = np.linspace(-2, 2, K)
+    dm0 = rng.uniform(-1, 1, B)
+    dm = torch.tensor(dm0, requires_grad=True)
+    loss = (ref.frac_ddtr_torch(x, dm, trials, ds) ** 2).sum()
+    (g,) = torch.autograd.grad(loss, dm)
+    h = 1e-7
+    for b in range(B):
+        e = np.zeros(B)
+        e[b] = h
+        lp = float((ref.frac_ddtr_torch(x, torch.tensor(dm0 + e), trials, ds) ** 2).sum())
+        lm = float((ref.frac_ddtr_torch(x, torch.tensor(dm0 - e), trials, ds) ** 2).sum())
+        fd = (lp - lm) / (2 * h)
+        assert abs(float(g[b]) - fd) <= 1e-5 * max(1.0, abs(fd)), (b, float(g[b]), fd)
+
+
+def test_plan_rejects_trial_spans_beyond_a_staged_row(aa):
+    import frac_ddtr
+    ds = np.full(64, 50.0, dtype=np.float32)
+    with pytest.raises(aa.AAHipError) as ei:   # 16 trials spanning 150 DM -> 7500 samples
+        frac_ddtr.FracPlan(ds, np.linspace(0, 150, 16), nsamp=1024, max_batch=1)
+    assert ei.value.status == aa.AA_HIP_ERROR_UNSUPPORTED
+    with pytest.raises(aa.AAHipError):
+        frac_ddtr.FracPlan(ds, np.zeros(4), nsamp=1022, max_batch=1)   # nsamp % 4 != 0
+
+
+# ------------------------------------------------------------ GPU: parity
+def _relerr(got, want):
+    got = np.asarray(got, dtype=np.float64)
+    want = np.asarray(want, dtype=np.float64)
+    return float(np.max(np.abs(got - want)) / max(1e-30, np.max(np.abs(want))))
+
+
+GEOMS = [
+    # B, C, T, K, dm spread
+    (3, 37, 2052, 21, 1.5),
+    (2, 256, 512, 64, 1.5),
+    (1, 13, 4, 3, 0.5),
+    (4, 100, 1024, 32, 6.0),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: "x".join(map(str, g[:4])))
+def test_hip_forward_and_adjoints_match_reference(cuda, geom):
+    import torch
+    import frac_ddtr
+    B, C, T, K, spread = geom
+    ds = band_delays(C)
+    trials = np.linspace(-2.0, 2.0, K).astype(np.float32)
+    plan = frac_ddtr.FracPlan(ds, trials, T, max_batch=B, dm_absmax=spread)
+    g = torch.Generator(device=cuda).manual_seed(11 + C)
+    x = torch.randn((B, C, T), generator=g, device=cuda)
+    dm = ((torch.rand((B,), generator=g, device=cuda) * 2 - 1) * spread).contiguous()
+    gy = torch.randn((B, K, T), generator=g, device=cuda)
+    y = plan.forward(x, dm)
+    gx, gdm = plan.adjoint(x, gy, dm)
+    torch.cuda.synchronize()
+
+    x64 = x.double().requires_grad_(True)
+    dm64 = dm.double().requires_grad_(True)
+    y_ref = ref.frac_ddtr_torch(x64, dm64, trials, ds)
+    gx_ref, gdm_ref = torch.autograd.grad((y_ref * gy.double()).sum(), (x64, dm64))
+    # float32 sums of 2*C terms (forward), 2*K terms (x adjoint), 34*K*C terms (DM adjoint)
+    assert _relerr(y.cpu(), y_ref.detach().cpu()) <= 2e-6
+    assert _relerr(gx.cpu(), gx_ref.cpu()) <= 2e-6
+    assert _relerr(gdm.cpu(), gdm_ref.cpu()) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_hip_x_adjoint_far_outside_the_planned_dm_range(cuda):
+    """|dm| far beyond dm_absmax: windows leave the staged rows and are read
+    from global memory; results are unchanged."""
+    import torch
+    import frac_ddtr
+    B, C, T, K = 2, 64, 2048, 16
+    ds = band_delays(C)
+    trials = np.linspace(-1.0, 1.0, K).astype(np.float32)
+    plan = frac_ddtr.FracPlan(ds, trials, T, max_batch=B, dm_absmax=1.0)
+    assert plan.grouping()[1] == 2
+    g = torch.Generator(device=cuda).manual_seed(5)
+    x = torch.randn((B, C, T), generator=g, device=cuda)
+    dm = torch.tensor([25.0, -18.0], device=cuda)
+    gy = torch.randn((B, K, T), generator=g, device=cuda)
+    y = plan.forward(x, dm)
+    gx, gdm = plan.adjoint(x, gy, dm)
+    x64 = x.double().requires_grad_(True)
+    dm64 = dm.double().requires_grad_(True)
+    y_ref = ref.frac_ddtr_torch(x64, dm64, trials, ds)
+    gx_ref, gdm_ref = torch.autograd.grad((y_ref * gy.double()).sum(), (x64, dm64))
+    assert _relerr(y.cpu(), y_ref.detach().cpu()) <= 2e-6
+    assert _relerr(gx.cpu(), gx_ref.cpu()) <= 2e-6
+    assert _relerr(gdm.cpu(), gdm_ref.cpu()) <= 1e-5
+
+
+def kink_free_dms(ds, trials, count, margin, seed):
+    """float32 DM offsets in [-1, 1] at which every delay tau = ds[c] * (trials[k] + dm)
+    lies at least `margin` samples from an integer (a cell boundary of the
+    interpolation), in float64 and in the kernels' float32 evaluation alike.
+    Returns (dm [count] float32, smallest distance per dm)."""
+    rng = np.random.default_rng(seed)
+    moving = np.asarray(ds, dtype=np.float32) != 0     # zero-delay channels never change cell
+    ds32 = np.asarray(ds, dtype=np.float32)[moving]
+    ds64 = ds32.astype(np.float64)
+    tr32 = np.asarray(trials, dtype=np.float32)
+    out, dist = [], []
+    for _ in range(10000):
+        if len(out) == count:
+            break
+        dm = np.float32(rng.uniform(-1.0, 1.0))
+        tau64 = ds64[:, None] * (tr32.astype(np.float64)[None, :] + np.float64(dm))
+        tau32 = (ds32[:, None] * (tr32[None, :] + dm)).astype(np.float64)
+        d = np.minimum(np.abs(tau64 - np.round(tau64)), np.abs(tau32 - np.round(tau32))).min()
+        if d >= margin and np.array_equal(np.floor(tau64), np.floor(tau32)):
+            out.append(dm)
+            dist.append(d)
+    if len(out) < count:
+        raise RuntimeError("no DM offsets with every delay away from a cell boundary")
+    return np.array(out, dtype=np.float32), np.array(dist)
+
+
+@pytest.mark.gpu
+def test_autograd_gradients_match_float64_finite_differences(cuda):
+    """loss = sum of squared profiles.  The loss is piecewise quadratic in
+    the DM (its slope changes wherever a delay crosses an integer), so the
+    check is made at DMs where every delay is at least 1e-4 samples from an
+    integer, with a step that crosses none: there the float64 central
+    difference is exact up to rounding, and float32 and float64 place every
+    delay in the same cell.  d loss / d x is checked along a random direction
+    (the loss is quadratic in x).  Bar: 1e-4 relative."""
+    import torch
+    import frac_ddtr
+    B, C, T, K = 4, 128, 1024, 16
+    ds = band_delays(C)
+    trials = np.linspace(-2.0, 2.0, K).astype(np.float32)
+    plan = frac_ddtr.FracPlan(ds, trials, T, max_batch=B)
+    x0, r, _ = frac_ddtr.synth_candidates(B, C, T, ds, seed=7, device=cuda)
+    dms, dist = kink_free_dms(ds, trials, B, margin=1e-4, seed=17)
+    dm0 = torch.tensor(dms, device=cuda)
+    x = x0.clone().requires_grad_(True)
+    dm = dm0.clone().requires_grad_(True)
+    loss = (frac_ddtr.frac_dedisperse(x, dm, plan) ** 2).sum()
+    loss.backward()
+    gdm = dm.grad.double().cpu().numpy()
+
+    x64 = x0.double()
+    dsmax = float(np.max(np.abs(ds)))
+    for b in range(B):
+        h = 0.5 * (dist[b] - 2e-5) / dsmax      # moves every delay by less than half its margin
+        base = float(dms[b])
+        lp = float((ref.frac_ddtr_torch(x64[b:b + 1], torch.tensor([base + h], dtype=torch.float64, device=cuda),
+                                        trials, ds) ** 2).sum())
+        lm = float((ref.frac_ddtr_torch(x64[b:b + 1], torch.tensor([base - h], dtype=torch.float64, device=cuda),
+                                        trials, ds) ** 2).sum())
+        fd = (lp - lm) / (2 * h)
+        assert abs(gdm[b] - fd) <= 1e-4 * abs(fd), (b, gdm[b], fd, h)
+
+    g = torch.Generator(device=cuda).manual_seed(3)
+    v = torch.randn(x0.shape, generator=g, device=cuda, dtype=torch.float64)
+    eps = 1e-3
+    lp = float((ref.frac_ddtr_torch(x64 + eps * v, dm0.double(), trials, ds) ** 2).sum())
+    lm = float((ref.frac_ddtr_torch(x64 - eps * v, dm0.double(), trials, ds) ** 2).sum())
+    fd_dir = (lp - lm) / (2 * eps)
+    got_dir = float((x.grad.double() * v).sum())
+    assert abs(got_dir - fd_dir) <= 1e-4 * abs(fd_dir), (got_dir, fd_dir)
+
+
+@pytest.mark.gpu
+def test_full_candidate_window_shape(cuda):
+    """4096 channels x 1024 samples x 64 DM offsets (two candidates of the
+    256-window batch): forward and both adjoints against the reference."""
+    import torch
+    import frac_ddtr
+    B, C, T, K = 2, 4096, 1024, 64
+    ds = band_delays(C)
+    trials = np.linspace(-2.0, 2.0, K).astype(np.float32)
+    plan = frac_ddtr.FracPlan(ds, trials, T, max_batch=256)
+    x, r, _ = frac_ddtr.synth_candidates(B, C, T, ds, seed=3, device=cuda)
+    dm = torch.tensor([0.0, 0.25], device=cuda)
+    y = plan.forward(x, dm)
+    gy = (2.0 * y).contiguous()
+    gx, gdm = plan.adjoint(x, gy, dm)
+    x64 = x.double().requires_grad_(True)
+    dm64 = dm.double().requires_grad_(True)
+    y_ref = ref.frac_ddtr_torch(x64, dm64, trials, ds)
+    gx_ref, gdm_ref = torch.autograd.grad((y_ref * gy.double()).sum(), (x64, dm64))
+    assert _relerr(y.cpu(), y_ref.detach().cpu()) <= 1e-5
+    assert _relerr(gx.cpu(), gx_ref.cpu()) <= 1e-5
+    assert _relerr(gdm.cpu(), gdm_ref.cpu()) <= 1e-5
+    # the pulse is recovered at the trial nearest to its residual DM
+    best = y.double().pow(2).sum(-1).argmax(-1).cpu().numpy()
+    for b in range(B):
+        u_best = trials[best[b]] + float(dm[b])
+        assert abs(u_best - float(r[b])) <= 2.0 * (4.0 / (K - 1)) + 1e-6
