@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <vector>
 
 #include "aa_ddtr_hip.h"
 #include "aa_ddtr_plan.hpp"
@@ -11,12 +12,17 @@
 #include "aa_device_info.hpp"
 #include "aa_log.hpp"
 #include "aa_pipeline_api.hpp"
+#include "ddtr_batch.hpp"
 #include "ddtr_engine.hpp"
 
 using namespace astroaccelerate;
 
 struct aa_hip_ddtr {
     std::unique_ptr<aa_hip::ddtr_engine> engine;
+};
+
+struct aa_hip_ddtr_batch {
+    std::unique_ptr<aa_hip::ddtr_batch> batch;
 };
 
 namespace {
@@ -274,6 +280,59 @@ int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_o
     return h->engine->run_range(range, t_processed, d_out, out_pitch, (hipStream_t)stream);
 }
 
+int aa_hip_ddtr_set_range_variant(aa_hip_ddtr *h, int range, int variant) {
+    if (!h || !h->engine) return bad("NULL handle");
+    return h->engine->set_range_variant(range, variant);
+}
+
+int aa_hip_ddtr_autotune_range(aa_hip_ddtr *h, int range, int t_processed, float *d_out, size_t out_pitch,
+                               int reps, void *stream, int *best_variant, float *variant_ms) {
+    if (!h || !h->engine) return bad("NULL handle");
+    return h->engine->autotune_range(range, t_processed, d_out, out_pitch, reps, (hipStream_t)stream, best_variant,
+                                     variant_ms);
+}
+
+int aa_hip_ddtr_batch_create(int nbeams, aa_hip_ddtr *const *beams, aa_hip_ddtr_batch **out) {
+    if (!out) return bad("NULL output handle");
+    *out = nullptr;
+    if (nbeams <= 0 || !beams) return bad("no beams");
+    std::vector<aa_hip::ddtr_engine *> engines((size_t)nbeams);
+    for (int b = 0; b < nbeams; ++b) {
+        if (!beams[b] || !beams[b]->engine) return bad("NULL beam handle");
+        engines[(size_t)b] = beams[b]->engine.get();
+    }
+    aa_hip::ddtr_batch *p = nullptr;
+    const int st = aa_hip::ddtr_batch::create(nbeams, engines.data(), &p);
+    if (st != AA_HIP_SUCCESS) return st;
+    auto *h = new (std::nothrow) aa_hip_ddtr_batch();
+    if (!h) {
+        delete p;
+        return AA_HIP_ERROR_OUT_OF_MEMORY;
+    }
+    h->batch.reset(p);
+    *out = h;
+    return AA_HIP_SUCCESS;
+}
+
+int aa_hip_ddtr_batch_destroy(aa_hip_ddtr_batch *h) {
+    delete h;
+    return AA_HIP_SUCCESS;
+}
+
+int aa_hip_ddtr_batch_execute(aa_hip_ddtr_batch *h, const void *const *d_inputs_tc, const int *t_processed,
+                              float *const *d_out, const size_t *out_pitch, int use_graph, void *stream) {
+    if (!h || !h->batch) return bad("NULL handle");
+    return h->batch->execute(d_inputs_tc, t_processed, d_out, out_pitch, use_graph != 0, (hipStream_t)stream);
+}
+
+int aa_hip_ddtr_batch_stats(const aa_hip_ddtr_batch *h, int *njobs, long *ntiles, int *ndirect) {
+    if (!h || !h->batch) return bad("NULL handle");
+    if (njobs) *njobs = h->batch->njobs();
+    if (ntiles) *ntiles = h->batch->ntiles();
+    if (ndirect) *ndirect = h->batch->ndirect();
+    return AA_HIP_SUCCESS;
+}
+
 int aa_hip_corner_turn_u8(const uint8_t *d_in_tc, int nsamp, int nchans, uint8_t *d_out_ct, size_t out_pitch,
                           void *stream) {
     if (!d_in_tc || !d_out_ct) return bad("NULL buffer");
@@ -303,13 +362,14 @@ int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nch
     if (nchans <= 0 || ndms < 0 || t_processed < 0 || inBin <= 0 || row_len < 0 || (size_t)row_len > pitch ||
         out_pitch < (size_t)t_processed)
         return bad("invalid geometry");
-    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS) return bad("unknown kernel variant");
+    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_C16) return bad("unknown kernel variant");
     const int st = aa_hip::check_delays(h_dmshifts, nchans, ndms, mstartdm, mdmstep, t_processed, row_len);
     if (st != AA_HIP_SUCCESS) return st;
-    return from_hip(aa_hip::dispatch_dedisperse(kernel, d_in, pitch, nchans, d_dmshifts, h_dmshifts, ndms,
-                                                t_processed, mstartdm, mdmstep, inBin, d_out, out_pitch,
-                                                (hipStream_t)stream),
-                    "dedisperse");
+    aa_hip::range_launch rl;
+    const int rs = aa_hip::resolve_launch(kernel, d_in, pitch, nchans, d_dmshifts, h_dmshifts, ndms, t_processed,
+                                          mstartdm, mdmstep, inBin, d_out, out_pitch, aa_hip::lds_choice{-2, 0}, &rl);
+    if (rs != AA_HIP_SUCCESS) return rs;
+    return from_hip(aa_hip::launch_range(rl, (hipStream_t)stream), "dedisperse");
 }
 
 int aa_hip_synth_noise_u8(uint64_t seed, int64_t n, double mean, double sigma, uint8_t *d_out_tc, void *stream) {

@@ -12,6 +12,7 @@
 #include "aa_ddtr_hip.h"
 #include "ddtr_engine.hpp"
 #include "frac_ddtr.h"
+#include "frac_refine.h"
 
 struct aa_hip_frac {
     int device = 0;
@@ -23,6 +24,15 @@ struct aa_hip_frac {
     double* d_part = nullptr;
     size_t part_entries = 0;
     size_t bytes = 0;
+};
+
+struct aa_hip_refine {
+    int device = 0;
+    int ncand = 0;
+    int grid = 0;
+    aa_hip::refine_cand* d_cands = nullptr;
+    int* d_order = nullptr;
+    int* d_counter = nullptr;
 };
 
 namespace {
@@ -209,6 +219,91 @@ int aa_hip_frac_adjoint(aa_hip_frac* h, const float* d_x, const float* d_gy, int
         if (st != AA_HIP_SUCCESS) return st;
     }
     return AA_HIP_SUCCESS;
+}
+
+int aa_hip_refine_create(int device, int ncand, const int64_t* x_offsets, const int64_t* ds_offsets,
+                         const int* nchans, const int* nsamp, aa_hip_refine** out) {
+    if (!out) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL output handle");
+    *out = nullptr;
+    if (ncand <= 0 || !x_offsets || !ds_offsets || !nchans || !nsamp)
+        return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "invalid candidate table");
+    std::vector<aa_hip::refine_cand> cands((size_t)ncand);
+    for (int b = 0; b < ncand; ++b) {
+        if (nchans[b] <= 0 || nchans[b] > aa_hip::kRefineMaxChannels || nsamp[b] <= 0 ||
+            nsamp[b] > aa_hip::kRefineMaxSamples || x_offsets[b] < 0 || ds_offsets[b] < 0)
+            return fail(AA_HIP_ERROR_UNSUPPORTED, "candidate " + std::to_string(b) +
+                                                      ": windows up to " + std::to_string(aa_hip::kRefineMaxChannels) +
+                                                      " channels x " + std::to_string(aa_hip::kRefineMaxSamples) +
+                                                      " samples, non-negative offsets");
+        cands[(size_t)b] = aa_hip::refine_cand{x_offsets[b], ds_offsets[b], nchans[b], nsamp[b]};
+    }
+    // largest windows first (longest work items start early)
+    std::vector<int> order((size_t)ncand);
+    for (int b = 0; b < ncand; ++b) order[(size_t)b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int i, int j) {
+        return (long)cands[(size_t)i].C * cands[(size_t)i].T > (long)cands[(size_t)j].C * cands[(size_t)j].T;
+    });
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(AA_HIP_ERROR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "device out of range");
+    auto* p = new (std::nothrow) aa_hip_refine();
+    if (!p) return fail(AA_HIP_ERROR_OUT_OF_MEMORY, "host allocation");
+    p->device = device;
+    p->ncand = ncand;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    hipError_t e = hipSetDevice(device);
+    int ncu = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    p->grid = std::max(1, ncu) * aa_hip::refine_blocks_per_cu();
+    if (e == hipSuccess) e = hipMalloc(&p->d_cands, sizeof(aa_hip::refine_cand) * (size_t)ncand);
+    if (e == hipSuccess) e = hipMalloc(&p->d_order, sizeof(int) * (size_t)ncand);
+    if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(int));
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_cands, cands.data(), sizeof(aa_hip::refine_cand) * (size_t)ncand, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_order, order.data(), sizeof(int) * (size_t)ncand, hipMemcpyHostToDevice);
+    (void)hipSetDevice(cur);
+    if (e != hipSuccess) {
+        const int st = from_hip(e, "aa_hip_refine_create");
+        aa_hip_refine_destroy(p);
+        return st;
+    }
+    *out = p;
+    return AA_HIP_SUCCESS;
+}
+
+int aa_hip_refine_destroy(aa_hip_refine* h) {
+    if (!h) return AA_HIP_SUCCESS;
+    if (h->d_cands) (void)hipFree(h->d_cands);
+    if (h->d_order) (void)hipFree(h->d_order);
+    if (h->d_counter) (void)hipFree(h->d_counter);
+    delete h;
+    return AA_HIP_SUCCESS;
+}
+
+int aa_hip_refine_run(aa_hip_refine* h, const float* d_x, const float* d_delay_per_dm, float* d_u, float* d_step,
+                      double* d_prev_grad, double* d_loss, double* d_grad, int nsteps, float step_max,
+                      float step_min, void* stream) {
+    if (!h || !d_x || !d_delay_per_dm || !d_u || !d_step || !d_prev_grad || !d_loss || !d_grad)
+        return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL argument");
+    if (nsteps < 0 || !(step_max > 0.0f) || !(step_min > 0.0f) || step_min > step_max)
+        return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "invalid step settings");
+    aa_hip::refine_args a{};
+    a.x = d_x;
+    a.ds = d_delay_per_dm;
+    a.cands = h->d_cands;
+    a.order = h->d_order;
+    a.ncand = h->ncand;
+    a.u = d_u;
+    a.step = d_step;
+    a.prev_grad = d_prev_grad;
+    a.loss = d_loss;
+    a.grad = d_grad;
+    a.nsteps = nsteps;
+    a.step_max = step_max;
+    a.step_min = step_min;
+    a.counter = h->d_counter;
+    return from_hip(aa_hip::launch_refine(a, h->grid, (hipStream_t)stream), "refine");
 }
 
 }  // extern "C"

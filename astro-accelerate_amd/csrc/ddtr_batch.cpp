@@ -1,0 +1,232 @@
+#include "ddtr_batch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <string>
+
+#include "aa_ddtr_hip.h"
+
+namespace aa_hip {
+
+namespace {
+
+int fail(int code, const std::string &msg) {
+    set_last_error(msg);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? AA_HIP_ERROR_OUT_OF_MEMORY : AA_HIP_ERROR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct device_guard {
+    int prev = -1;
+    explicit device_guard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    }
+    ~device_guard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+int ddtr_batch::create(int nbeams, ddtr_engine *const *beams, ddtr_batch **out) {
+    if (!out) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    if (nbeams <= 0 || !beams) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "no beams");
+    for (int b = 0; b < nbeams; ++b) {
+        if (!beams[b]) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "beam " + std::to_string(b) + " is NULL");
+        if (beams[b]->device() != beams[0]->device())
+            return fail(AA_HIP_ERROR_UNSUPPORTED, "all beams of a batch must be on one device");
+    }
+    auto *p = new (std::nothrow) ddtr_batch();
+    if (!p) return fail(AA_HIP_ERROR_OUT_OF_MEMORY, "host allocation");
+    p->m_device = beams[0]->device();
+    p->m_beams.assign(beams, beams + nbeams);
+    for (auto *e : p->m_beams) p->m_nranges_total += e->nranges();
+    device_guard guard(p->m_device);
+    int ncu = 0;
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->m_device);
+    // one 1024-thread work-group per CU (its LDS use fills the CU)
+    p->m_grid = 8 * std::max(1, (ncu + 7) / 8);
+    if (e == hipSuccess) e = hipMalloc(&p->m_d_bounds, 9 * sizeof(long));
+    if (e == hipSuccess) e = hipMalloc(&p->m_d_counters, 8 * sizeof(int));
+    if (e != hipSuccess) {
+        const int st = hip_fail(e, "batch create");
+        delete p;
+        return st;
+    }
+    *out = p;
+    return AA_HIP_SUCCESS;
+}
+
+ddtr_batch::~ddtr_batch() {
+    device_guard guard(m_device);
+    release_graph();
+    if (m_d_jobs) (void)hipFree(m_d_jobs);
+    if (m_d_bounds) (void)hipFree(m_d_bounds);
+    if (m_d_counters) (void)hipFree(m_d_counters);
+    if (m_capture_stream) (void)hipStreamDestroy(m_capture_stream);
+}
+
+void ddtr_batch::release_graph() {
+    if (m_exec) (void)hipGraphExecDestroy(m_exec);
+    if (m_graph) (void)hipGraphDestroy(m_graph);
+    m_exec = nullptr;
+    m_graph = nullptr;
+    m_key.clear();
+}
+
+// Launches ingest and binning of every beam, resolves every range, and the
+// batched dedispersion.  Builds m_jobs; the caller uploads them (outside any
+// graph capture) before the batched kernel runs.
+int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float *const *d_out,
+                        const size_t *out_pitch, hipStream_t s) {
+    std::vector<lds_job> jobs;
+    std::vector<range_launch> direct;
+    int flat = 0;
+    for (size_t b = 0; b < m_beams.size(); ++b) {
+        ddtr_engine *e = m_beams[b];
+        int st = e->ingest(inputs[b], 8, t_processed[flat], s);
+        if (st != AA_HIP_SUCCESS) return st;
+        for (int r = 0; r < e->nranges(); ++r, ++flat) {
+            range_launch rl;
+            st = e->prepare_range(r, t_processed[flat], d_out[flat], out_pitch[flat], s, &rl);
+            if (st != AA_HIP_SUCCESS) return st;
+            if (rl.ndms <= 0 || rl.t_processed <= 0) continue;
+            if (rl.variant >= 0) {
+                lds_job j{};
+                j.p = lds_params_of(rl);
+                j.variant = rl.variant;
+                jobs.push_back(j);
+            } else {
+                direct.push_back(rl);
+            }
+        }
+    }
+    // Largest tiles (most channels) first: the costliest work starts early
+    // and the queue tails are made of the cheapest tiles.
+    std::stable_sort(jobs.begin(), jobs.end(),
+                     [](const lds_job &a, const lds_job &b) { return a.p.nchans > b.p.nchans; });
+    long begin = 0;
+    for (auto &j : jobs) {
+        j.tile_begin = begin;
+        begin += (long)j.p.n_dm_tiles * j.p.n_t_tiles;
+    }
+    m_jobs.swap(jobs);
+    m_direct.swap(direct);
+    m_ntiles = begin;
+    return AA_HIP_SUCCESS;
+}
+
+int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float *const *d_out,
+                        const size_t *out_pitch, bool use_graph, hipStream_t s) {
+    if (!inputs || !t_processed || !d_out || !out_pitch) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL array");
+    device_guard guard(m_device);
+    std::vector<uintptr_t> key;
+    key.reserve(m_beams.size() + 3 * (size_t)m_nranges_total + 1);
+    key.push_back((uintptr_t)s);
+    for (size_t b = 0; b < m_beams.size(); ++b) key.push_back((uintptr_t)inputs[b]);
+    for (int i = 0; i < m_nranges_total; ++i) {
+        key.push_back((uintptr_t)d_out[i]);
+        key.push_back((uintptr_t)out_pitch[i]);
+        key.push_back((uintptr_t)(unsigned)t_processed[i]);
+    }
+    if (use_graph && m_exec && key == m_key) {
+        const hipError_t e = hipGraphLaunch(m_exec, s);
+        return e == hipSuccess ? AA_HIP_SUCCESS : hip_fail(e, "batch graph launch");
+    }
+    release_graph();  // the eager pass rewrites the job table a captured graph reads
+
+    // Eager pass: builds and uploads the job list, and computes this call.
+    int st = enqueue(inputs, t_processed, d_out, out_pitch, s);
+    if (st != AA_HIP_SUCCESS) return st;
+    if (m_jobs.size() > m_jobs_cap) {
+        if (m_d_jobs) (void)hipFree(m_d_jobs);
+        m_d_jobs = nullptr;
+        m_jobs_cap = 0;
+        const hipError_t e = hipMalloc(&m_d_jobs, m_jobs.size() * sizeof(lds_job));
+        if (e != hipSuccess) return hip_fail(e, "batch job table");
+        m_jobs_cap = m_jobs.size();
+    }
+    // Queue bounds: 8 contiguous slices of the tile list of about equal work
+    // (tiles weighted by channel count).
+    std::vector<long> bounds(9, m_ntiles);
+    {
+        double total = 0.0;
+        for (const auto &j : m_jobs) total += (double)j.p.nchans * j.p.n_dm_tiles * j.p.n_t_tiles;
+        bounds[0] = 0;
+        double acc = 0.0;
+        int x = 1;
+        for (const auto &j : m_jobs) {
+            const long n = (long)j.p.n_dm_tiles * j.p.n_t_tiles;
+            const double w = (double)j.p.nchans;
+            while (x < 8 && acc + w * n >= total * x / 8.0) {
+                const long cut = (long)std::ceil((total * x / 8.0 - acc) / w);
+                bounds[(size_t)x] = j.tile_begin + std::min(std::max(cut, 0L), n);
+                ++x;
+            }
+            acc += w * n;
+        }
+        for (; x < 8; ++x) bounds[(size_t)x] = m_ntiles;
+        for (int i = 1; i < 9; ++i) bounds[(size_t)i] = std::max(bounds[(size_t)i], bounds[(size_t)i - 1]);
+    }
+    hipError_t e = hipSuccess;
+    if (!m_jobs.empty()) {
+        e = hipMemcpyAsync(m_d_jobs, m_jobs.data(), m_jobs.size() * sizeof(lds_job), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(m_d_bounds, bounds.data(), 9 * sizeof(long), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);  // host vectors may change before the copies run
+        if (e == hipSuccess)
+            e = launch_dedisperse_lds_batch(m_d_jobs, (int)m_jobs.size(), m_d_bounds, m_d_counters, m_grid, s);
+    }
+    for (size_t i = 0; e == hipSuccess && i < m_direct.size(); ++i) e = launch_range(m_direct[i], s);
+    if (e != hipSuccess) return hip_fail(e, "batch launch");
+    if (!use_graph) return AA_HIP_SUCCESS;
+
+    // Capture the same sequence (the job table is already on the device and
+    // is the same for these buffers); later calls replay it.
+    // The legacy default stream cannot be captured: capture on a stream of
+    // the batch's own (the graph is launched on the caller's stream).
+    hipStream_t cs = s;
+    if (!cs) {
+        if (!m_capture_stream) {
+            e = hipStreamCreateWithFlags(&m_capture_stream, hipStreamNonBlocking);
+            if (e != hipSuccess) return hip_fail(e, "batch capture stream");
+        }
+        e = hipStreamSynchronize(s);  // the eager pass above ran on s
+        if (e != hipSuccess) return hip_fail(e, "batch capture");
+        cs = m_capture_stream;
+    }
+    const size_t njobs = m_jobs.size();
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return hip_fail(e, "batch capture");
+    st = enqueue(inputs, t_processed, d_out, out_pitch, cs);
+    if (st == AA_HIP_SUCCESS && m_jobs.size() != njobs) st = fail(AA_HIP_ERROR_DEVICE, "batch changed in capture");
+    if (st == AA_HIP_SUCCESS && njobs)
+        e = launch_dedisperse_lds_batch(m_d_jobs, (int)njobs, m_d_bounds, m_d_counters, m_grid, cs);
+    for (size_t i = 0; st == AA_HIP_SUCCESS && e == hipSuccess && i < m_direct.size(); ++i)
+        e = launch_range(m_direct[i], cs);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &g);
+    if (st != AA_HIP_SUCCESS) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e == hipSuccess) e = ec;
+    if (e == hipSuccess) e = hipGraphInstantiate(&m_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        m_exec = nullptr;
+        return hip_fail(e, "batch graph");
+    }
+    m_graph = g;
+    m_key.swap(key);
+    return AA_HIP_SUCCESS;
+}
+
+}  // namespace aa_hip

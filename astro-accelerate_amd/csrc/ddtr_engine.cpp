@@ -71,34 +71,75 @@ int check_delays(const float *dmshifts, int nchans, int ndms, float mstartdm, fl
     return AA_HIP_SUCCESS;
 }
 
-hipError_t dispatch_dedisperse(int kernel, const uint8_t *L, size_t pitch, int nchans,
-                               const float *d_dmshifts, const float *h_dmshifts, int ndms,
-                               int t_processed, float mstartdm, float mdmstep, int inBin,
-                               float *out, size_t out_pitch, hipStream_t s, int row_quads) {
-    if (kernel != AA_HIP_KERNEL_DIRECT && (pitch % 16) == 0 && ((uintptr_t)L % 16) == 0) {
-        if (row_quads == 0) row_quads = lds_row_quads(h_dmshifts, nchans, ndms, mstartdm, mdmstep);
-        if (row_quads > 0) {
-            lds_params p{};
-            p.L = L;
-            p.pitch = pitch;
-            p.nchans = nchans;
-            p.dmshifts = d_dmshifts;
-            p.ndms = ndms;
-            p.t_processed = t_processed;
-            p.mstartdm = mstartdm;
-            p.mdmstep = mdmstep;
-            p.fnchans = (float)nchans;
-            p.fbin = (float)inBin;
-            p.out = out;
-            p.out_pitch = out_pitch;
-            p.row_quads = row_quads;
-            return launch_dedisperse_lds(p, s);
+lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant) {
+    lds_choice c{-1, -1};
+    for (int v = 0; v < kLdsVariants; ++v) {
+        if (variant >= 0 && v != variant) continue;
+        const int q = lds_row_quads(v, h_dmshifts, nchans, ndms, mstartdm, mdmstep);
+        if (q > 0) {
+            c.variant = v;
+            c.row_quads = q;
+            return c;
         }
-        if (kernel == AA_HIP_KERNEL_LDS) return hipErrorInvalidValue;  // asked for LDS, geometry does not fit
     }
-    if (kernel == AA_HIP_KERNEL_LDS) return hipErrorInvalidValue;
-    return launch_dedisperse_direct_u8(L, pitch, nchans, d_dmshifts, ndms, t_processed, mstartdm,
-                                       mdmstep, inBin, out, out_pitch, s);
+    return c;
+}
+
+int resolve_launch(int kernel, const uint8_t *L, size_t pitch, int nchans, const float *d_dmshifts,
+                   const float *h_dmshifts, int ndms, int t_processed, float mstartdm, float mdmstep, int inBin,
+                   float *out, size_t out_pitch, lds_choice choice, range_launch *rl) {
+    *rl = range_launch{};
+    rl->L = L;
+    rl->pitch = pitch;
+    rl->nchans = nchans;
+    rl->d_dmshifts = d_dmshifts;
+    rl->ndms = ndms;
+    rl->t_processed = t_processed;
+    rl->mstartdm = mstartdm;
+    rl->mdmstep = mdmstep;
+    rl->inBin = inBin;
+    rl->out = out;
+    rl->out_pitch = out_pitch;
+    rl->variant = -1;
+    const int forced = kernel == AA_HIP_KERNEL_LDS_C32 ? 0 : kernel == AA_HIP_KERNEL_LDS_C16 ? 1 : -1;
+    if (kernel != AA_HIP_KERNEL_DIRECT && (pitch % 16) == 0 && ((uintptr_t)L % 16) == 0) {
+        if (forced >= 0) choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, forced);
+        else if (choice.variant == -2) choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, -1);
+        if (choice.variant >= 0) {
+            rl->variant = choice.variant;
+            rl->row_quads = choice.row_quads;
+            return AA_HIP_SUCCESS;
+        }
+    }
+    if (kernel == AA_HIP_KERNEL_LDS || forced >= 0)
+        return fail(AA_HIP_ERROR_UNSUPPORTED, "the LDS kernel does not fit this geometry (delay spread per tile "
+                                              "or row alignment)");
+    return AA_HIP_SUCCESS;
+}
+
+lds_params lds_params_of(const range_launch &rl) {
+    lds_params p{};
+    p.L = rl.L;
+    p.pitch = rl.pitch;
+    p.nchans = rl.nchans;
+    p.dmshifts = rl.d_dmshifts;
+    p.ndms = rl.ndms;
+    p.t_processed = rl.t_processed;
+    p.mstartdm = rl.mstartdm;
+    p.mdmstep = rl.mdmstep;
+    p.fnchans = (float)rl.nchans;
+    p.fbin = (float)rl.inBin;
+    p.out = rl.out;
+    p.out_pitch = rl.out_pitch;
+    p.row_quads = rl.row_quads;
+    lds_job_tiles(p);
+    return p;
+}
+
+hipError_t launch_range(const range_launch &rl, hipStream_t s) {
+    if (rl.variant >= 0) return launch_dedisperse_lds(lds_params_of(rl), rl.variant, s);
+    return launch_dedisperse_direct_u8(rl.L, rl.pitch, rl.nchans, rl.d_dmshifts, rl.ndms, rl.t_processed,
+                                       rl.mstartdm, rl.mdmstep, rl.inBin, rl.out, rl.out_pitch, s);
 }
 
 int ddtr_engine::create(int device, int nchans, const float *dmshifts, int nranges,
@@ -176,7 +217,7 @@ ddtr_engine::~ddtr_engine() {
 }
 
 int ddtr_engine::set_kernel(int kernel) {
-    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS)
+    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_C16)
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "unknown kernel variant");
     m_kernel = kernel;
     return AA_HIP_SUCCESS;
@@ -238,7 +279,8 @@ int ddtr_engine::ensure_level(int k, hipStream_t s) {
     return AA_HIP_SUCCESS;
 }
 
-int ddtr_engine::run_range(int r, int t_processed, float *d_out, size_t out_pitch, hipStream_t s) {
+int ddtr_engine::prepare_range(int r, int t_processed, float *d_out, size_t out_pitch, hipStream_t s,
+                               range_launch *rl) {
     if (r < 0 || r >= (int)m_ranges.size()) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "bad range index");
     if (!d_out || out_pitch < (size_t)std::max(t_processed, 0))
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "bad output buffer");
@@ -258,12 +300,74 @@ int ddtr_engine::run_range(int r, int t_processed, float *d_out, size_t out_pitc
     range_scalars(r, &mstartdm, &mdmstep);
     st = check_delays(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, t_processed, i_nsamp);
     if (st != AA_HIP_SUCCESS) return st;
-    if (m_row_quads.size() != m_ranges.size()) m_row_quads.assign(m_ranges.size(), 0);
-    if (m_row_quads[(size_t)r] == 0)
-        m_row_quads[(size_t)r] = lds_row_quads(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep);
-    hipError_t e = dispatch_dedisperse(m_kernel, lv.ptr, lv.pitch, m_nchans, m_d_dmshifts, m_dmshifts.data(),
-                                       rd.ndms, t_processed, mstartdm, mdmstep, rd.inBin, d_out, out_pitch, s,
-                                       m_row_quads[(size_t)r]);
+    if (m_lds.size() != m_ranges.size()) m_lds.assign(m_ranges.size(), lds_choice{-2, 0});
+    lds_choice &c = m_lds[(size_t)r];
+    if (c.variant == -2) {
+        const int want = r < (int)m_variant_pref.size() ? m_variant_pref[(size_t)r] : -1;
+        c = choose_lds(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, want);
+        if (c.variant < 0 && want >= 0)  // preferred shape does not fit: any that does
+            c = choose_lds(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, -1);
+    }
+    return resolve_launch(m_kernel, lv.ptr, lv.pitch, m_nchans, m_d_dmshifts, m_dmshifts.data(), rd.ndms,
+                          t_processed, mstartdm, mdmstep, rd.inBin, d_out, out_pitch, c, rl);
+}
+
+int ddtr_engine::set_range_variant(int r, int variant) {
+    if (r < 0 || r >= (int)m_ranges.size()) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "bad range index");
+    if (variant < -1 || variant >= kLdsVariants) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "bad LDS variant");
+    if (m_variant_pref.size() != m_ranges.size()) m_variant_pref.assign(m_ranges.size(), -1);
+    m_variant_pref[(size_t)r] = variant;
+    if (m_lds.size() == m_ranges.size()) m_lds[(size_t)r] = lds_choice{-2, 0};
+    return AA_HIP_SUCCESS;
+}
+
+int ddtr_engine::autotune_range(int r, int t_processed, float *d_out, size_t out_pitch, int reps, hipStream_t s,
+                                int *best, float *ms) {
+    if (!best || !ms || reps <= 0) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "bad autotune arguments");
+    if (r < 0 || r >= (int)m_ranges.size()) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "bad range index");
+    device_guard guard(m_device);
+    const int saved_kernel = m_kernel;
+    m_kernel = AA_HIP_KERNEL_AUTO;
+    *best = -1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    int st = AA_HIP_SUCCESS;
+    float best_ms = 0.0f;
+    for (int v = 0; v < kLdsVariants && e == hipSuccess && st == AA_HIP_SUCCESS; ++v) {
+        ms[v] = -1.0f;
+        st = set_range_variant(r, v);
+        range_launch rl;
+        if (st == AA_HIP_SUCCESS) st = prepare_range(r, t_processed, d_out, out_pitch, s, &rl);
+        if (st != AA_HIP_SUCCESS || rl.variant != v) continue;
+        e = launch_range(rl, s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch_range(rl, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) break;
+        ms[v] = t / (float)reps;
+        if (*best < 0 || ms[v] < best_ms) {
+            *best = v;
+            best_ms = ms[v];
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    m_kernel = saved_kernel;
+    if (st == AA_HIP_SUCCESS && e != hipSuccess) st = hip_fail(e, "autotune");
+    if (st == AA_HIP_SUCCESS) st = set_range_variant(r, *best);
+    return st;
+}
+
+int ddtr_engine::run_range(int r, int t_processed, float *d_out, size_t out_pitch, hipStream_t s) {
+    range_launch rl;
+    const int st = prepare_range(r, t_processed, d_out, out_pitch, s, &rl);
+    if (st != AA_HIP_SUCCESS) return st;
+    device_guard guard(m_device);
+    const hipError_t e = launch_range(rl, s);
     if (e != hipSuccess) return hip_fail(e, "dedisperse");
     return AA_HIP_SUCCESS;
 }

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "ddtr_kernels.h"
+#include "ddtr_lds.h"
 
 namespace aa_hip {
 
@@ -34,6 +35,30 @@ struct range_desc {
 // Thread-local message of the most recent failure.
 void set_last_error(const std::string &msg);
 const char *last_error();
+
+// LDS chunk shape for a range and the staged row width it needs; variant -1
+// when no shape fits (the direct kernel then runs), -2 = not yet chosen.
+struct lds_choice {
+    int variant;
+    int row_quads;
+};
+// First shape (or only `variant` when >= 0) whose rows hold the range's windows.
+lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant);
+
+// One range, resolved to a kernel: variant >= 0 = LDS kernel with that chunk
+// shape, -1 = direct kernel.
+struct range_launch {
+    const uint8_t *L = nullptr;
+    size_t pitch = 0;
+    int nchans = 0;
+    const float *d_dmshifts = nullptr;
+    int ndms = 0, t_processed = 0, inBin = 1;
+    float mstartdm = 0.0f, mdmstep = 0.0f;
+    float *out = nullptr;
+    size_t out_pitch = 0;
+    int variant = -1;
+    int row_quads = 0;
+};
 
 class ddtr_engine {
 public:
@@ -64,6 +89,19 @@ public:
     // since the last call (and clears the flag), 0 if not, <0 on error.
     int take_input_overflow(hipStream_t s);
     int run_range(int r, int t_processed, float *d_out, size_t out_pitch, hipStream_t s);
+    // Everything run_range does before the launch (binning up to the range's
+    // level on `s`, validation, kernel choice); the launch itself is left to
+    // the caller (launch_range, or a batched launch).
+    int prepare_range(int r, int t_processed, float *d_out, size_t out_pitch, hipStream_t s, range_launch *rl);
+    // LDS chunk shape preferred for range r (-1 = first that fits); used when
+    // it fits the range's delay spread.
+    int set_range_variant(int r, int variant);
+    // Times every LDS chunk shape that fits range r (reps launches each, after
+    // one untimed launch; output written to d_out) and keeps the fastest as
+    // the range's preference.  ms[v] = average launch time of shape v, or -1
+    // when it does not fit.  *best = -1 when no shape fits.
+    int autotune_range(int r, int t_processed, float *d_out, size_t out_pitch, int reps, hipStream_t s,
+                       int *best, float *ms);
     int execute_chunk(const void *d_input_tc, int input_bits, const int *t_processed,
                       float *const *d_out, const size_t *out_pitch, hipStream_t s);
 
@@ -94,7 +132,8 @@ private:
     int *m_d_err = nullptr;
     std::vector<level> m_levels;
     int m_resident = -1;  // highest level valid for the current chunk
-    std::vector<int> m_row_quads;  // per range: LDS row width, -1 = does not fit, 0 = not computed
+    std::vector<lds_choice> m_lds;     // per range: chosen LDS shape (-2 = not computed, -1 = direct)
+    std::vector<int> m_variant_pref;   // per range: preferred LDS shape, -1 = any
     size_t m_device_bytes = 0;
 };
 
@@ -110,10 +149,13 @@ inline int host_delay(float dmshift, int d, float mstartdm, float mdmstep) {
     return (int)prod;
 }
 
-// Dispatch of one range to a kernel variant (no validation).
-hipError_t dispatch_dedisperse(int kernel, const uint8_t *L, size_t pitch, int nchans,
-                               const float *d_dmshifts, const float *h_dmshifts, int ndms,
-                               int t_processed, float mstartdm, float mdmstep, int inBin,
-                               float *out, size_t out_pitch, hipStream_t s, int row_quads = 0);
+// Resolves the kernel for one range (no launch).  choice.variant == -2 picks
+// the first LDS shape that fits.  Fails with UNSUPPORTED if kernel is
+// AA_HIP_KERNEL_LDS and no shape fits.
+int resolve_launch(int kernel, const uint8_t *L, size_t pitch, int nchans, const float *d_dmshifts,
+                   const float *h_dmshifts, int ndms, int t_processed, float mstartdm, float mdmstep, int inBin,
+                   float *out, size_t out_pitch, lds_choice choice, range_launch *rl);
+lds_params lds_params_of(const range_launch &rl);
+hipError_t launch_range(const range_launch &rl, hipStream_t s);
 
 }  // namespace aa_hip

@@ -7,9 +7,12 @@
 
 namespace aa_hip {
 
-// Largest staged row (uint4 units) that fits two 32-channel buffers plus the
-// offset tables in 160 KiB of LDS.
+// Largest staged row (uint4 units) of the 32-channel chunk shape: two
+// 32-channel buffers plus the offset tables fill 160 KiB of LDS.  The
+// 16-channel shape holds rows twice as long in the same buffers.
 constexpr int kLdsRowQuadsMax = 152;
+// Chunk shapes: 0 = 32 channels x 152 quads, 1 = 16 channels x 304 quads.
+constexpr int kLdsVariants = 2;
 
 struct lds_params {
     const uint8_t* L;        // nchans rows of u8 samples
@@ -23,17 +26,39 @@ struct lds_params {
     float* out;
     size_t out_pitch;        // floats between output rows
     int row_quads;           // staged row width in uint4 (from lds_row_quads)
-    int n_dm_tiles, n_t_tiles;  // filled by the launcher
+    int n_dm_tiles, n_t_tiles;  // filled by the launcher / lds_job_tiles
 };
 
-// Staged row width (uint4) this geometry needs, or -1 when it exceeds LDS.
-// Host only; evaluates the delay rule exactly as the kernel does.
-int lds_row_quads(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep);
+// One range of a batched launch: its tiles are items tile_begin ..
+// tile_begin + n_dm_tiles * n_t_tiles - 1 of the batch's tile list.
+struct lds_job {
+    lds_params p;
+    int variant;             // chunk shape (0 or 1)
+    int pad;
+    long tile_begin;
+};
 
-// Samples per time tile; rows must hold this many readable bytes past the
-// last processed sample plus the maximum delay (the engine pads by 4 KiB).
+// Staged row width (uint4) this geometry needs with chunk shape `variant`,
+// or -1 when it exceeds that shape's rows.  Host only; evaluates the delay
+// rule exactly as the kernel does.
+int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep);
+
+// Tile geometry: samples per tile and DM trials per tile.  Rows must hold
+// lds_tile_samples() readable bytes past the last processed sample plus the
+// maximum delay (the engine pads by 4 KiB).
 long lds_tile_samples();
+int lds_tile_dms();
 
-hipError_t launch_dedisperse_lds(const lds_params& p, hipStream_t s);
+hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s);
+
+// Fills p.n_dm_tiles and p.n_t_tiles.
+void lds_job_tiles(lds_params& p);
+
+// Persistent launch over jobs[0..njobs) (device array, sorted by tile_begin,
+// the first at 0); d_bounds[0..8] splits the tile list into 8 queues (one per
+// XCD), d_counters[0..7] is scratch (reset by the launch itself, with a
+// kernel, so the sequence may be captured in a graph).  grid: multiple of 8.
+hipError_t launch_dedisperse_lds_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
+                                       int grid, hipStream_t s);
 
 }  // namespace aa_hip

@@ -30,23 +30,36 @@ namespace {
 constexpr int kR = 32;              // samples per lane
 constexpr int kT = 64 * kR;         // samples per tile
 constexpr int kDms = 16;            // DM trials per work-group (= waves)
-constexpr int kC = 32;              // channels per LDS chunk
 constexpr int kThreads = 64 * kDms; // 1024
-constexpr int kFlushChunks = 256 / kC;
-// A staged row is kLdsRowQuadsMax uint4: the even quads of the row's window
-// in the first half, the odd quads in the second half (kHalf apart).  A lane
-// run starts at quad E + 2*lane; every lane has the parity of E, so each of
-// the run's three reads is a stride-1 access across the wave (no LDS bank
-// conflicts), where a plain layout would be stride 2.
-constexpr int kHalf = kLdsRowQuadsMax / 2;
+
+// Channel-chunk shape: KC channels per LDS chunk, staged rows of up to RQ
+// uint4.  A staged row holds the even quads of the row's window in its first
+// half and the odd quads in its second half (kHalf apart).  A lane run
+// starts at quad E + 2*lane; every lane has the parity of E, so each of the
+// run's three reads is a stride-1 access across the wave (no LDS bank
+// conflicts), where a plain layout would be stride 2.  Both shapes fill the
+// same LDS buffers: 32 channels x 152 quads, or 16 channels x 304 quads for
+// plans whose tiles need a longer window (larger delay spread).
+template <int KC, int RQ>
+struct lds_shape {
+    static constexpr int kC = KC;
+    static constexpr int kRowQuads = RQ;
+    static constexpr int kHalf = RQ / 2;
+    static constexpr int kFlushChunks = 256 / KC;  // 16-bit fields hold 256 * 255
+};
+using shape_c32 = lds_shape<32, kLdsRowQuadsMax>;
+using shape_c16 = lds_shape<16, 2 * kLdsRowQuadsMax>;
+constexpr int kBufQuads = 32 * kLdsRowQuadsMax;
+constexpr int kOffsMax = kDms * 32;
+static_assert(shape_c16::kC * shape_c16::kRowQuads == kBufQuads, "shapes share the LDS buffers");
 
 // Two separate LDS arrays (not one [2][...] array), so that the compiler can
 // tell the buffer being summed from the buffer being filled by the
 // direct-to-LDS copies and does not wait for those copies before each read.
-__shared__ uint4 g_rows0[kC * kLdsRowQuadsMax];
-__shared__ uint4 g_rows1[kC * kLdsRowQuadsMax];
-__shared__ int g_offs0[kDms * kC];
-__shared__ int g_offs1[kDms * kC];
+__shared__ uint4 g_rows0[kBufQuads];
+__shared__ uint4 g_rows1[kBufQuads];
+__shared__ int g_offs0[kOffsMax];
+__shared__ int g_offs1[kOffsMax];
 
 __device__ __forceinline__ int dev_delay(float dmshift, int d, float mstartdm, float mdmstep)
 {
@@ -91,8 +104,10 @@ __device__ __forceinline__ void accumulate_one(uint32_t (&acc)[kR / 2], const ui
 // 2k+1 at row[kHalf + k].
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+template <class S>
 __device__ __forceinline__ void load_run(const uint4* row, int E, int lane, uint32_t (&w)[12])
 {
+    constexpr int kHalf = S::kHalf;
     // Offsets as integer arithmetic on the parity p (a select between two
     // pointers would hide the LDS array from the compiler's alias analysis
     // and make it wait for the in-flight copies before these reads).
@@ -152,8 +167,10 @@ struct tile_ctx {
 // w + 16 of the chunk, 64 uint4 per instruction.  The row base comes from
 // a scalar load of the row's dmshift, so no vector load precedes the copies
 // (vmcnt is in order: waiting on such a load would wait on the copies too).
+template <class S>
 __device__ __forceinline__ void stage_chunk(const lds_params& p, const tile_ctx& t, int chunk, uint4* rows)
 {
+    constexpr int kC = S::kC, kHalf = S::kHalf;
     const int c0 = chunk * kC;
     const int nrows = (p.nchans - c0) < kC ? (p.nchans - c0) : kC;
     for (int row = t.wave; row < nrows; row += kDms) {
@@ -161,7 +178,7 @@ __device__ __forceinline__ void stage_chunk(const lds_params& p, const tile_ctx&
         const float ds = t.dmshifts[c];
         const int b16 = __builtin_amdgcn_readfirstlane(dev_delay(ds, t.d0, p.mstartdm, p.mdmstep) & ~15);
         const uint8_t* src = t.L + (size_t)c * p.pitch + (size_t)t.t0 + (size_t)b16;
-        uint4* dst = rows + row * kLdsRowQuadsMax;
+        uint4* dst = rows + row * S::kRowQuads;
         const int n_even = (t.Wq + 1) >> 1, n_odd = t.Wq >> 1;
         for (int j = 0; j < n_even; j += 64) {
             if (j + t.lane < n_even)
@@ -176,8 +193,10 @@ __device__ __forceinline__ void stage_chunk(const lds_params& p, const tile_ctx&
 }
 
 // Offset e = delay(c, d0 + wd) - floor16(delay(c, d0)) of (wd, ci) = (tid / kC, tid % kC).
+template <class S>
 __device__ __forceinline__ int chunk_offset(const lds_params& p, const tile_ctx& t, int chunk)
 {
+    constexpr int kC = S::kC;
     int e = 0;
     if (t.tid < kDms * kC) {
         const int wd = t.tid / kC, ci = t.tid % kC;
@@ -194,7 +213,7 @@ __device__ __forceinline__ int chunk_offset(const lds_params& p, const tile_ctx&
 // All channel pairs of the chunk whose two runs have dword shifts (QA, QB):
 // bit 2p of `m` set means pair (2p, 2p+1), whose offsets sit in lanes 2p and
 // 2p+1 of e_v.  A plain loop, so the accumulators stay in their registers.
-template <int QA, int QB>
+template <class S, int QA, int QB>
 __device__ __forceinline__ void sum_group(uint64_t m, int e_v, const uint4* rows, int lane, uint32_t (&acc16)[kR / 2])
 {
     while (m) {
@@ -203,39 +222,41 @@ __device__ __forceinline__ void sum_group(uint64_t m, int e_v, const uint4* rows
         const int ea = __builtin_amdgcn_readlane(e_v, l);
         const int eb = __builtin_amdgcn_readlane(e_v, l + 1);
         uint32_t wa[12], wb[12];
-        load_run(rows + l * kLdsRowQuadsMax, ea >> 4, lane, wa);
-        load_run(rows + (l + 1) * kLdsRowQuadsMax, eb >> 4, lane, wb);
+        load_run<S>(rows + l * S::kRowQuads, ea >> 4, lane, wa);
+        load_run<S>(rows + (l + 1) * S::kRowQuads, eb >> 4, lane, wb);
         const uint32_t sa0 = pair_selector(ea & 3), sa1 = sa0 + 0x00020002u;
         const uint32_t sb0 = pair_selector(eb & 3), sb1 = sb0 + 0x00020002u;
         accumulate_pair<QA, QB>(acc16, wa, wb, sa0, sa1, sb0, sb1);
     }
 }
 
-template <int V>
+template <class S, int V>
 __device__ __forceinline__ void sum_groups(int variant, bool voter, int e_v, const uint4* rows, int lane,
                                            uint32_t (&acc16)[kR / 2])
 {
     const uint64_t m = __builtin_amdgcn_ballot_w64(voter && variant == V);
-    sum_group<V / 4, V % 4>(m, e_v, rows, lane, acc16);
-    if constexpr (V + 1 < 16) sum_groups<V + 1>(variant, voter, e_v, rows, lane, acc16);
+    sum_group<S, V / 4, V % 4>(m, e_v, rows, lane, acc16);
+    if constexpr (V + 1 < 16) sum_groups<S, V + 1>(variant, voter, e_v, rows, lane, acc16);
 }
 
 // Sums one staged chunk into the wave's packed accumulators.  Lane l holds
 // the offset of chunk channel l; pairs are grouped by their (dword shift a,
 // dword shift b) variant with one ballot per variant.
+template <class S>
 __device__ __forceinline__ void sum_chunk(const lds_params& p, const tile_ctx& t, int chunk, const uint4* rows,
                                           const int* offs, uint32_t (&acc16)[kR / 2])
 {
+    constexpr int kC = S::kC;
     const int nrows = (p.nchans - chunk * kC) < kC ? (p.nchans - chunk * kC) : kC;
     const int e_v = t.lane < nrows ? offs[t.wave * kC + t.lane] : 0;
     const int q = (e_v >> 2) & 3;
     const int q_next = __shfl_down(q, 1, 64);
     const bool voter = ((t.lane & 1) == 0) && (t.lane + 1 < nrows);
-    sum_groups<0>(q * 4 + q_next, voter, e_v, rows, t.lane, acc16);
+    sum_groups<S, 0>(q * 4 + q_next, voter, e_v, rows, t.lane, acc16);
     if (nrows & 1) {  // odd channel count in the last chunk: its last channel alone
         const int ea = __builtin_amdgcn_readlane(e_v, nrows - 1);
         uint32_t wa[12];
-        load_run(rows + (nrows - 1) * kLdsRowQuadsMax, ea >> 4, t.lane, wa);
+        load_run<S>(rows + (nrows - 1) * S::kRowQuads, ea >> 4, t.lane, wa);
         const uint32_t sa0 = pair_selector(ea & 3), sa1 = sa0 + 0x00020002u;
         switch ((ea >> 2) & 3) {
             case 0: accumulate_one<0>(acc16, wa, sa0, sa1); break;
@@ -259,49 +280,46 @@ __device__ __forceinline__ void flush(uint32_t (&acc16)[kR / 2], uint32_t (&acc3
 // One pipeline step: sum chunk `chunk` from (rows_cur, offs_cur) while the
 // copy of chunk + 1 into rows_next is in flight; publish its offsets, wait
 // for the copies, barrier.
+template <class S>
 __device__ __forceinline__ void pipeline_step(const lds_params& p, const tile_ctx& t, int chunk, int nchunks,
                                               bool active, const uint4* rows_cur, const int* offs_cur,
                                               uint4* rows_next, int* offs_next, uint32_t (&acc16)[kR / 2],
                                               uint32_t (&acc32)[kR])
 {
     const bool more = chunk + 1 < nchunks;
-    if (more) stage_chunk(p, t, chunk + 1, rows_next);
+    if (more) stage_chunk<S>(p, t, chunk + 1, rows_next);
     if (active) {
-        sum_chunk(p, t, chunk, rows_cur, offs_cur, acc16);
-        if ((chunk + 1) % kFlushChunks == 0 || !more) flush(acc16, acc32);
+        sum_chunk<S>(p, t, chunk, rows_cur, offs_cur, acc16);
+        if ((chunk + 1) % S::kFlushChunks == 0 || !more) flush(acc16, acc32);
     }
     if (more) {
-        const int e = chunk_offset(p, t, chunk + 1);
-        if (t.tid < kDms * kC) offs_next[t.tid] = e;
+        const int e = chunk_offset<S>(p, t, chunk + 1);
+        if (t.tid < kDms * S::kC) offs_next[t.tid] = e;
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's copies have landed
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kThreads) void dedisperse_lds_kernel(const uint8_t* __restrict__ L,
-                                                                  const float* __restrict__ dmshifts,
-                                                                  float* __restrict__ out, lds_params p)
+// One tile: DM trials d0 .. d0 + kDms - 1 (one per wave) x samples
+// t0 .. t0 + kT - 1 of the range described by p.  Every wave of the
+// work-group takes part in every barrier; waves past the last DM trial only
+// stage rows.  Ends with all LDS reads done (last barrier), so the caller
+// may start another tile at once.
+template <class S>
+__device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t0)
 {
-    // XCD-aware tile order: work-groups are dispatched round-robin over the
-    // 8 XCDs, so each XCD takes a contiguous block of tiles (consecutive DM
-    // tiles of a time tile share most of their input window in its L2).
-    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
-    const int per_xcd = (ntiles + 7) / 8;
-    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (w >= ntiles) return;  // whole work-group leaves before any barrier
-
     tile_ctx t;
-    t.L = L;
-    t.dmshifts = dmshifts;
-    t.d0 = (w % p.n_dm_tiles) * kDms;
-    t.t0 = (long)(w / p.n_dm_tiles) * kT;
+    t.L = p.L;
+    t.dmshifts = p.dmshifts;
+    t.d0 = d0;
+    t.t0 = t0;
     t.tid = threadIdx.x;
     t.lane = t.tid & 63;
     t.wave = __builtin_amdgcn_readfirstlane(t.tid >> 6);
     t.Wq = p.row_quads;
     const int d = t.d0 + t.wave;
     const bool active = d < p.ndms;
-    const int nchunks = (p.nchans + kC - 1) / kC;
+    const int nchunks = (p.nchans + S::kC - 1) / S::kC;
 
     uint32_t acc16[kR / 2];
     uint32_t acc32[kR];
@@ -310,34 +328,100 @@ __global__ __launch_bounds__(kThreads) void dedisperse_lds_kernel(const uint8_t*
 #pragma unroll
     for (int j = 0; j < kR; ++j) acc32[j] = 0;
 
-    stage_chunk(p, t, 0, g_rows0);
+    stage_chunk<S>(p, t, 0, g_rows0);
     {
-        const int e = chunk_offset(p, t, 0);
-        if (t.tid < kDms * kC) g_offs0[t.tid] = e;
+        const int e = chunk_offset<S>(p, t, 0);
+        if (t.tid < kDms * S::kC) g_offs0[t.tid] = e;
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
     for (int chunk = 0; chunk < nchunks; chunk += 2) {
-        pipeline_step(p, t, chunk, nchunks, active, g_rows0, g_offs0, g_rows1, g_offs1, acc16, acc32);
+        pipeline_step<S>(p, t, chunk, nchunks, active, g_rows0, g_offs0, g_rows1, g_offs1, acc16, acc32);
         if (chunk + 1 >= nchunks) break;
-        pipeline_step(p, t, chunk + 1, nchunks, active, g_rows1, g_offs1, g_rows0, g_offs0, acc16, acc32);
+        pipeline_step<S>(p, t, chunk + 1, nchunks, active, g_rows1, g_offs1, g_rows0, g_offs0, acc16, acc32);
     }
 
-    if (!active) return;
-    const long tl = t.t0 + (long)kR * t.lane;
-    float* orow = out + (size_t)d * p.out_pitch;
+    if (active) {
+        const long tl = t.t0 + (long)kR * t.lane;
+        float* orow = p.out + (size_t)d * p.out_pitch;
 #pragma unroll
-    for (int j = 0; j < kR; ++j) {
-        const long tt = tl + j;
-        if (tt < p.t_processed) orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[j], p.fnchans), p.fbin);
+        for (int j = 0; j < kR; ++j) {
+            const long tt = tl + j;
+            if (tt < p.t_processed) orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[j], p.fnchans), p.fbin);
+        }
     }
+}
+
+// One range: a flat grid of tiles (DM-major within a time tile).
+template <class S>
+__global__ __launch_bounds__(kThreads) void dedisperse_lds_kernel(lds_params p)
+{
+    // XCD-aware tile order: work-groups are dispatched round-robin over the
+    // 8 XCDs, so each XCD takes a contiguous block of tiles (consecutive DM
+    // tiles of a time tile share most of their input window in its L2).
+    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (w >= ntiles) return;  // whole work-group leaves before any barrier
+    process_tile<S>(p, (w % p.n_dm_tiles) * kDms, (long)(w / p.n_dm_tiles) * kT);
+}
+
+// Many ranges (of many beams) in one persistent launch.  The tiles of all
+// jobs form one list, split into 8 contiguous queues of about equal work, one
+// per XCD; a work-group takes tiles from its own XCD's queue (neighbouring
+// tiles share input rows in that XCD's L2) and, once it is empty, from the
+// others in turn.  Every work-group leaves when all 8 queues are empty.
+__global__ __launch_bounds__(kThreads) void dedisperse_lds_batch_kernel(const lds_job* __restrict__ jobs, int njobs,
+                                                                        const long* __restrict__ bounds,
+                                                                        int* __restrict__ counters)
+{
+    __shared__ long s_item;
+    const int xcd = (int)(blockIdx.x % 8);
+    int q = 0;  // queues found empty so far (thread 0)
+    for (;;) {
+        if (threadIdx.x == 0) {
+            long item = -1;
+            while (q < 8) {
+                const int x = (xcd + q) & 7;
+                const long i = bounds[x] + (long)atomicAdd(&counters[x], 1);
+                if (i < bounds[x + 1]) {
+                    item = i;
+                    break;
+                }
+                ++q;
+            }
+            s_item = item;
+        }
+        __syncthreads();
+        const long item = s_item;
+        __syncthreads();
+        if (item < 0) break;
+        int lo = 0, hi = njobs - 1;  // last job with tile_begin <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile_begin <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const lds_job& J = jobs[lo];
+        const long local = item - J.tile_begin;
+        const int d0 = (int)(local % J.p.n_dm_tiles) * kDms;
+        const long t0 = (local / J.p.n_dm_tiles) * kT;
+        if (J.variant == 0) process_tile<shape_c32>(J.p, d0, t0);
+        else process_tile<shape_c16>(J.p, d0, t0);
+    }
+}
+
+__global__ void batch_reset_kernel(int* counters)
+{
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
 }
 
 }  // namespace
 
-int lds_row_quads(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
+int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
 {
+    if (variant < 0 || variant >= kLdsVariants) return -1;
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
     // 16*floor(e/16) + 32*lane.
@@ -355,14 +439,17 @@ int lds_row_quads(const float* h_dmshifts, int nchans, int ndms, float mstartdm,
         }
     }
     const long quads = (e_max >> 4) + 2 * 63 + 3;  // last lane's third quad
-    return quads > kLdsRowQuadsMax ? -1 : (int)quads;
+    const long cap = variant == 0 ? shape_c32::kRowQuads : shape_c16::kRowQuads;
+    return quads > cap ? -1 : (int)quads;
 }
 
 long lds_tile_samples() { return kT; }
+int lds_tile_dms() { return kDms; }
 
-hipError_t launch_dedisperse_lds(const lds_params& p, hipStream_t s)
+hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s)
 {
-    if (p.row_quads <= 0 || p.row_quads > kLdsRowQuadsMax) return hipErrorInvalidValue;
+    const int cap = variant == 0 ? shape_c32::kRowQuads : shape_c16::kRowQuads;
+    if (variant < 0 || variant >= kLdsVariants || p.row_quads <= 0 || p.row_quads > cap) return hipErrorInvalidValue;
     if (p.ndms <= 0 || p.t_processed <= 0) return hipSuccess;
     lds_params q = p;
     q.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
@@ -370,7 +457,27 @@ hipError_t launch_dedisperse_lds(const lds_params& p, hipStream_t s)
     const long ntiles = (long)q.n_dm_tiles * q.n_t_tiles;
     const long grid = 8 * ((ntiles + 7) / 8);
     if (grid > 0x7fffffffL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(dedisperse_lds_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, p.L, p.dmshifts, p.out, q);
+    if (variant == 0)
+        hipLaunchKernelGGL(dedisperse_lds_kernel<shape_c32>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+    else
+        hipLaunchKernelGGL(dedisperse_lds_kernel<shape_c16>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+    return hipGetLastError();
+}
+
+void lds_job_tiles(lds_params& p)
+{
+    p.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
+    p.n_t_tiles = (int)((p.t_processed + kT - 1) / kT);
+}
+
+hipError_t launch_dedisperse_lds_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
+                                       int grid, hipStream_t s)
+{
+    if (njobs <= 0) return hipSuccess;
+    if (grid <= 0 || grid % 8 != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, d_counters);
+    hipLaunchKernelGGL(dedisperse_lds_batch_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, d_jobs, njobs, d_bounds,
+                       d_counters);
     return hipGetLastError();
 }
 

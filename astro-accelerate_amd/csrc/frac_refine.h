@@ -1,0 +1,38 @@
+// Batched DM refinement of candidate windows (see frac_refine.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace aa_hip {
+
+constexpr int kRefineMaxSamples = 4096;    // per window (1024 threads x 4)
+constexpr int kRefineMaxChannels = 4096;   // per window (delay table in LDS)
+
+struct refine_cand {
+    int64_t x_off;    // window at x + x_off, [C][T] float32
+    int64_t ds_off;   // delay per unit DM at ds + ds_off, C entries
+    int C, T;
+};
+
+struct refine_args {
+    const float* x;
+    const float* ds;
+    const refine_cand* cands;   // [ncand]
+    const int* order;           // processing order (largest windows first)
+    int ncand;
+    float* u;                   // [ncand] residual DM, updated in place
+    float* step;                // [ncand] Rprop step, updated in place
+    double* prev_grad;          // [ncand] previous dL/du, updated in place
+    double* loss;               // [ncand] L at the final u
+    double* grad;               // [ncand] dL/du at the final u
+    int nsteps;
+    float step_max, step_min;
+    int* counter;               // queue head (reset by the launch)
+};
+
+// grid: resident work-groups (the kernel is persistent).
+hipError_t launch_refine(const refine_args& a, int grid, hipStream_t s);
+int refine_blocks_per_cu();
+
+}  // namespace aa_hip

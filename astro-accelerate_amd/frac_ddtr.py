@@ -10,9 +10,9 @@ table (dmshift[c], src/aa_ddtr_strategy.cpp:81-92) divided by tsamp.
 
 For candidate b, trial k and output sample t (x is zero outside [0, T)):
 
-    u   = trials[k] + dm[b]             tau = delay_per_dm[c] * u   (float32)
+    u   = trials[k] + dm[b]             tau = delay_per_dm[c] * u   (in double)
     y[b, k, t] = sum_c (1 - f) x[b, c, t + i0] + f x[b, c, t + i0 + 1],
-    i0 = floor(tau), f = tau - i0.
+    i0 = floor(tau), f = (float)(tau - i0).
 
 The backward pass recomputes the interpolation from x (nothing from the
 forward pass is kept besides its inputs).  torch provides device memory and
@@ -196,3 +196,131 @@ def synth_candidates(batch: int, nchans: int, nsamp: int, delay_per_dm, seed: in
         pulse = amplitude * torch.exp(-0.5 * ((t[None, :] - centre[:, None]) / sigma) ** 2)
         x[b] += pulse.to(torch.float32)
     return x, r, dm_true
+
+
+# ------------------------------------------------------------ refinement
+class RefinePlan:
+    """Batched DM refinement of candidate windows of uneven size
+    (aa_hip_refine_*): windows packed in one float32 buffer, [C_b][T_b] at
+    x_offsets[b], delay tables at ds_offsets[b] of a second buffer."""
+
+    def __init__(self, nchans: Sequence[int], nsamp: Sequence[int], x_offsets: Sequence[int],
+                 ds_offsets: Sequence[int], device: int = 0):
+        lib = aa.load_library()
+        n = len(nchans)
+        self.ncand = n
+        xo = (ctypes.c_int64 * n)(*[int(v) for v in x_offsets])
+        do = (ctypes.c_int64 * n)(*[int(v) for v in ds_offsets])
+        nc = (ctypes.c_int * n)(*[int(v) for v in nchans])
+        ns = (ctypes.c_int * n)(*[int(v) for v in nsamp])
+        h = ctypes.c_void_p()
+        aa.check(lib.aa_hip_refine_create(int(device), n, xo, do, nc, ns, ctypes.byref(h)), "aa_hip_refine_create")
+        self.handle = h
+        # extents the buffers must cover (checked before every launch)
+        self.x_extent = max(int(o) + int(c) * int(t) for o, c, t in zip(x_offsets, nchans, nsamp))
+        self.ds_extent = max(int(o) + int(c) for o, c in zip(ds_offsets, nchans))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            aa.load_library().aa_hip_refine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, x, ds, u, step, prev_grad, loss, grad, nsteps: int, step_max: float = 0.5,
+            step_min: float = 1e-4):
+        """nsteps fused forward + adjoint + update steps for every candidate
+        (state tensors updated in place; loss/grad at the final u)."""
+        import torch
+        for t, dt in ((x, torch.float32), (ds, torch.float32), (u, torch.float32), (step, torch.float32),
+                      (prev_grad, torch.float64), (loss, torch.float64), (grad, torch.float64)):
+            if not (t.is_cuda and t.dtype == dt and t.is_contiguous()):
+                raise ValueError("refine buffers: contiguous CUDA tensors of the documented dtypes")
+        for t in (u, step, prev_grad, loss, grad):
+            if t.numel() != self.ncand:
+                raise ValueError("state tensors need one entry per candidate")
+        if x.numel() < self.x_extent or ds.numel() < self.ds_extent:
+            raise ValueError("window or delay-table buffer smaller than the candidate table needs")
+        p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        aa.check(aa.load_library().aa_hip_refine_run(self.handle, p(x), p(ds), p(u), p(step), p(prev_grad), p(loss),
+                                                      p(grad), int(nsteps), float(step_max), float(step_min),
+                                                      _stream_handle()), "aa_hip_refine_run")
+
+
+def synth_refine_candidates(ncand: int, seed: int, device, nchans_set=(1024, 1536, 2048, 2560, 3072, 3584, 4096),
+                            nsamp_set=(256, 512, 1024, 2048, 4096), residual_max: float = 1.0,
+                            sigma: float = 4.0, amplitude: float = 0.5):
+    """Uneven candidate windows, seeded: N(0, 1) noise plus a Gaussian pulse at
+    residual DM r_b (uniform in +-residual_max) in the frame of the
+    candidate's DM, centred in the window; band 1760 MHz down 810 MHz, 64 us.
+    Returns dict(x, ds, x_offsets, ds_offsets, nchans, nsamp, r, tables)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    nchans = [int(rng.choice(nchans_set)) for _ in range(ncand)]
+    nsamp = [int(rng.choice(nsamp_set)) for _ in range(ncand)]
+    r = rng.uniform(-residual_max, residual_max, ncand)
+    tables, ds_off, parts = {}, {}, []
+    off = 0
+    for C in sorted(set(nchans)):
+        tables[C] = delay_per_dm(C, 1760.0, -810.0 / C, 64e-6)
+        ds_off[C] = off
+        parts.append(tables[C])
+        off += C
+    ds = torch.from_numpy(np.concatenate(parts)).to(device)
+    sizes = [C * T for C, T in zip(nchans, nsamp)]
+    x_offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    x = torch.empty(int(sum(sizes)), dtype=torch.float32, device=device)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    for b in range(ncand):
+        C, T = nchans[b], nsamp[b]
+        w = x[int(x_offsets[b]):int(x_offsets[b]) + C * T].view(C, T)
+        w.normal_(generator=g)
+        t = torch.arange(T, device=device, dtype=torch.float64)
+        centre = T / 2.0 + torch.as_tensor(tables[C], device=device, dtype=torch.float64) * float(r[b])
+        w += (amplitude * torch.exp(-0.5 * ((t[None, :] - centre[:, None]) / sigma) ** 2)).to(torch.float32)
+    return dict(x=x, ds=ds, x_offsets=x_offsets, ds_offsets=np.array([ds_off[C] for C in nchans], dtype=np.int64),
+                nchans=nchans, nsamp=nsamp, r=r, tables=tables)
+
+
+def dense_grid_search_plain(cands, u_lo: float = -2.24, du: float = 0.01, ndms: int = 448, tsamp: float = 64e-6):
+    """Best residual DM of every candidate on a dense grid, with the plain u8
+    dedispersion path (aa_hip_dedisperse_u8): each window is quantised to u8
+    (round(128 + 16 x), clipped), each channel row is shifted by the plain
+    path's own delay at u = 0 so that grid trial k is residual u_lo + k du,
+    and the trial with the largest profile energy sum_t (out - 128)^2 wins.
+    Returns the residual DM per candidate."""
+    import torch
+    dev = cands["x"].device
+    best = np.zeros(len(cands["nchans"]))
+    dmshift_cache = {}
+    for b, (C, T) in enumerate(zip(cands["nchans"], cands["nsamp"])):
+        if C not in dmshift_cache:
+            dms = aa_dmshifts(C, 1760.0, -810.0 / C, tsamp)
+            mst = np.float32(0.0)                                   # plain-path trial 0 = residual u_lo
+            mstep = np.float32(np.float32(du) / np.float32(tsamp))
+            k0 = int(round(-u_lo / du))                             # the trial at residual 0
+            o = np.trunc(dms * np.float32(np.float32(k0) * mstep)).astype(np.int64)
+            maxd = int(np.trunc(np.float32(dms[-1] * np.float32(np.float32(ndms - 1) * mstep)))) + 1
+            dmshift_cache[C] = (dms, torch.from_numpy(dms).to(dev), torch.from_numpy(o).to(dev), float(mst),
+                                float(mstep), maxd)
+        dms, d_dms, o, mst, mstep, maxd = dmshift_cache[C]
+        w = cands["x"][int(cands["x_offsets"][b]):int(cands["x_offsets"][b]) + C * T].view(C, T)
+        q = torch.clamp(torch.round(128.0 + 16.0 * w), 0, 255).to(torch.uint8)
+        row_len = T + maxd
+        pitch = ((row_len + 4096 + 255) // 256) * 256
+        L = torch.full((C, pitch), 128, dtype=torch.uint8, device=dev)
+        tt = torch.arange(row_len, device=dev)
+        src = tt[None, :] - o[:, None]                                         # L[c][t'] = q[c][t' - o_c]
+        ok = (src >= 0) & (src < T)
+        L[:, :row_len] = torch.where(ok, torch.gather(q, 1, src.clamp(0, T - 1)), torch.full_like(src, 128,
+                                                                                               dtype=torch.uint8))
+        out = torch.empty((ndms, T), dtype=torch.float32, device=dev)
+        aa.dedisperse_u8(L, pitch, row_len, dms, d_dms, ndms, T, mst, mstep, 1, out, T)
+        energy = ((out.double() - 128.0) ** 2).sum(dim=1)
+        best[b] = u_lo + du * int(torch.argmax(energy))
+    return best

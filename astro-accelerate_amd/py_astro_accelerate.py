@@ -154,6 +154,16 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_dedisperse_u8": ([vp, sz, i, i, PFLOAT, vp, i, i, f, f, i, vp, sz, i, vp], i),
         "aa_hip_synth_noise_u8": ([u64, i64, d, d, vp, vp], i),
         "aa_hip_inject_pulse_u8": ([vp, i, i, vp, d, d, i, i, i, vp], i),
+        "aa_hip_ddtr_set_range_variant": ([vp, i, i], i),
+        "aa_hip_ddtr_autotune_range": ([vp, i, i, vp, sz, i, vp, pi, PFLOAT], i),
+        "aa_hip_ddtr_batch_create": ([i, ctypes.POINTER(vp), ctypes.POINTER(vp)], i),
+        "aa_hip_ddtr_batch_destroy": ([vp], i),
+        "aa_hip_ddtr_batch_execute": ([vp, ctypes.POINTER(vp), pi, ctypes.POINTER(vp), ctypes.POINTER(sz), i, vp],
+                                      i),
+        "aa_hip_ddtr_batch_stats": ([vp, pi, ctypes.POINTER(ctypes.c_long), pi], i),
+        "aa_hip_refine_create": ([i, i, ctypes.POINTER(i64), ctypes.POINTER(i64), pi, pi, ctypes.POINTER(vp)], i),
+        "aa_hip_refine_destroy": ([vp], i),
+        "aa_hip_refine_run": ([vp, vp, vp, vp, vp, vp, vp, vp, i, f, f, vp], i),
         "aa_hip_frac_create": ([i, i, PFLOAT, i, PFLOAT, i, i, f, ctypes.POINTER(vp)], i),
         "aa_hip_frac_destroy": ([vp], i),
         "aa_hip_frac_grouping": ([vp, pi, pi], i),
@@ -560,6 +570,64 @@ class DdtrPlan:
         check(_lib_or_raise().aa_hip_ddtr_run_range(self.handle, int(r), int(t_processed), _ptr(out),
                                                     out.stride(0), _stream_ptr(stream)), "run_range")
 
+    def set_range_variant(self, r: int, variant: int):
+        """Preferred LDS chunk shape of range r (0: 32 channels, 1: 16 channels, -1: first that fits)."""
+        check(_lib_or_raise().aa_hip_ddtr_set_range_variant(self.handle, int(r), int(variant)), "set_range_variant")
+
+    def autotune_range(self, r: int, t_processed: int, out, reps: int = 2, stream=None):
+        """Times each LDS chunk shape that fits range r; keeps the fastest.
+        Returns (best variant or -1, [ms per variant, -1 = does not fit])."""
+        best = ctypes.c_int()
+        ms = (ctypes.c_float * 2)()
+        check(_lib_or_raise().aa_hip_ddtr_autotune_range(self.handle, int(r), int(t_processed), _ptr(out),
+                                                         out.stride(0), int(reps), _stream_ptr(stream),
+                                                         ctypes.byref(best), ms), "autotune_range")
+        return best.value, [float(v) for v in ms]
+
+
+class DdtrBatch:
+    """Many beams (DdtrPlan handles) dedispersed in one persistent launch,
+    optionally replayed from a hipGraph (aa_hip_ddtr_batch_*)."""
+
+    def __init__(self, plans: Sequence["DdtrPlan"]):
+        lib = _lib_or_raise()
+        self.plans = list(plans)   # keep the beams alive as long as the batch
+        arr = (ctypes.c_void_p * len(self.plans))(*[p.handle.value for p in self.plans])
+        h = ctypes.c_void_p()
+        check(lib.aa_hip_ddtr_batch_create(len(self.plans), arr, ctypes.byref(h)), "aa_hip_ddtr_batch_create")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib_or_raise().aa_hip_ddtr_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute(self, inputs, t_processed, outs, use_graph: bool = True, stream=None):
+        """inputs[b]: [t][c] uint8 tensor; t_processed[b][r]; outs[b][r]: float32 [ndms][>= t_processed]."""
+        lib = _lib_or_raise()
+        flat_tp = [int(v) for tp in t_processed for v in tp]
+        flat_out = [o for ob in outs for o in ob]
+        n = len(flat_out)
+        ins = (ctypes.c_void_p * len(inputs))(*[x.data_ptr() for x in inputs])
+        tp = (ctypes.c_int * n)(*flat_tp)
+        ptrs = (ctypes.c_void_p * n)(*[o.data_ptr() for o in flat_out])
+        pitches = (ctypes.c_size_t * n)(*[o.stride(0) for o in flat_out])
+        check(lib.aa_hip_ddtr_batch_execute(self.handle, ins, tp, ptrs, pitches, int(bool(use_graph)),
+                                            _stream_ptr(stream)), "aa_hip_ddtr_batch_execute")
+
+    def stats(self):
+        nj, nd = ctypes.c_int(), ctypes.c_int()
+        nt = ctypes.c_long()
+        check(_lib_or_raise().aa_hip_ddtr_batch_stats(self.handle, ctypes.byref(nj), ctypes.byref(nt),
+                                                      ctypes.byref(nd)), "aa_hip_ddtr_batch_stats")
+        return {"jobs": nj.value, "tiles": nt.value, "direct_jobs": nd.value}
+
 
 def corner_turn_u8(d_in_tc, nsamp: int, nchans: int, d_out, out_pitch: int, stream=None):
     check(_lib_or_raise().aa_hip_corner_turn_u8(_ptr(d_in_tc), int(nsamp), int(nchans), _ptr(d_out),
@@ -600,4 +668,4 @@ def inject_pulse_u8(d_data_tc, nsamp: int, nchans: int, d_dmshifts, dm: float, t
                                                  _stream_ptr(stream)), "inject_pulse_u8")
 
 
-KERNEL_AUTO, KERNEL_DIRECT, KERNEL_LDS = 0, 1, 2
+KERNEL_AUTO, KERNEL_DIRECT, KERNEL_LDS, KERNEL_LDS_C32, KERNEL_LDS_C16 = 0, 1, 2, 3, 4

@@ -43,9 +43,11 @@ typedef enum {
 
 /* Kernel variants for aa_hip_dedisperse_u8 / aa_hip_ddtr_set_kernel. */
 typedef enum {
-    AA_HIP_KERNEL_AUTO = 0,     /* fastest valid variant (autotuned tiles) */
+    AA_HIP_KERNEL_AUTO = 0,     /* LDS kernel when a chunk shape fits (tuned shape if set), else direct */
     AA_HIP_KERNEL_DIRECT = 1,   /* one thread per (DM, sample), global gathers */
-    AA_HIP_KERNEL_LDS = 2       /* LDS-staged windows, packed integer sums */
+    AA_HIP_KERNEL_LDS = 2,      /* LDS-staged windows, packed integer sums, first shape that fits */
+    AA_HIP_KERNEL_LDS_C32 = 3,  /* LDS kernel, 32-channel chunks (rows up to 2432 bytes) */
+    AA_HIP_KERNEL_LDS_C16 = 4   /* LDS kernel, 16-channel chunks (rows up to 4864 bytes) */
 } aa_hip_kernel;
 
 /* Message for the calling thread's most recent aa_hip_* failure ("" if none). */
@@ -232,6 +234,43 @@ int aa_hip_ddtr_ingest(aa_hip_ddtr *h, const void *d_input_tc, int input_bits, i
 int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_out,
                           size_t out_pitch, void *stream);
 
+/* Tile tuning (no reference counterpart: the reference picks its kernel by a
+ * fixed LDS-size test, src/aa_dedisperse.cpp:90-131).  The LDS kernel has
+ * two channel-chunk shapes, 0 = 32 channels x 2432-byte rows and 1 = 16
+ * channels x 4864-byte rows; set_range_variant records a preference for
+ * range r (-1 = first that fits), used whenever it fits.  autotune_range
+ * times every shape that fits range r on `stream` (reps launches each,
+ * writing d_out), keeps the fastest as the preference and reports it:
+ * *best_variant (-1 if none fits) and variant_ms[0..1] (-1 = does not fit).
+ * The range's level must be resident (after ingest). */
+int aa_hip_ddtr_set_range_variant(aa_hip_ddtr *h, int range, int variant);
+int aa_hip_ddtr_autotune_range(aa_hip_ddtr *h, int range, int t_processed, float *d_out,
+                               size_t out_pitch, int reps, void *stream, int *best_variant,
+                               float *variant_ms);
+
+/* Many beams in one launch (no reference counterpart: the reference runs one
+ * beam per process, include/aa_pipeline_api.hpp:113-117).  A batch borrows
+ * the per-beam handles (their plans, levels and tuning); they must outlive
+ * it and share one device.  execute: per beam b, d_inputs_tc[b] is its
+ * [t][c] u8 chunk; t_processed, d_out and out_pitch are flattened
+ * beam-major over each beam's ranges (as execute_chunk takes them per beam).
+ * Every beam is corner-turned and binned, then all ranges run in one
+ * persistent LDS-kernel launch over a shared tile list (ranges no chunk
+ * shape fits run the direct kernel after it).  With use_graph != 0 the
+ * sequence is captured into a hipGraph on the first call for a given set of
+ * buffers and replayed by later calls with the same buffers.  Results are
+ * bit-identical to aa_hip_ddtr_execute_chunk per beam.  Calls on one batch
+ * (or on its beams) must not overlap. */
+typedef struct aa_hip_ddtr_batch aa_hip_ddtr_batch;
+int aa_hip_ddtr_batch_create(int nbeams, aa_hip_ddtr *const *beams, aa_hip_ddtr_batch **out);
+int aa_hip_ddtr_batch_destroy(aa_hip_ddtr_batch *h);
+int aa_hip_ddtr_batch_execute(aa_hip_ddtr_batch *h, const void *const *d_inputs_tc,
+                              const int *t_processed, float *const *d_out,
+                              const size_t *out_pitch, int use_graph, void *stream);
+/* Jobs (ranges) in the persistent launch, tiles, and ranges on the direct kernel
+ * of the last execute. */
+int aa_hip_ddtr_batch_stats(const aa_hip_ddtr_batch *h, int *njobs, long *ntiles, int *ndirect);
+
 /* Replaces corner_turn_SM_inplace (src/aa_corner_turn.cpp:118-121) for the
  * 8-bit path: [t][c] -> [c][t] into a separate u8 buffer with row pitch
  * out_pitch bytes (>= nsamp). */
@@ -304,6 +343,27 @@ int aa_hip_frac_forward(aa_hip_frac *h, const float *d_x, int nbatch, const floa
  * from x: nothing from the forward pass is stored. */
 int aa_hip_frac_adjoint(aa_hip_frac *h, const float *d_x, const float *d_gy, int nbatch,
                         const float *d_dm, float *d_gx, double *d_gdm, void *stream);
+
+/* Batched DM refinement of candidate windows of uneven size (no reference
+ * counterpart).  Candidate b's window is float [nchans[b]][nsamp[b]] at
+ * d_x + x_offsets[b] (elements), its delay-per-unit-DM table nchans[b]
+ * floats at d_delay_per_dm + ds_offsets[b]; up to 4096 x 4096.  For its
+ * residual DM u[b] the energy L = sum_t y(t)^2 of the dedispersed profile
+ * (the aa_hip_frac_forward transform with one trial at offset 0) and dL/du
+ * are evaluated by one fused forward + adjoint pass (the adjoint recomputes
+ * the interpolation), and u[b] climbs L by nsteps sign-based adaptive steps:
+ * step[b] grows x1.2 (up to step_max) while the sign of dL/du holds and
+ * halves (down to step_min) when it flips; u += sign(dL/du) * step.  One
+ * persistent launch for all candidates and steps.  u, step, prev_grad
+ * (initialise to 0) are updated in place; loss and grad receive L and dL/du
+ * at the final u.  nsteps = 0 only evaluates. */
+typedef struct aa_hip_refine aa_hip_refine;
+int aa_hip_refine_create(int device, int ncand, const int64_t *x_offsets, const int64_t *ds_offsets,
+                         const int *nchans, const int *nsamp, aa_hip_refine **out);
+int aa_hip_refine_destroy(aa_hip_refine *h);
+int aa_hip_refine_run(aa_hip_refine *h, const float *d_x, const float *d_delay_per_dm, float *d_u,
+                      float *d_step, double *d_prev_grad, double *d_loss, double *d_grad, int nsteps,
+                      float step_max, float step_min, void *stream);
 
 #ifdef __cplusplus
 }

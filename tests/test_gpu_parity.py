@@ -11,7 +11,9 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [1, 2]  # AA_HIP_KERNEL_DIRECT, AA_HIP_KERNEL_LDS
+# AA_HIP_KERNEL_DIRECT, _LDS (first chunk shape that fits), _LDS_C32, _LDS_C16
+KERNELS = [1, 2, 3, 4]
+KERNELS_FULL = [1, 2, 4]
 
 
 def say(msg):
@@ -96,8 +98,13 @@ def test_dedisperse_range(aa, oracle, cuda, geom, kernel):
     d_ds = to_dev(ds, cuda)
     out_pitch = tp + 13
     out = torch.full((ndms, out_pitch), -1.0, dtype=torch.float32, device=cuda)
-    aa.dedisperse_u8(to_dev(L, cuda), pitch, row_len, ds, d_ds, ndms, tp, float(ms), float(st), ib, out,
-                     out_pitch, kernel=kernel)
+    try:
+        aa.dedisperse_u8(to_dev(L, cuda), pitch, row_len, ds, d_ds, ndms, tp, float(ms), float(st), ib, out,
+                         out_pitch, kernel=kernel)
+    except aa.AAHipError as e:
+        # a forced chunk shape may not hold this geometry's windows: refused on the host
+        assert kernel == 3 and e.status == aa.AA_HIP_ERROR_UNSUPPORTED, e
+        return
     got = out.cpu().numpy()
     np.testing.assert_array_equal(got[:, :tp], want)
     assert np.all(got[:, tp:] == -1.0)
@@ -122,7 +129,7 @@ def _chunked_reference(oracle, ranges, nchans, nsamp, fch1, foff, tsamp, free, s
     return s, data, oracle.run(s, data)
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS_FULL)
 def test_execute_chunk_config1_full(aa, oracle, cuda, kernel):
     """512 ch x 131072 samples, DM 0-500/0.5: the whole plane, bit-exact."""
     import torch
@@ -141,7 +148,7 @@ def test_execute_chunk_config1_full(aa, oracle, cuda, kernel):
     assert d == 500 and 60000 <= t < 60008
 
 
-@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("kernel", KERNELS_FULL)
 def test_execute_chunks_multi_range_multi_chunk(aa, oracle, cuda, kernel):
     """Several chunks (small free memory), 4 ranges with x2/x4/x8 binning."""
     import torch
