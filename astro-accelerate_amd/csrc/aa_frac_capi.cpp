@@ -282,11 +282,12 @@ int aa_hip_refine_destroy(aa_hip_refine* h) {
 }
 
 int aa_hip_refine_run(aa_hip_refine* h, const float* d_x, const float* d_delay_per_dm, float* d_u, float* d_step,
-                      double* d_prev_grad, double* d_loss, double* d_grad, int nsteps, float step_max,
-                      float step_min, void* stream) {
+                      double* d_prev_grad, double* d_loss, double* d_grad, int scan_n, float scan_lo,
+                      float scan_step, int nsteps, float step_max, float step_min, void* stream) {
     if (!h || !d_x || !d_delay_per_dm || !d_u || !d_step || !d_prev_grad || !d_loss || !d_grad)
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL argument");
-    if (nsteps < 0 || !(step_max > 0.0f) || !(step_min > 0.0f) || step_min > step_max)
+    if (nsteps < 0 || !(step_max > 0.0f) || !(step_min > 0.0f) || step_min > step_max || scan_n < 0 ||
+        !std::isfinite(scan_lo) || !std::isfinite(scan_step))
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "invalid step settings");
     aa_hip::refine_args a{};
     a.x = d_x;
@@ -302,6 +303,9 @@ int aa_hip_refine_run(aa_hip_refine* h, const float* d_x, const float* d_delay_p
     a.nsteps = nsteps;
     a.step_max = step_max;
     a.step_min = step_min;
+    a.scan_n = scan_n;
+    a.scan_lo = scan_lo;
+    a.scan_step = scan_step;
     a.counter = h->d_counter;
     return from_hip(aa_hip::launch_refine(a, h->grid, (hipStream_t)stream), "refine");
 }

@@ -28,6 +28,8 @@ struct refine_args {
     double* grad;               // [ncand] dL/du at the final u
     int nsteps;
     float step_max, step_min;
+    int scan_n;                 // coarse scan points before the steps (0 = none)
+    float scan_lo, scan_step;   // scan offsets relative to the starting u
     int* counter;               // queue head (reset by the launch)
 };
 

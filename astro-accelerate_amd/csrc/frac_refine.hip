@@ -8,14 +8,18 @@
 // with x = 0 outside [0, T) and ds[c] the delay per unit DM in samples.
 //     dL/du = sum_c ds[c] sum_t 2 y(t) (x[c][t + i_c + 1] - x[c][t + i_c]).
 //
+// An optional coarse scan first evaluates L (forward pass only) at scan_n
+// offsets u + scan_lo + k * scan_step and starts the steps from the best.
+//
 // One persistent launch: a work-group (1024 threads) takes candidates from a
 // queue (largest first) and runs every step of that candidate itself, so the
 // forward pass, the adjoint pass and the update are fused: pass 1 sums the
-// profile (each thread owns up to 4 samples, t = tid + 1024 j, and keeps y in
-// registers), pass 2 re-reads the window (recomputation: no shifted tile is
+// profile in double (each thread owns up to 4 samples, t = tid + 1024 j, and
+// keeps y in registers), pass 2 re-reads the window (recomputation: no shifted tile is
 // stored) and accumulates the DM derivative, and thread 0 applies a
 // sign-based adaptive step (Rprop: the step grows by 1.2 while the sign of
-// dL/du holds and halves when it flips).  Windows are read straight from
+// dL/du holds and halves when it flips); the result is the best point
+// visited.  Windows are read straight from
 // global memory, coalesced along t: with one trial per candidate there is no
 // reuse to stage, and the passes are bound by HBM bandwidth.
 #include "frac_refine.h"
@@ -51,8 +55,10 @@ __device__ __forceinline__ float load_or_zero(const float* __restrict__ row, int
     return (idx >= 0 && idx < T) ? row[idx] : 0.0f;
 }
 
-// (L, dL/du) of candidate `cd` at residual DM u; every thread gets both.
-__device__ void evaluate(const refine_args& a, const refine_cand& cd, float u, int tid, double* loss, double* grad)
+// (L, dL/du) of candidate `cd` at residual DM u; every thread gets both
+// (dL/du only when want_grad).
+__device__ void evaluate(const refine_args& a, const refine_cand& cd, float u, int tid, bool want_grad,
+                         double* loss, double* grad)
 {
     const float* __restrict__ x = a.x + cd.x_off;
     const float* __restrict__ ds = a.ds + cd.ds_off;
@@ -65,16 +71,19 @@ __device__ void evaluate(const refine_args& a, const refine_cand& cd, float u, i
     }
     __syncthreads();
 
-    float y[kPerThread];
+    // The profile is accumulated in double: at an aligned pulse y reaches
+    // ~C/2 and float32 rounding of y would cost ~1e-5 of dL/du.
+    double y[kPerThread];
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) y[j] = 0.0f;
+    for (int j = 0; j < kPerThread; ++j) y[j] = 0.0;
     int c = 0;
     for (; c + kUnroll <= C; c += kUnroll) {
-        float x0[kUnroll][kPerThread], x1[kUnroll][kPerThread], w[kUnroll];
+        float x0[kUnroll][kPerThread], x1[kUnroll][kPerThread];
+        double w[kUnroll];
 #pragma unroll
         for (int k = 0; k < kUnroll; ++k) {
             const int2 e = r_tab[c + k];
-            w[k] = __int_as_float(e.y);
+            w[k] = (double)__int_as_float(e.y);
             const float* row = x + (size_t)(c + k) * T;
 #pragma unroll
             for (int j = 0; j < kPerThread; ++j) {
@@ -87,55 +96,61 @@ __device__ void evaluate(const refine_args& a, const refine_cand& cd, float u, i
         for (int k = 0; k < kUnroll; ++k)
 #pragma unroll
             for (int j = 0; j < kPerThread; ++j)
-                y[j] = __builtin_fmaf(w[k], x1[k][j], __builtin_fmaf(1.0f - w[k], x0[k][j], y[j]));
+                y[j] = __builtin_fma(w[k], (double)x1[k][j], __builtin_fma(1.0 - w[k], (double)x0[k][j], y[j]));
     }
     for (; c < C; ++c) {
         const int2 e = r_tab[c];
-        const float f = __int_as_float(e.y);
+        const double f = (double)__int_as_float(e.y);
         const float* row = x + (size_t)c * T;
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
             const int i = tid + kThreads * j + e.x;
-            y[j] = __builtin_fmaf(f, load_or_zero(row, i + 1, T), __builtin_fmaf(1.0f - f, load_or_zero(row, i, T), y[j]));
+            y[j] = __builtin_fma(f, (double)load_or_zero(row, i + 1, T),
+                                 __builtin_fma(1.0 - f, (double)load_or_zero(row, i, T), y[j]));
         }
     }
     double l2 = 0.0;
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
-        if (tid + kThreads * j >= T) y[j] = 0.0f;   // outputs only for 0 <= t < T
-        l2 += (double)y[j] * (double)y[j];
+        if (tid + kThreads * j >= T) y[j] = 0.0;    // outputs only for 0 <= t < T
+        l2 += y[j] * y[j];
     }
 
+    if (!want_grad) {
+        *loss = block_sum(l2, tid);
+        *grad = 0.0;
+        return;
+    }
     // pass 2: dL/du, recomputing the window reads
     double g = 0.0;
     c = 0;
     for (; c + kUnroll <= C; c += kUnroll) {
-        float d[kUnroll];
+        double d[kUnroll];
 #pragma unroll
         for (int k = 0; k < kUnroll; ++k) {
             const int2 e = r_tab[c + k];
             const float* row = x + (size_t)(c + k) * T;
-            float s = 0.0f;
+            double s = 0.0;
 #pragma unroll
             for (int j = 0; j < kPerThread; ++j) {
                 const int i = tid + kThreads * j + e.x;
-                s = __builtin_fmaf(y[j], load_or_zero(row, i + 1, T) - load_or_zero(row, i, T), s);
+                s = __builtin_fma(y[j], (double)(load_or_zero(row, i + 1, T) - load_or_zero(row, i, T)), s);
             }
             d[k] = s;
         }
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) g = __builtin_fma((double)ds[c + k], (double)d[k], g);
+        for (int k = 0; k < kUnroll; ++k) g = __builtin_fma((double)ds[c + k], d[k], g);
     }
     for (; c < C; ++c) {
         const int2 e = r_tab[c];
         const float* row = x + (size_t)c * T;
-        float s = 0.0f;
+        double s = 0.0;
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
             const int i = tid + kThreads * j + e.x;
-            s = __builtin_fmaf(y[j], load_or_zero(row, i + 1, T) - load_or_zero(row, i, T), s);
+            s = __builtin_fma(y[j], (double)(load_or_zero(row, i + 1, T) - load_or_zero(row, i, T)), s);
         }
-        g = __builtin_fma((double)ds[c], (double)s, g);
+        g = __builtin_fma((double)ds[c], s, g);
     }
     *loss = block_sum(l2, tid);
     *grad = 2.0 * block_sum(g, tid);
@@ -157,9 +172,31 @@ __global__ __launch_bounds__(kThreads) void refine_kernel(refine_args a)
         float step = a.step[b];
         double prev = a.prev_grad[b];
         double loss = 0.0, grad = 0.0;
+        if (a.scan_n > 0) {  // coarse energy scan around u; the best offset starts the steps
+            const float u0 = r_u;
+            float best_u = u0;
+            double best_l = -1.0;
+            for (int k = 0; k < a.scan_n; ++k) {
+                const float uk = u0 + a.scan_lo + a.scan_step * (float)k;
+                evaluate(a, cd, uk, tid, false, &loss, &grad);
+                if (loss > best_l) {
+                    best_l = loss;
+                    best_u = uk;
+                }
+            }
+            if (tid == 0) r_u = best_u;
+            __syncthreads();
+        }
+        float best_u = r_u;
+        double best_l = -1.0, best_g = 0.0;
         for (int s = 0; s <= a.nsteps; ++s) {
             const float u = r_u;
-            evaluate(a, cd, u, tid, &loss, &grad);
+            evaluate(a, cd, u, tid, true, &loss, &grad);
+            if (loss > best_l) {  // the result is the best point visited
+                best_l = loss;
+                best_u = u;
+                best_g = grad;
+            }
             if (s == a.nsteps) break;
             if (tid == 0) {
                 if (grad * prev > 0.0) step = fminf(step * 1.2f, a.step_max);
@@ -171,11 +208,11 @@ __global__ __launch_bounds__(kThreads) void refine_kernel(refine_args a)
             __syncthreads();
         }
         if (tid == 0) {
-            a.u[b] = r_u;
+            a.u[b] = best_u;
             a.step[b] = step;
             a.prev_grad[b] = prev;
-            a.loss[b] = loss;
-            a.grad[b] = grad;
+            a.loss[b] = best_l;
+            a.grad[b] = best_g;
         }
         __syncthreads();
     }

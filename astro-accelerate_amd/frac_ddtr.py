@@ -232,9 +232,10 @@ class RefinePlan:
             pass
 
     def run(self, x, ds, u, step, prev_grad, loss, grad, nsteps: int, step_max: float = 0.5,
-            step_min: float = 1e-4):
-        """nsteps fused forward + adjoint + update steps for every candidate
-        (state tensors updated in place; loss/grad at the final u)."""
+            step_min: float = 1e-4, scan=None):
+        """Optional coarse energy scan (scan = (n, lo, step): offsets relative
+        to u), then nsteps fused forward + adjoint + update steps for every
+        candidate (state tensors updated in place; loss/grad at the final u)."""
         import torch
         for t, dt in ((x, torch.float32), (ds, torch.float32), (u, torch.float32), (step, torch.float32),
                       (prev_grad, torch.float64), (loss, torch.float64), (grad, torch.float64)):
@@ -246,9 +247,11 @@ class RefinePlan:
         if x.numel() < self.x_extent or ds.numel() < self.ds_extent:
             raise ValueError("window or delay-table buffer smaller than the candidate table needs")
         p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+        sn, slo, sst = scan if scan is not None else (0, 0.0, 0.0)
         aa.check(aa.load_library().aa_hip_refine_run(self.handle, p(x), p(ds), p(u), p(step), p(prev_grad), p(loss),
-                                                      p(grad), int(nsteps), float(step_max), float(step_min),
-                                                      _stream_handle()), "aa_hip_refine_run")
+                                                      p(grad), int(sn), float(slo), float(sst), int(nsteps),
+                                                      float(step_max), float(step_min), _stream_handle()),
+                 "aa_hip_refine_run")
 
 
 def synth_refine_candidates(ncand: int, seed: int, device, nchans_set=(1024, 1536, 2048, 2560, 3072, 3584, 4096),

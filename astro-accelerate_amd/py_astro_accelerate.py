@@ -163,7 +163,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_ddtr_batch_stats": ([vp, pi, ctypes.POINTER(ctypes.c_long), pi], i),
         "aa_hip_refine_create": ([i, i, ctypes.POINTER(i64), ctypes.POINTER(i64), pi, pi, ctypes.POINTER(vp)], i),
         "aa_hip_refine_destroy": ([vp], i),
-        "aa_hip_refine_run": ([vp, vp, vp, vp, vp, vp, vp, vp, i, f, f, vp], i),
+        "aa_hip_refine_run": ([vp, vp, vp, vp, vp, vp, vp, vp, i, f, f, i, f, f, vp], i),
         "aa_hip_frac_create": ([i, i, PFLOAT, i, PFLOAT, i, i, f, ctypes.POINTER(vp)], i),
         "aa_hip_frac_destroy": ([vp], i),
         "aa_hip_frac_grouping": ([vp, pi, pi], i),

@@ -351,19 +351,23 @@ int aa_hip_frac_adjoint(aa_hip_frac *h, const float *d_x, const float *d_gy, int
  * residual DM u[b] the energy L = sum_t y(t)^2 of the dedispersed profile
  * (the aa_hip_frac_forward transform with one trial at offset 0) and dL/du
  * are evaluated by one fused forward + adjoint pass (the adjoint recomputes
- * the interpolation), and u[b] climbs L by nsteps sign-based adaptive steps:
+ * the interpolation).  With scan_n > 0, L is first evaluated (forward only)
+ * at u + scan_lo + k * scan_step, k < scan_n, and the best point starts the
+ * steps.  Then u[b] climbs L by nsteps sign-based adaptive steps:
  * step[b] grows x1.2 (up to step_max) while the sign of dL/du holds and
  * halves (down to step_min) when it flips; u += sign(dL/du) * step.  One
- * persistent launch for all candidates and steps.  u, step, prev_grad
- * (initialise to 0) are updated in place; loss and grad receive L and dL/du
- * at the final u.  nsteps = 0 only evaluates. */
+ * persistent launch for all candidates and steps.  u receives the best point
+ * visited (largest L), loss and grad L and dL/du there; step and prev_grad
+ * (initialise prev_grad to 0) carry the step state.  nsteps = 0 only
+ * evaluates (after the scan, if any). */
 typedef struct aa_hip_refine aa_hip_refine;
 int aa_hip_refine_create(int device, int ncand, const int64_t *x_offsets, const int64_t *ds_offsets,
                          const int *nchans, const int *nsamp, aa_hip_refine **out);
 int aa_hip_refine_destroy(aa_hip_refine *h);
 int aa_hip_refine_run(aa_hip_refine *h, const float *d_x, const float *d_delay_per_dm, float *d_u,
-                      float *d_step, double *d_prev_grad, double *d_loss, double *d_grad, int nsteps,
-                      float step_max, float step_min, void *stream);
+                      float *d_step, double *d_prev_grad, double *d_loss, double *d_grad, int scan_n,
+                      float scan_lo, float scan_step, int nsteps, float step_max, float step_min,
+                      void *stream);
 
 #ifdef __cplusplus
 }

@@ -31,8 +31,8 @@ def small_specs():
         dict(nchans=96, nsamp=16384, ranges=[(0, 50, 0.25, 1, 1), (50, 150, 0.5, 2, 2), (150, 300, 1.0, 4, 4)],
              bw=100.0, seed=13, pulses=[(200.0, 0.1, 16)]),
         dict(nchans=256, nsamp=32768, ranges=[(0, 300, 1.0, 1, 1)], bw=100.0, seed=14, pulses=[(33.0, 0.4, 1)]),
-        # full band: 16-channel shape needed for range 1 (step/inBin 0.5)
-        dict(nchans=64, nsamp=65536, ranges=[(0, 100, 0.25, 1, 1), (100, 300, 1.0, 2, 2)], bw=810.0, seed=15,
+        # full band: 16-channel shape needed for range 1 (step/inBin 0.6: 15 trials span ~440 samples)
+        dict(nchans=64, nsamp=65536, ranges=[(0, 100, 0.25, 1, 1), (100, 300, 1.2, 2, 2)], bw=810.0, seed=15,
              pulses=[(250.0, 0.2, 3)]),
         # full band, DM step 8: no LDS shape holds a tile's window -> direct kernel
         dict(nchans=48, nsamp=32768, ranges=[(0, 300, 8.0, 1, 1)], bw=810.0, seed=16, pulses=[(160.0, 0.3, 6)]),
@@ -124,6 +124,11 @@ def test_autotune_keeps_results_bit_exact(aa, oracle, cuda, beams, tmp_path):
     say("autotune then batched launch")
     cache = mb.autotune(beams, cache_path=str(tmp_path / "tune.json"), reps=1)
     assert cache and all(v["variant"] in (-1, 0, 1) for v in cache.values())
+    for v in cache.values():   # the fastest shape that fits; -1 only when none fits
+        fits = [ms for ms in v["ms"] if ms >= 0]
+        assert (v["variant"] == -1) == (not fits)
+        if fits:
+            assert v["ms"][v["variant"]] == min(fits)
     # the full-band beam's second range fits only the 16-channel shape
     key = repr(mb.shape_class(beams[4], 1))
     assert cache[key]["variant"] == 1 and cache[key]["ms"][0] == -1.0
