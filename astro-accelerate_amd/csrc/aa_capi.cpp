@@ -362,7 +362,7 @@ int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nch
     if (nchans <= 0 || ndms < 0 || t_processed < 0 || inBin <= 0 || row_len < 0 || (size_t)row_len > pitch ||
         out_pitch < (size_t)t_processed)
         return bad("invalid geometry");
-    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_C16) return bad("unknown kernel variant");
+    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_PAIR) return bad("unknown kernel variant");
     const int st = aa_hip::check_delays(h_dmshifts, nchans, ndms, mstartdm, mdmstep, t_processed, row_len);
     if (st != AA_HIP_SUCCESS) return st;
     aa_hip::range_launch rl;

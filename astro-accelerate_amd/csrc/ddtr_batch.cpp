@@ -98,7 +98,10 @@ int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float
             st = e->prepare_range(r, t_processed[flat], d_out[flat], out_pitch[flat], s, &rl);
             if (st != AA_HIP_SUCCESS) return st;
             if (rl.ndms <= 0 || rl.t_processed <= 0) continue;
-            if (rl.variant >= 0) {
+            // The persistent kernel holds the two LDS chunk shapes; the pair
+            // shape (whose registers would make every batched tile spill)
+            // and ranges no shape fits run as launches of their own.
+            if (rl.variant == 0 || rl.variant == 1) {
                 lds_job j{};
                 j.p = lds_params_of(rl);
                 j.variant = rl.variant;

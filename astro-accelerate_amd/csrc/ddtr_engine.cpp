@@ -101,7 +101,10 @@ int resolve_launch(int kernel, const uint8_t *L, size_t pitch, int nchans, const
     rl->out = out;
     rl->out_pitch = out_pitch;
     rl->variant = -1;
-    const int forced = kernel == AA_HIP_KERNEL_LDS_C32 ? 0 : kernel == AA_HIP_KERNEL_LDS_C16 ? 1 : -1;
+    const int forced = kernel == AA_HIP_KERNEL_LDS_C32    ? 0
+                       : kernel == AA_HIP_KERNEL_LDS_C16  ? 1
+                       : kernel == AA_HIP_KERNEL_LDS_PAIR ? 2
+                                                          : -1;
     if (kernel != AA_HIP_KERNEL_DIRECT && (pitch % 16) == 0 && ((uintptr_t)L % 16) == 0) {
         if (forced >= 0) choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, forced);
         else if (choice.variant == -2) choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, -1);
@@ -217,7 +220,7 @@ ddtr_engine::~ddtr_engine() {
 }
 
 int ddtr_engine::set_kernel(int kernel) {
-    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_C16)
+    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_PAIR)
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "unknown kernel variant");
     m_kernel = kernel;
     return AA_HIP_SUCCESS;

@@ -11,8 +11,9 @@ namespace aa_hip {
 // 32-channel buffers plus the offset tables fill 160 KiB of LDS.  The
 // 16-channel shape holds rows twice as long in the same buffers.
 constexpr int kLdsRowQuadsMax = 152;
-// Chunk shapes: 0 = 32 channels x 152 quads, 1 = 16 channels x 304 quads.
-constexpr int kLdsVariants = 2;
+// Chunk shapes: 0 = 32 channels x 152 quads, 1 = 16 channels x 304 quads,
+// 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums).
+constexpr int kLdsVariants = 3;
 
 struct lds_params {
     const uint8_t* L;        // nchans rows of u8 samples
@@ -33,7 +34,7 @@ struct lds_params {
 // tile_begin + n_dm_tiles * n_t_tiles - 1 of the batch's tile list.
 struct lds_job {
     lds_params p;
-    int variant;             // chunk shape (0 or 1)
+    int variant;             // chunk shape (0 or 1: the shapes the batched kernel holds)
     int pad;
     long tile_begin;
 };

@@ -23,6 +23,9 @@
 // with delays trunc(dmshift[c] * fma(d, mdmstep, mstartdm)) (:430,440).
 #include "ddtr_lds.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace aa_hip {
 
 namespace {
@@ -353,6 +356,361 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pair pre-summation (chunk shape 2).  For a 16-DM tile the relative delay
+// v = delay(c+1, d) - delay(c, d) of two adjacent channels takes only a few
+// values (about two: the truncations of nearly equal delays differ by k or
+// k+1).  Per chunk of 16 channels (8 pairs) the work-group builds, for each
+// pair and each of its values v, one row of 16-bit sums
+//     S_v[j] = L[c][base + j] + L[c+1][base + v + j],  j < 2432,
+// base = 8-sample floor of the pair's smallest delay in the tile.  A DM
+// trial then adds the row of its own v at offset delay(c, d) - base: two
+// channels per 16-bit lane, exactly the sum of the single-channel kernel.
+// Rows are built from global memory (L2) through registers: the loads of
+// chunk k+1 are issued before chunk k is summed, the sums and LDS stores
+// after; one barrier per chunk.  A row is 304 float4 quads (8 samples each)
+// in four phase planes (quad q at plane q % 4, slot q / 4), so the five
+// ds_read_b128 of a lane run (quads 4*lane + e/8 + m) are stride-1 across
+// the wave.  16 rows per buffer; the host admits a range only if no chunk of
+// any tile needs more rows or a longer window (pair_plan_fits).
+// ---------------------------------------------------------------------------
+constexpr int kPairs = 8;                    // pairs per chunk
+constexpr int kPairChunk = 2 * kPairs;       // 16 channels
+constexpr int kPRQ = 2 * kLdsRowQuadsMax;    // 304 quads per row (2432 samples)
+constexpr int kPPlane = kPRQ / 4;            // 76 slots per phase plane
+constexpr int kPRows = kBufQuads / kPRQ;     // 16 rows per buffer
+constexpr int kPFlushChunks = 256 / kPairChunk;   // 128 pair sums (<= 510 each) per 16-bit field
+// lane 63 reads quads 252 + e/8 .. +4 of a row
+constexpr int kPairMaxOffset = 8 * (kPRQ - 257) + 7;
+static_assert(kPRows * kPRQ == kBufQuads, "pair rows fill the LDS buffers");
+static_assert(kPRows == kDms, "one row per wave is built");
+
+// Per-chunk tables, three chunks in flight (summed, being built, next).
+// A pair whose relative delay takes more than two values in the tile is
+// split into two single-channel rows (each DM then adds both), so a chunk
+// never needs more than 16 rows.
+struct pair_tables {
+    // per (DM slot, pair): x = row1 | row2 << 8 (0xff = none), y = e1 | e2 << 16
+    int2 ent[3][kDms * kPairs];
+    int2 row[3][kPRows];          // per row: (channel in chunk, v; -1 = single channel)
+    int rbase[3][kPRows];         // per row: first sample of its window
+    int nrows[3];
+};
+__shared__ pair_tables g_ptab;
+
+__device__ __forceinline__ pair_tables& ptab() { return g_ptab; }
+
+// Tables of chunk `chunk` into slot `sl`; run by wave 0 (lanes: DM slot
+// lane % 16, pairs lane / 16 and lane / 16 + 4).
+__device__ __forceinline__ void pair_table(const lds_params& p, int d0, int chunk, int sl, int lane)
+{
+    pair_tables& T = ptab();
+    const int c0 = chunk * kPairChunk;
+    const int w = lane & 15, g = lane >> 4;
+    const int d = min(d0 + w, p.ndms - 1);
+    int v[2], ea[2], eb[2], vmin[2], nv[2], nr[2], basea[2], baseb[2];
+    bool have[2], hasb[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int a = c0 + 2 * (g + 4 * h);
+        v[h] = ea[h] = eb[h] = basea[h] = baseb[h] = 0;
+        have[h] = a < p.nchans;
+        hasb[h] = a + 1 < p.nchans;
+        if (have[h]) {
+            const float dsa = p.dmshifts[a];
+            const int da = dev_delay(dsa, d, p.mstartdm, p.mdmstep);
+            basea[h] = dev_delay(dsa, d0, p.mstartdm, p.mdmstep) & ~7;
+            ea[h] = da - basea[h];
+            if (hasb[h]) {
+                const float dsb = p.dmshifts[a + 1];
+                const int db = dev_delay(dsb, d, p.mstartdm, p.mdmstep);
+                baseb[h] = dev_delay(dsb, d0, p.mstartdm, p.mdmstep) & ~7;
+                eb[h] = db - baseb[h];
+                v[h] = db - da;
+            }
+        }
+        int lo = v[h], hi = v[h];
+#pragma unroll
+        for (int m = 8; m > 0; m >>= 1) {
+            lo = min(lo, __shfl_xor(lo, m, 16));
+            hi = max(hi, __shfl_xor(hi, m, 16));
+        }
+        vmin[h] = lo;
+        nv[h] = hi - lo + 1;
+        nr[h] = !have[h] ? 0 : (nv[h] <= 2 ? nv[h] : 2);   // split pair: two single rows
+    }
+    // first row of each of the 8 pairs (pair g + 4h lives in lanes 16g..16g+15)
+    int first[2];
+    int total = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        int f = 0;
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg) {
+            if (gg == g) f = total;
+            total += __builtin_amdgcn_readlane(nr[h], 16 * gg);
+        }
+        first[h] = f;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int pp = g + 4 * h;
+        int r1 = 0xff, r2 = 0xff, e1 = ea[h], e2 = 0;
+        if (nr[h] && nv[h] <= 2) {
+            r1 = first[h] + (v[h] - vmin[h]);
+            if (w < nv[h]) {
+                T.row[sl][first[h] + w] = make_int2(2 * pp, vmin[h] + w);
+                T.rbase[sl][first[h] + w] = basea[h];
+            }
+        } else if (nr[h]) {
+            r1 = first[h];
+            r2 = first[h] + 1;
+            e2 = eb[h];
+            if (w < 2) {
+                T.row[sl][first[h] + w] = make_int2(2 * pp + w, -1);
+                T.rbase[sl][first[h] + w] = w ? baseb[h] : basea[h];
+            }
+        }
+        T.ent[sl][w * kPairs + pp] = make_int2(r1 | (r2 << 8), (e1 & 0xffff) | (e2 << 16));
+    }
+    if (lane == 0) T.nrows[sl] = total;
+}
+
+// Wave w builds row w of a chunk (when the chunk has that many rows).  Lane
+// l writes whole quads (8 samples) q = 4 (l % 16 + 16 i) + l / 16: plane
+// l / 16, slot l % 16 + 16 i, so 16 consecutive lanes write 16 consecutive
+// slots of one plane (stride-1 ds_write_b128, no bank conflicts), and the
+// wave's loads cover 512 contiguous bytes per i.  The row's channel,
+// relative delay and window start are wave-uniform.  Items are loaded and
+// stored in kPParts parts, each overlapped with a part of the summation, to
+// keep the prefetch registers few.
+constexpr int kPIt = (kPRQ + 63) / 64;         // quads per lane per row (5)
+constexpr int kPParts = 2;
+constexpr int kPHalf = (kPIt + kPParts - 1) / kPParts;
+struct pair_build {
+    uint32_t xa[kPHalf][2], xb[kPHalf][3];
+};
+
+__host__ __device__ constexpr int pair_part_items(int h)
+{
+    return kPIt - h * kPHalf < kPHalf ? kPIt - h * kPHalf : kPHalf;
+}
+__device__ __forceinline__ int pair_quad(int lane, int i) { return 4 * ((lane & 15) + 16 * i) + (lane >> 4); }
+
+// Global loads of part H of row `wave` of chunk `chunk` (table slot sl).
+template <int H>
+__device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, int chunk, int sl, int wave, int lane,
+                                                pair_build& B)
+{
+    const pair_tables& T = ptab();
+    const int nrows = __builtin_amdgcn_readfirstlane(T.nrows[sl]);
+    if (wave >= nrows) return;
+    const int2 rv = T.row[sl][wave];
+    const int ch = __builtin_amdgcn_readfirstlane(rv.x);
+    const int v = __builtin_amdgcn_readfirstlane(rv.y);
+    const int base = __builtin_amdgcn_readfirstlane(T.rbase[sl][wave]);
+    const int c0 = chunk * kPairChunk;
+    const uint8_t* ra = p.L + (size_t)(c0 + ch) * p.pitch + t0 + base;
+    const bool hasb = v >= 0 && c0 + ch + 1 < p.nchans;
+    const uint8_t* rb = p.L + (size_t)(c0 + ch + 1) * p.pitch + t0 + ((base + (hasb ? v : 0)) & ~3);
+#pragma unroll
+    for (int i = 0; i < pair_part_items(H); ++i) {
+        const int q = pair_quad(lane, H * kPHalf + i);
+        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0, b2 = 0;
+        if (q < kPRQ) {
+            const uint2 av = *reinterpret_cast<const uint2*>(ra + 8 * q);
+            a0 = av.x;
+            a1 = av.y;
+            if (hasb) {
+                const uint32_t* bp = reinterpret_cast<const uint32_t*>(rb + 8 * q);
+                b0 = bp[0];
+                b1 = bp[1];
+                b2 = bp[2];
+            }
+        }
+        B.xa[i][0] = a0;
+        B.xa[i][1] = a1;
+        B.xb[i][0] = b0;
+        B.xb[i][1] = b1;
+        B.xb[i][2] = b2;
+    }
+}
+
+// Pair sums of part H of row `wave`, stored into `rows`.
+template <int H>
+__device__ __forceinline__ void pair_build_store(int sl, int wave, int lane, const pair_build& B, uint4* rows)
+{
+    const pair_tables& T = ptab();
+    const int nrows = __builtin_amdgcn_readfirstlane(T.nrows[sl]);
+    if (wave >= nrows) return;
+    const uint32_t sh = (uint32_t)(__builtin_amdgcn_readfirstlane(T.row[sl][wave].y) & 3);  // -1 (single): xb = 0
+    uint4* r = rows + wave * kPRQ + (lane >> 4) * kPPlane + (lane & 15);
+#pragma unroll
+    for (int i = 0; i < pair_part_items(H); ++i) {
+        const int ii = H * kPHalf + i;
+        if (pair_quad(lane, ii) < kPRQ) {
+            const uint32_t x0 = __builtin_amdgcn_alignbyte(B.xb[i][1], B.xb[i][0], sh);
+            const uint32_t x1 = __builtin_amdgcn_alignbyte(B.xb[i][2], B.xb[i][1], sh);
+            uint4 o;
+            o.x = __builtin_amdgcn_perm(0u, B.xa[i][0], 0x0c010c00u) + __builtin_amdgcn_perm(0u, x0, 0x0c010c00u);
+            o.y = __builtin_amdgcn_perm(0u, B.xa[i][0], 0x0c030c02u) + __builtin_amdgcn_perm(0u, x0, 0x0c030c02u);
+            o.z = __builtin_amdgcn_perm(0u, B.xa[i][1], 0x0c010c00u) + __builtin_amdgcn_perm(0u, x1, 0x0c010c00u);
+            o.w = __builtin_amdgcn_perm(0u, B.xa[i][1], 0x0c030c02u) + __builtin_amdgcn_perm(0u, x1, 0x0c030c02u);
+            r[16 * ii] = o;   // slot (lane % 16) + 16 ii of plane lane / 16
+        }
+    }
+}
+
+// acc16[j] += the 16 dwords (32 samples) of a lane run whose five quads
+// are q[m], SD dwords and H half-dwords into the first quad.
+template <int SD, int H>
+__device__ __forceinline__ void pair_accumulate(const uint4* const (&q)[5], uint32_t (&acc)[kR / 2])
+{
+    constexpr int nq = (SD + 16 + H + 3) / 4;
+    uint32_t w[20];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        u32x4 t = {0u, 0u, 0u, 0u};
+        if (m < nq) t = *reinterpret_cast<const u32x4*>(q[m]);
+        w[4 * m] = t.x;
+        w[4 * m + 1] = t.y;
+        w[4 * m + 2] = t.z;
+        w[4 * m + 3] = t.w;
+    }
+#pragma unroll
+    for (int j = 0; j < kR / 2; ++j) {
+        const uint32_t x = H ? __builtin_amdgcn_alignbyte(w[SD + j + 1], w[SD + j], 2u) : w[SD + j];
+        asm("v_add_u32 %0, %0, %1" : "+v"(acc[j]) : "v"(x));
+    }
+}
+
+// All rows of the wave's entries (lane l < 16: entry l) whose sub-quad
+// offset is SUB: a plain loop per offset class, so the accumulators stay in
+// their registers (no switch joins).
+template <int SUB>
+__device__ __forceinline__ void pair_sum_class(uint64_t m, int row_v, int e_v, int lane, const uint4* rows,
+                                               uint32_t (&acc)[kR / 2])
+{
+    while (m) {
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        const int row = __builtin_amdgcn_readlane(row_v, l);
+        const int e8 = __builtin_amdgcn_readlane(e_v, l) >> 3;
+        // quad 4*lane + e8 + k of the row: plane (e8 + k) & 3, slot lane + (e8 + k) / 4
+        const uint4* rb = rows + row * kPRQ + lane;
+        const uint4* q[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) q[k] = rb + ((e8 + k) & 3) * kPPlane + ((e8 + k) >> 2);
+        pair_accumulate<(SUB >> 1), (SUB & 1)>(q, acc);
+    }
+}
+
+template <int SUB>
+__device__ __forceinline__ void pair_sum_classes(bool valid, int sub, int row_v, int e_v, int lane,
+                                                 const uint4* rows, uint32_t (&acc)[kR / 2])
+{
+    pair_sum_class<SUB>(__builtin_amdgcn_ballot_w64(valid && sub == SUB), row_v, e_v, lane, rows, acc);
+    if constexpr (SUB + 1 < 8) pair_sum_classes<SUB + 1>(valid, sub, row_v, e_v, lane, rows, acc);
+}
+
+// Pairs [P0, P0 + kPairs / kPParts) of chunk slot sl: lanes 0..3 (or the
+// part's pairs) hold the first row of each pair, lanes 8.. the second row
+// of split pairs.
+template <int P0>
+__device__ __forceinline__ void pair_sum_half(int sl, int wave, int lane, const uint4* rows, uint32_t (&acc)[kR / 2])
+{
+    constexpr int NP = kPairs / kPParts;
+    const pair_tables& T = ptab();
+    const int pp = P0 + (lane & (NP - 1));
+    const int2 en = T.ent[sl][wave * kPairs + pp];
+    const bool second = (lane & 8) != 0;
+    const int row = second ? ((en.x >> 8) & 0xff) : (en.x & 0xff);
+    // host-checked to lie in [0, kPairMaxOffset]; clamped so that a row read
+    // can never leave the buffer
+    const int e = min(max(second ? ((en.y >> 16) & 0xffff) : (en.y & 0xffff), 0), kPairMaxOffset);
+    const bool valid = (lane & 7) < NP && lane < 16 && row < kPRows;
+    pair_sum_classes<0>(valid, e & 7, row, e, lane, rows, acc);
+}
+
+// Part H: load part H of chunk k+1's build, sum pairs of part H of chunk k,
+// store the build part.
+template <int H>
+__device__ __forceinline__ void pair_part(const lds_params& p, long t0, int k, bool more, bool active, int sn,
+                                          int wave, int lane, const uint4* rows_cur, uint4* rows_next,
+                                          uint32_t (&acc16)[kR / 2])
+{
+    pair_build B;
+    if (more) pair_build_load<H>(p, t0, k + 1, sn, wave, lane, B);
+    if (active) pair_sum_half<H * (kPairs / kPParts)>(k % 3, wave, lane, rows_cur, acc16);
+    if (more) pair_build_store<H>(sn, wave, lane, B, rows_next);
+}
+
+__device__ __forceinline__ void pair_step(const lds_params& p, long t0, int d0, int k, int nchunks, bool active,
+                                          int wave, int lane, int tid, const uint4* rows_cur, uint4* rows_next,
+                                          uint32_t (&acc16)[kR / 2], uint32_t (&acc32)[kR])
+{
+    const bool more = k + 1 < nchunks;
+    const int sn = (k + 1) % 3;
+    pair_part<0>(p, t0, k, more, active, sn, wave, lane, rows_cur, rows_next, acc16);
+    pair_part<1>(p, t0, k, more, active, sn, wave, lane, rows_cur, rows_next, acc16);
+    if (active && ((k + 1) % kPFlushChunks == 0 || !more)) flush(acc16, acc32);
+    if (wave == 0 && k + 2 < nchunks) pair_table(p, d0, k + 2, (k + 2) % 3, lane);
+    __syncthreads();
+}
+
+__device__ __forceinline__ void process_pair_tile(const lds_params& p, int d0, long t0)
+{
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool active = d0 + wave < p.ndms;
+    const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
+    uint32_t acc16[kR / 2];
+    uint32_t acc32[kR];
+#pragma unroll
+    for (int j = 0; j < kR / 2; ++j) acc16[j] = 0;
+#pragma unroll
+    for (int j = 0; j < kR; ++j) acc32[j] = 0;
+
+    if (wave == 0) {
+        pair_table(p, d0, 0, 0, lane);
+        if (nchunks > 1) pair_table(p, d0, 1, 1, lane);
+    }
+    __syncthreads();
+    {
+        pair_build B;
+        pair_build_load<0>(p, t0, 0, 0, wave, lane, B);
+        pair_build_store<0>(0, wave, lane, B, g_rows0);
+        pair_build_load<1>(p, t0, 0, 0, wave, lane, B);
+        pair_build_store<1>(0, wave, lane, B, g_rows0);
+    }
+    __syncthreads();
+    for (int k = 0; k < nchunks; k += 2) {
+        pair_step(p, t0, d0, k, nchunks, active, wave, lane, tid, g_rows0, g_rows1, acc16, acc32);
+        if (k + 1 >= nchunks) break;
+        pair_step(p, t0, d0, k + 1, nchunks, active, wave, lane, tid, g_rows1, g_rows0, acc16, acc32);
+    }
+
+    if (active) {
+        const int d = d0 + wave;
+        const long tl = t0 + (long)kR * lane;
+        float* orow = p.out + (size_t)d * p.out_pitch;
+#pragma unroll
+        for (int j = 0; j < kR; ++j) {
+            const long tt = tl + j;
+            if (tt < p.t_processed) orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[j], p.fnchans), p.fbin);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void dedisperse_pair_kernel(lds_params p)
+{
+    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (w >= ntiles) return;  // whole work-group leaves before any barrier
+    process_pair_tile(p, (w % p.n_dm_tiles) * kDms, (long)(w / p.n_dm_tiles) * kT);
+}
+
 // One range: a flat grid of tiles (DM-major within a time tile).
 template <class S>
 __global__ __launch_bounds__(kThreads) void dedisperse_lds_kernel(lds_params p)
@@ -419,9 +777,12 @@ __global__ void batch_reset_kernel(int* counters)
 
 }  // namespace
 
+static bool pair_plan_fits(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep);
+
 int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
 {
     if (variant < 0 || variant >= kLdsVariants) return -1;
+    if (variant == 2) return pair_plan_fits(h_dmshifts, nchans, ndms, mstartdm, mdmstep) ? kPRQ : -1;
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
     // 16*floor(e/16) + 32*lane.
@@ -443,12 +804,38 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
     return quads > cap ? -1 : (int)quads;
 }
 
+// Pair shape: every window offset of every tile fits a row (the same
+// arithmetic as pair_table; rows per chunk never exceed 16 by construction).
+static bool pair_plan_fits(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
+{
+    std::vector<int> dl((size_t)kDms * nchans), d0l((size_t)nchans);
+    for (int d0 = 0; d0 < ndms; d0 += kDms) {
+        const float s0 = __builtin_fmaf((float)d0, mdmstep, mstartdm);
+        for (int c = 0; c < nchans; ++c) d0l[(size_t)c] = (int)(h_dmshifts[c] * s0);
+        for (int w = 0; w < kDms; ++w) {
+            const int d = std::min(d0 + w, ndms - 1);
+            const float sd = __builtin_fmaf((float)d, mdmstep, mstartdm);
+            for (int c = 0; c < nchans; ++c) dl[(size_t)w * nchans + c] = (int)(h_dmshifts[c] * sd);
+        }
+        for (int c = 0; c < nchans; ++c) {
+            const int base = d0l[(size_t)c] & ~7;
+            for (int w = 0; w < kDms; ++w) {
+                const int e = dl[(size_t)w * nchans + c] - base;
+                if (e < 0 || e > kPairMaxOffset) return false;
+                if ((c & 1) == 0 && c + 1 < nchans && dl[(size_t)w * nchans + c + 1] < dl[(size_t)w * nchans + c])
+                    return false;   // relative delay of a pair must not be negative
+            }
+        }
+    }
+    return true;
+}
+
 long lds_tile_samples() { return kT; }
 int lds_tile_dms() { return kDms; }
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s)
 {
-    const int cap = variant == 0 ? shape_c32::kRowQuads : shape_c16::kRowQuads;
+    const int cap = variant == 0 ? shape_c32::kRowQuads : variant == 1 ? shape_c16::kRowQuads : kPRQ;
     if (variant < 0 || variant >= kLdsVariants || p.row_quads <= 0 || p.row_quads > cap) return hipErrorInvalidValue;
     if (p.ndms <= 0 || p.t_processed <= 0) return hipSuccess;
     lds_params q = p;
@@ -459,8 +846,10 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
     if (grid > 0x7fffffffL) return hipErrorInvalidValue;
     if (variant == 0)
         hipLaunchKernelGGL(dedisperse_lds_kernel<shape_c32>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
-    else
+    else if (variant == 1)
         hipLaunchKernelGGL(dedisperse_lds_kernel<shape_c16>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+    else
+        hipLaunchKernelGGL(dedisperse_pair_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
     return hipGetLastError();
 }
 

@@ -47,7 +47,8 @@ typedef enum {
     AA_HIP_KERNEL_DIRECT = 1,   /* one thread per (DM, sample), global gathers */
     AA_HIP_KERNEL_LDS = 2,      /* LDS-staged windows, packed integer sums, first shape that fits */
     AA_HIP_KERNEL_LDS_C32 = 3,  /* LDS kernel, 32-channel chunks (rows up to 2432 bytes) */
-    AA_HIP_KERNEL_LDS_C16 = 4   /* LDS kernel, 16-channel chunks (rows up to 4864 bytes) */
+    AA_HIP_KERNEL_LDS_C16 = 4,  /* LDS kernel, 16-channel chunks (rows up to 4864 bytes) */
+    AA_HIP_KERNEL_LDS_PAIR = 5  /* LDS kernel, 16-channel chunks pre-summed in channel pairs */
 } aa_hip_kernel;
 
 /* Message for the calling thread's most recent aa_hip_* failure ("" if none). */
@@ -236,12 +237,16 @@ int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_o
 
 /* Tile tuning (no reference counterpart: the reference picks its kernel by a
  * fixed LDS-size test, src/aa_dedisperse.cpp:90-131).  The LDS kernel has
- * two channel-chunk shapes, 0 = 32 channels x 2432-byte rows and 1 = 16
- * channels x 4864-byte rows; set_range_variant records a preference for
- * range r (-1 = first that fits), used whenever it fits.  autotune_range
- * times every shape that fits range r on `stream` (reps launches each,
- * writing d_out), keeps the fastest as the preference and reports it:
- * *best_variant (-1 if none fits) and variant_ms[0..1] (-1 = does not fit).
+ * three channel-chunk shapes, 0 = 32 channels x 2432-byte rows, 1 = 16
+ * channels x 4864-byte rows, 2 = 16 channels in pairs; set_range_variant
+ * records a preference for
+ * range r (-1 = first that fits), used whenever it fits; shape 2 sums
+ * 16-channel chunks pre-added in channel pairs (rows of 16-bit pair sums,
+ * one per distinct relative delay of the pair in a 16-DM tile).
+ * autotune_range times every shape that fits range r on `stream` (reps
+ * launches each, writing d_out), keeps the fastest as the preference and
+ * reports it: *best_variant (-1 if none fits) and variant_ms[0..2] (-1 =
+ * does not fit).
  * The range's level must be resident (after ingest). */
 int aa_hip_ddtr_set_range_variant(aa_hip_ddtr *h, int range, int variant);
 int aa_hip_ddtr_autotune_range(aa_hip_ddtr *h, int range, int t_processed, float *d_out,
@@ -255,8 +260,8 @@ int aa_hip_ddtr_autotune_range(aa_hip_ddtr *h, int range, int t_processed, float
  * [t][c] u8 chunk; t_processed, d_out and out_pitch are flattened
  * beam-major over each beam's ranges (as execute_chunk takes them per beam).
  * Every beam is corner-turned and binned, then all ranges run in one
- * persistent LDS-kernel launch over a shared tile list (ranges no chunk
- * shape fits run the direct kernel after it).  With use_graph != 0 the
+ * persistent LDS-kernel launch over a shared tile list (ranges on the pair
+ * shape, or that no chunk shape fits, run launches of their own after it).  With use_graph != 0 the
  * sequence is captured into a hipGraph on the first call for a given set of
  * buffers and replayed by later calls with the same buffers.  Results are
  * bit-identical to aa_hip_ddtr_execute_chunk per beam.  Calls on one batch
@@ -267,8 +272,8 @@ int aa_hip_ddtr_batch_destroy(aa_hip_ddtr_batch *h);
 int aa_hip_ddtr_batch_execute(aa_hip_ddtr_batch *h, const void *const *d_inputs_tc,
                               const int *t_processed, float *const *d_out,
                               const size_t *out_pitch, int use_graph, void *stream);
-/* Jobs (ranges) in the persistent launch, tiles, and ranges on the direct kernel
- * of the last execute. */
+/* Jobs (ranges) in the persistent launch, tiles, and ranges launched on their
+ * own (direct kernel or pair shape) in the last execute. */
 int aa_hip_ddtr_batch_stats(const aa_hip_ddtr_batch *h, int *njobs, long *ntiles, int *ndirect);
 
 /* Replaces corner_turn_SM_inplace (src/aa_corner_turn.cpp:118-121) for the
