@@ -369,7 +369,22 @@ int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nch
     const int rs = aa_hip::resolve_launch(kernel, d_in, pitch, nchans, d_dmshifts, h_dmshifts, ndms, t_processed,
                                           mstartdm, mdmstep, inBin, d_out, out_pitch, aa_hip::lds_choice{-2, 0}, &rl);
     if (rs != AA_HIP_SUCCESS) return rs;
-    return from_hip(aa_hip::launch_range(rl, (hipStream_t)stream), "dedisperse");
+    if (rl.variant != 2) return from_hip(aa_hip::launch_range(rl, (hipStream_t)stream), "dedisperse");
+    // Chunk shape 2 reads plan tables; this handle-free call builds them,
+    // launches, waits for the stream and frees them (handles keep theirs).
+    std::vector<uint8_t> bytes;
+    if (aa_hip::lds_pair_tables(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, &bytes) != rl.row_quads)
+        return bad("pair plan tables disagree with the chosen row width");
+    void *t = nullptr;
+    hipError_t e = hipMalloc(&t, bytes.size());
+    if (e == hipSuccess) e = hipMemcpy(t, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        rl.pair_recs = t;
+        e = aa_hip::launch_range(rl, (hipStream_t)stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (t) (void)hipFree(t);
+    return from_hip(e, "dedisperse");
 }
 
 int aa_hip_synth_noise_u8(uint64_t seed, int64_t n, double mean, double sigma, uint8_t *d_out_tc, void *stream) {

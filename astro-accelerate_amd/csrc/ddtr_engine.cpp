@@ -135,6 +135,7 @@ lds_params lds_params_of(const range_launch &rl) {
     p.out = rl.out;
     p.out_pitch = rl.out_pitch;
     p.row_quads = rl.row_quads;
+    p.pair_recs = rl.pair_recs;
     lds_job_tiles(p);
     return p;
 }
@@ -217,6 +218,8 @@ ddtr_engine::~ddtr_engine() {
         if (lv.ptr) (void)hipFree(lv.ptr);
     if (m_d_dmshifts) (void)hipFree(m_d_dmshifts);
     if (m_d_err) (void)hipFree(m_d_err);
+    for (void *t : m_pair_recs)
+        if (t) (void)hipFree(t);
 }
 
 int ddtr_engine::set_kernel(int kernel) {
@@ -311,8 +314,29 @@ int ddtr_engine::prepare_range(int r, int t_processed, float *d_out, size_t out_
         if (c.variant < 0 && want >= 0)  // preferred shape does not fit: any that does
             c = choose_lds(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, -1);
     }
-    return resolve_launch(m_kernel, lv.ptr, lv.pitch, m_nchans, m_d_dmshifts, m_dmshifts.data(), rd.ndms,
-                          t_processed, mstartdm, mdmstep, rd.inBin, d_out, out_pitch, c, rl);
+    st = resolve_launch(m_kernel, lv.ptr, lv.pitch, m_nchans, m_d_dmshifts, m_dmshifts.data(), rd.ndms,
+                        t_processed, mstartdm, mdmstep, rd.inBin, d_out, out_pitch, c, rl);
+    if (st != AA_HIP_SUCCESS || rl->variant != 2) return st;
+    // Chunk shape 2 reads plan tables; they depend on the range only, so
+    // they are built once (on the first call that needs them, never while a
+    // later call is being captured into a graph).
+    if (m_pair_recs.size() != m_ranges.size()) m_pair_recs.assign(m_ranges.size(), nullptr);
+    void *&t = m_pair_recs[(size_t)r];
+    if (!t) {
+        std::vector<uint8_t> bytes;
+        if (lds_pair_tables(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, lv.pitch, &bytes) != rl->row_quads)
+            return fail(AA_HIP_ERROR_UNSUPPORTED, "pair plan tables disagree with the chosen row width");
+        hipError_t e = hipMalloc(&t, bytes.size());
+        if (e == hipSuccess) e = hipMemcpy(t, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (t) (void)hipFree(t);
+            t = nullptr;
+            return hip_fail(e, "pair plan tables");
+        }
+        m_device_bytes += bytes.size();
+    }
+    rl->pair_recs = t;
+    return AA_HIP_SUCCESS;
 }
 
 int ddtr_engine::set_range_variant(int r, int variant) {

@@ -58,6 +58,7 @@ struct range_launch {
     size_t out_pitch = 0;
     int variant = -1;
     int row_quads = 0;
+    const void *pair_recs = nullptr;  // chunk shape 2: device plan tables
 };
 
 class ddtr_engine {
@@ -134,6 +135,7 @@ private:
     int m_resident = -1;  // highest level valid for the current chunk
     std::vector<lds_choice> m_lds;     // per range: chosen LDS shape (-2 = not computed, -1 = direct)
     std::vector<int> m_variant_pref;   // per range: preferred LDS shape, -1 = any
+    std::vector<void *> m_pair_recs;   // per range: chunk shape 2 plan tables (device), built on first use
     size_t m_device_bytes = 0;
 };
 

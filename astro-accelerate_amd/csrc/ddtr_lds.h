@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace aa_hip {
 
@@ -27,6 +28,7 @@ struct lds_params {
     float* out;
     size_t out_pitch;        // floats between output rows
     int row_quads;           // staged row width in uint4 (from lds_row_quads)
+    const void* pair_recs;   // chunk shape 2: device plan tables (from lds_pair_tables)
     int n_dm_tiles, n_t_tiles;  // filled by the launcher / lds_job_tiles
 };
 
@@ -43,6 +45,13 @@ struct lds_job {
 // or -1 when it exceeds that shape's rows.  Host only; evaluates the delay
 // rule exactly as the kernel does.
 int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep);
+
+// Chunk shape 2 (pair pre-summation) plan tables of one range over rows
+// `pitch` bytes apart: fills `bytes` (when not null) with the records the
+// kernel reads, to be copied to the device as they are, and returns the row
+// width in quads, or -1 when the range does not fit the shape.  Host only.
+int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
+                    std::vector<uint8_t>* bytes);
 
 // Tile geometry: samples per tile and DM trials per tile.  Rows must hold
 // lds_tile_samples() readable bytes past the last processed sample plus the

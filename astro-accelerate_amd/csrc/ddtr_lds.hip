@@ -23,6 +23,8 @@
 // with delays trunc(dmshift[c] * fma(d, mdmstep, mstartdm)) (:430,440).
 #include "ddtr_lds.h"
 
+#include <cstring>
+
 #include <algorithm>
 #include <vector>
 
@@ -362,353 +364,240 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
 // values (about two: the truncations of nearly equal delays differ by k or
 // k+1).  Per chunk of 16 channels (8 pairs) the work-group builds, for each
 // pair and each of its values v, one row of 16-bit sums
-//     S_v[j] = L[c][base + j] + L[c+1][base + v + j],  j < 2432,
-// base = 8-sample floor of the pair's smallest delay in the tile.  A DM
-// trial then adds the row of its own v at offset delay(c, d) - base: two
-// channels per 16-bit lane, exactly the sum of the single-channel kernel.
-// Rows are built from global memory (L2) through registers: the loads of
-// chunk k+1 are issued before chunk k is summed, the sums and LDS stores
-// after; one barrier per chunk.  A row is 304 float4 quads (8 samples each)
-// in four phase planes (quad q at plane q % 4, slot q / 4), so the five
-// ds_read_b128 of a lane run (quads 4*lane + e/8 + m) are stride-1 across
-// the wave.  16 rows per buffer; the host admits a range only if no chunk of
-// any tile needs more rows or a longer window (pair_plan_fits).
+//     S_v[i] = L[c][base + i] + L[c+1][base + v + i],
+// base = the pair's delay at the tile's first DM rounded down to 4 samples.
+// A DM trial then adds the row of its own v at offset e = delay(c, d) - base:
+// two channels per 16-bit field, exactly the sum of the single-channel
+// kernel.
+//
+// Tiles are 16 DMs x 1024 samples; each wave sums one DM.  Transposed
+// accumulators: lane L holds output samples t0 + L + 64 j (low half of
+// acc16[j]) and t0 + 512 + L + 64 j (high half), j < 8, and a row stores
+// dword k = S[k] | S[k + 512] << 16.  The eight dwords a lane adds for
+// offset e are then k = e + L + 64 j: every read is stride-1 across the
+// wave whatever e is (two dwords per ds_read2st64_b32), a DM's delay is only
+// a base address, and two rows are summed per v_add3.
+//
+// Which rows a chunk needs and where each DM reads them depend only on the
+// plan, not on the data: the host computes them once per range
+// (pair_build_records: one pair_rec per DM tile and chunk, read by every
+// time tile through L2).  Rows are built from global memory through
+// registers: the loads of chunk k+1 are issued before chunk k is summed and
+// stored after it; one barrier per chunk.
 // ---------------------------------------------------------------------------
-constexpr int kPairs = 8;                    // pairs per chunk
-constexpr int kPairChunk = 2 * kPairs;       // 16 channels
-constexpr int kPRQ = 2 * kLdsRowQuadsMax;    // 304 quads per row (2432 samples)
-constexpr int kPPlane = kPRQ / 4;            // 76 slots per phase plane
-constexpr int kPRows = kBufQuads / kPRQ;     // 16 rows per buffer
+constexpr int kPT = 1024;                      // samples per tile
+constexpr int kPR = kPT / 64;                  // 16 samples per lane
+constexpr int kPSplit = kPT / 2;               // samples between the halves of a dword
+constexpr int kPairs = 8;                      // pairs per chunk
+constexpr int kPairChunk = 2 * kPairs;         // 16 channels
+constexpr int kPIt = 5;                        // quads per lane per row
+constexpr int kPRQ = 64 * kPIt;                // 320 quads per row
+constexpr int kPRowDw = 4 * kPRQ;              // 1280 dwords per row
+constexpr int kPRows = 16;                     // rows per buffer (one per wave)
+constexpr int kPMaxEnt = 2 * kPairs;           // row entries per DM per chunk
 constexpr int kPFlushChunks = 256 / kPairChunk;   // 128 pair sums (<= 510 each) per 16-bit field
-// lane 63 reads quads 252 + e/8 .. +4 of a row
-constexpr int kPairMaxOffset = 8 * (kPRQ - 257) + 7;
-static_assert(kPRows * kPRQ == kBufQuads, "pair rows fill the LDS buffers");
+constexpr int kPairMaxOffset = kPRowDw - kPSplit; // e + 511 < kPRowDw
 static_assert(kPRows == kDms, "one row per wave is built");
+static_assert(kPSplit == 64 * (kPR / 2), "a lane's eight dwords span half a tile");
 
-// Per-chunk tables, three chunks in flight (summed, being built, next).
-// A pair whose relative delay takes more than two values in the tile is
-// split into two single-channel rows (each DM then adds both), so a chunk
-// never needs more than 16 rows.
-struct pair_tables {
-    // per (DM slot, pair): x = row1 | row2 << 8 (0xff = none), y = e1 | e2 << 16
-    int2 ent[3][kDms * kPairs];
-    int2 row[3][kPRows];          // per row: (channel in chunk, v; -1 = single channel)
-    int rbase[3][kPRows];         // per row: first sample of its window
-    int nrows[3];
+// Plan record of one (DM tile, chunk).  A pair whose relative delay takes
+// more than two values in the tile is split into two single-channel rows
+// (each DM then adds both), so a chunk never needs more than 16 rows or a DM
+// more than 16 entries; entries 0 .. npairs-1 belong to the pairs in order,
+// the second rows of split pairs follow.
+struct pair_row {
+    long long offa, offb;   // bytes from the level start to the two channels' window starts (offb: dword floor)
+    uint32_t sh, mb;        // byte shift of b's window in its dwords; ~0 pair row, 0 single row (b masked off)
+    int pad0, pad1;
 };
-__shared__ pair_tables g_ptab;
+struct pair_rec {
+    uint16_t ent[kDms][kPMaxEnt];  // per DM slot: dword offset (row * kPRowDw + e) of each entry
+    pair_row row[kPRows];
+    int nrows, nent, pad0, pad1;
+};
+static_assert(sizeof(pair_row) == 32 && sizeof(pair_rec) % 16 == 0, "records are 16-byte aligned");
+static_assert(kPRows * kPRowDw <= 65536, "entry offsets fit 16 bits");
 
-__device__ __forceinline__ pair_tables& ptab() { return g_ptab; }
+__shared__ uint4 g_prow0[kPRows * kPRQ];   // 2 x 80 KiB: the whole LDS
+__shared__ uint4 g_prow1[kPRows * kPRQ];
 
-// Tables of chunk `chunk` into slot `sl`; run by wave 0 (lanes: DM slot
-// lane % 16, pairs lane / 16 and lane / 16 + 4).
-__device__ __forceinline__ void pair_table(const lds_params& p, int d0, int chunk, int sl, int lane)
-{
-    pair_tables& T = ptab();
-    const int c0 = chunk * kPairChunk;
-    const int w = lane & 15, g = lane >> 4;
-    const int d = min(d0 + w, p.ndms - 1);
-    int v[2], ea[2], eb[2], vmin[2], nv[2], nr[2], basea[2], baseb[2];
-    bool have[2], hasb[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int a = c0 + 2 * (g + 4 * h);
-        v[h] = ea[h] = eb[h] = basea[h] = baseb[h] = 0;
-        have[h] = a < p.nchans;
-        hasb[h] = a + 1 < p.nchans;
-        if (have[h]) {
-            const float dsa = p.dmshifts[a];
-            const int da = dev_delay(dsa, d, p.mstartdm, p.mdmstep);
-            basea[h] = dev_delay(dsa, d0, p.mstartdm, p.mdmstep) & ~7;
-            ea[h] = da - basea[h];
-            if (hasb[h]) {
-                const float dsb = p.dmshifts[a + 1];
-                const int db = dev_delay(dsb, d, p.mstartdm, p.mdmstep);
-                baseb[h] = dev_delay(dsb, d0, p.mstartdm, p.mdmstep) & ~7;
-                eb[h] = db - baseb[h];
-                v[h] = db - da;
-            }
-        }
-        int lo = v[h], hi = v[h];
-#pragma unroll
-        for (int m = 8; m > 0; m >>= 1) {
-            lo = min(lo, __shfl_xor(lo, m, 16));
-            hi = max(hi, __shfl_xor(hi, m, 16));
-        }
-        vmin[h] = lo;
-        nv[h] = hi - lo + 1;
-        nr[h] = !have[h] ? 0 : (nv[h] <= 2 ? nv[h] : 2);   // split pair: two single rows
-    }
-    // first row of each of the 8 pairs (pair g + 4h lives in lanes 16g..16g+15)
-    int first[2];
-    int total = 0;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        int f = 0;
-#pragma unroll
-        for (int gg = 0; gg < 4; ++gg) {
-            if (gg == g) f = total;
-            total += __builtin_amdgcn_readlane(nr[h], 16 * gg);
-        }
-        first[h] = f;
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int pp = g + 4 * h;
-        int r1 = 0xff, r2 = 0xff, e1 = ea[h], e2 = 0;
-        if (nr[h] && nv[h] <= 2) {
-            r1 = first[h] + (v[h] - vmin[h]);
-            if (w < nv[h]) {
-                T.row[sl][first[h] + w] = make_int2(2 * pp, vmin[h] + w);
-                T.rbase[sl][first[h] + w] = basea[h];
-            }
-        } else if (nr[h]) {
-            r1 = first[h];
-            r2 = first[h] + 1;
-            e2 = eb[h];
-            if (w < 2) {
-                T.row[sl][first[h] + w] = make_int2(2 * pp + w, -1);
-                T.rbase[sl][first[h] + w] = w ? baseb[h] : basea[h];
-            }
-        }
-        T.ent[sl][w * kPairs + pp] = make_int2(r1 | (r2 << 8), (e1 & 0xffff) | (e2 << 16));
-    }
-    if (lane == 0) T.nrows[sl] = total;
-}
-
-// Wave w builds row w of a chunk (when the chunk has that many rows).  Lane
-// l writes whole quads (8 samples) q = 4 (l % 16 + 16 i) + l / 16: plane
-// l / 16, slot l % 16 + 16 i, so 16 consecutive lanes write 16 consecutive
-// slots of one plane (stride-1 ds_write_b128, no bank conflicts), and the
-// wave's loads cover 512 contiguous bytes per i.  The row's channel,
-// relative delay and window start are wave-uniform.  Items are loaded and
-// stored in kPParts parts, each overlapped with a part of the summation, to
-// keep the prefetch registers few.
-constexpr int kPIt = (kPRQ + 63) / 64;         // quads per lane per row (5)
-constexpr int kPParts = 2;
-constexpr int kPHalf = (kPIt + kPParts - 1) / kPParts;
+// Registers of one row in flight: quads q = lane + 64 i, i < NI.
+template <int NI>
 struct pair_build {
-    uint32_t xa[kPHalf][2], xb[kPHalf][3];
+    uint32_t al[NI], ah[NI], bl[NI][2], bh[NI][2];
+    uint32_t sh, mb;
 };
 
-__host__ __device__ constexpr int pair_part_items(int h)
+// Loads of row `wave` of a chunk (record rec), issued whatever the row's
+// use (no register merges on a branch): a wave without a row in the chunk
+// reads the zeroed record's window at the level start.  A lane over-reads at
+// most 63 quads (252 bytes) past the window's needed width, inside the
+// 4096-byte slack every row has past its longest window.
+template <int NI>
+__device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, const pair_rec* rec, int wave,
+                                                int lane, pair_build<NI>& B)
 {
-    return kPIt - h * kPHalf < kPHalf ? kPIt - h * kPHalf : kPHalf;
-}
-__device__ __forceinline__ int pair_quad(int lane, int i) { return 4 * ((lane & 15) + 16 * i) + (lane >> 4); }
-
-// Global loads of part H of row `wave` of chunk `chunk` (table slot sl).
-template <int H>
-__device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, int chunk, int sl, int wave, int lane,
-                                                pair_build& B)
-{
-    const pair_tables& T = ptab();
-    const int nrows = __builtin_amdgcn_readfirstlane(T.nrows[sl]);
-    if (wave >= nrows) return;
-    const int2 rv = T.row[sl][wave];
-    const int ch = __builtin_amdgcn_readfirstlane(rv.x);
-    const int v = __builtin_amdgcn_readfirstlane(rv.y);
-    const int base = __builtin_amdgcn_readfirstlane(T.rbase[sl][wave]);
-    const int c0 = chunk * kPairChunk;
-    const uint8_t* ra = p.L + (size_t)(c0 + ch) * p.pitch + t0 + base;
-    const bool hasb = v >= 0 && c0 + ch + 1 < p.nchans;
-    const uint8_t* rb = p.L + (size_t)(c0 + ch + 1) * p.pitch + t0 + ((base + (hasb ? v : 0)) & ~3);
+    const pair_row& R = rec->row[wave];
+    const uint8_t* ra = p.L + t0 + R.offa;
+    const uint8_t* rb = p.L + t0 + R.offb;
+    B.sh = R.sh;
+    B.mb = R.mb;
 #pragma unroll
-    for (int i = 0; i < pair_part_items(H); ++i) {
-        const int q = pair_quad(lane, H * kPHalf + i);
-        uint32_t a0 = 0, a1 = 0, b0 = 0, b1 = 0, b2 = 0;
-        if (q < kPRQ) {
-            const uint2 av = *reinterpret_cast<const uint2*>(ra + 8 * q);
-            a0 = av.x;
-            a1 = av.y;
-            if (hasb) {
-                const uint32_t* bp = reinterpret_cast<const uint32_t*>(rb + 8 * q);
-                b0 = bp[0];
-                b1 = bp[1];
-                b2 = bp[2];
-            }
-        }
-        B.xa[i][0] = a0;
-        B.xa[i][1] = a1;
-        B.xb[i][0] = b0;
-        B.xb[i][1] = b1;
-        B.xb[i][2] = b2;
+    for (int i = 0; i < NI; ++i) {
+        const int q = lane + 64 * i;
+        const uint32_t* pa = reinterpret_cast<const uint32_t*>(ra + 4 * q);
+        const uint32_t* pb = reinterpret_cast<const uint32_t*>(rb + 4 * q);
+        B.al[i] = pa[0];
+        B.ah[i] = pa[kPSplit / 4];
+        B.bl[i][0] = pb[0];
+        B.bl[i][1] = pb[1];
+        B.bh[i][0] = pb[kPSplit / 4];
+        B.bh[i][1] = pb[kPSplit / 4 + 1];
     }
 }
 
-// Pair sums of part H of row `wave`, stored into `rows`.
-template <int H>
-__device__ __forceinline__ void pair_build_store(int sl, int wave, int lane, const pair_build& B, uint4* rows)
+// Pair sums of row `wave` into `rows`: dword m of quad q is
+// (a[4q+m] + b[4q+m]) | (a[512+4q+m] + b[512+4q+m]) << 16.  Whole items are
+// stored (rows are 320 quads wide).
+template <int NI>
+__device__ __forceinline__ void pair_build_store(int wave, int lane, const pair_build<NI>& B, uint4* rows)
 {
-    const pair_tables& T = ptab();
-    const int nrows = __builtin_amdgcn_readfirstlane(T.nrows[sl]);
-    if (wave >= nrows) return;
-    const uint32_t sh = (uint32_t)(__builtin_amdgcn_readfirstlane(T.row[sl][wave].y) & 3);  // -1 (single): xb = 0
-    uint4* r = rows + wave * kPRQ + (lane >> 4) * kPPlane + (lane & 15);
+    uint4* r = rows + wave * kPRQ + lane;
 #pragma unroll
-    for (int i = 0; i < pair_part_items(H); ++i) {
-        const int ii = H * kPHalf + i;
-        if (pair_quad(lane, ii) < kPRQ) {
-            const uint32_t x0 = __builtin_amdgcn_alignbyte(B.xb[i][1], B.xb[i][0], sh);
-            const uint32_t x1 = __builtin_amdgcn_alignbyte(B.xb[i][2], B.xb[i][1], sh);
-            uint4 o;
-            o.x = __builtin_amdgcn_perm(0u, B.xa[i][0], 0x0c010c00u) + __builtin_amdgcn_perm(0u, x0, 0x0c010c00u);
-            o.y = __builtin_amdgcn_perm(0u, B.xa[i][0], 0x0c030c02u) + __builtin_amdgcn_perm(0u, x0, 0x0c030c02u);
-            o.z = __builtin_amdgcn_perm(0u, B.xa[i][1], 0x0c010c00u) + __builtin_amdgcn_perm(0u, x1, 0x0c010c00u);
-            o.w = __builtin_amdgcn_perm(0u, B.xa[i][1], 0x0c030c02u) + __builtin_amdgcn_perm(0u, x1, 0x0c030c02u);
-            r[16 * ii] = o;   // slot (lane % 16) + 16 ii of plane lane / 16
-        }
+    for (int i = 0; i < NI; ++i) {
+        const uint32_t bl = __builtin_amdgcn_alignbyte(B.bl[i][1], B.bl[i][0], B.sh) & B.mb;
+        const uint32_t bh = __builtin_amdgcn_alignbyte(B.bh[i][1], B.bh[i][0], B.sh) & B.mb;
+        // byte m of the low word -> bits 0..7, byte m of the high word -> bits 16..23
+        uint4 o;
+        o.x = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c040c00u) + __builtin_amdgcn_perm(bh, bl, 0x0c040c00u);
+        o.y = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c050c01u) + __builtin_amdgcn_perm(bh, bl, 0x0c050c01u);
+        o.z = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c060c02u) + __builtin_amdgcn_perm(bh, bl, 0x0c060c02u);
+        o.w = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c070c03u) + __builtin_amdgcn_perm(bh, bl, 0x0c070c03u);
+        r[64 * i] = o;
     }
 }
 
-// acc16[j] += the 16 dwords (32 samples) of a lane run whose five quads
-// are q[m], SD dwords and H half-dwords into the first quad.
-template <int SD, int H>
-__device__ __forceinline__ void pair_accumulate(const uint4* const (&q)[5], uint32_t (&acc)[kR / 2])
+// acc16[j] += row dword k = e + L + 64 j of the entries at dword offsets o0
+// (and o1): two rows per v_add3.
+__device__ __forceinline__ void pair_add2(const uint32_t* base, int o0, int o1, uint32_t (&acc)[kPR / 2])
 {
-    constexpr int nq = (SD + 16 + H + 3) / 4;
-    uint32_t w[20];
+    const uint32_t* r0 = base + o0;
+    const uint32_t* r1 = base + o1;
+    uint32_t x0[kPR / 2], x1[kPR / 2];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        u32x4 t = {0u, 0u, 0u, 0u};
-        if (m < nq) t = *reinterpret_cast<const u32x4*>(q[m]);
-        w[4 * m] = t.x;
-        w[4 * m + 1] = t.y;
-        w[4 * m + 2] = t.z;
-        w[4 * m + 3] = t.w;
+    for (int j = 0; j < kPR / 2; ++j) {
+        x0[j] = r0[64 * j];
+        x1[j] = r1[64 * j];
     }
 #pragma unroll
-    for (int j = 0; j < kR / 2; ++j) {
-        const uint32_t x = H ? __builtin_amdgcn_alignbyte(w[SD + j + 1], w[SD + j], 2u) : w[SD + j];
-        asm("v_add_u32 %0, %0, %1" : "+v"(acc[j]) : "v"(x));
-    }
+    for (int j = 0; j < kPR / 2; ++j) acc[j] += x0[j] + x1[j];
 }
 
-// All rows of the wave's entries (lane l < 16: entry l) whose sub-quad
-// offset is SUB: a plain loop per offset class, so the accumulators stay in
-// their registers (no switch joins).
-template <int SUB>
-__device__ __forceinline__ void pair_sum_class(uint64_t m, int row_v, int e_v, int lane, const uint4* rows,
-                                               uint32_t (&acc)[kR / 2])
+__device__ __forceinline__ void pair_add1(const uint32_t* base, int o0, uint32_t (&acc)[kPR / 2])
 {
-    while (m) {
-        const int l = __builtin_ctzll(m);
-        m &= m - 1;
-        const int row = __builtin_amdgcn_readlane(row_v, l);
-        const int e8 = __builtin_amdgcn_readlane(e_v, l) >> 3;
-        // quad 4*lane + e8 + k of the row: plane (e8 + k) & 3, slot lane + (e8 + k) / 4
-        const uint4* rb = rows + row * kPRQ + lane;
-        const uint4* q[5];
+    const uint32_t* r0 = base + o0;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) q[k] = rb + ((e8 + k) & 3) * kPPlane + ((e8 + k) >> 2);
-        pair_accumulate<(SUB >> 1), (SUB & 1)>(q, acc);
+    for (int j = 0; j < kPR / 2; ++j) acc[j] += r0[64 * j];
+}
+
+// The entries of this wave's DM in chunk record rc, two per v_add3.  They
+// are read with scalar loads (wave-uniform addresses), which the compiler
+// waits for separately from the vector loads of the rows in flight.
+__device__ __forceinline__ void pair_sum(const pair_rec& rc, int wave, int lane, const uint4* rows,
+                                         uint32_t (&acc)[kPR / 2])
+{
+    const uint32_t* base = reinterpret_cast<const uint32_t*>(rows) + lane;
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);   // two entries per dword
+    const int n = rc.nent;
+    if (n == kPairs) {
+        uint32_t e[kPairs / 2];
+#pragma unroll
+        for (int i = 0; i < kPairs / 2; ++i) e[i] = ew[i];
+#pragma unroll
+        for (int i = 0; i < kPairs / 2; ++i) pair_add2(base, (int)(e[i] & 0xffffu), (int)(e[i] >> 16), acc);
+        return;
+    }
+    int i = 0;
+    for (; i + 1 < n; i += 2) {
+        const uint32_t e = ew[i >> 1];
+        pair_add2(base, (int)(e & 0xffffu), (int)(e >> 16), acc);
+    }
+    if (i < n) pair_add1(base, (int)(ew[i >> 1] & 0xffffu), acc);
+}
+
+__device__ __forceinline__ void pair_flush(uint32_t (&acc16)[kPR / 2], uint32_t (&acc32)[kPR])
+{
+#pragma unroll
+    for (int j = 0; j < kPR / 2; ++j) {
+        acc32[j] += acc16[j] & 0xFFFFu;
+        acc32[kPR / 2 + j] += acc16[j] >> 16;
+        acc16[j] = 0;
     }
 }
 
-template <int SUB>
-__device__ __forceinline__ void pair_sum_classes(bool valid, int sub, int row_v, int e_v, int lane,
-                                                 const uint4* rows, uint32_t (&acc)[kR / 2])
-{
-    pair_sum_class<SUB>(__builtin_amdgcn_ballot_w64(valid && sub == SUB), row_v, e_v, lane, rows, acc);
-    if constexpr (SUB + 1 < 8) pair_sum_classes<SUB + 1>(valid, sub, row_v, e_v, lane, rows, acc);
-}
-
-// Pairs [P0, P0 + kPairs / kPParts) of chunk slot sl: lanes 0..3 (or the
-// part's pairs) hold the first row of each pair, lanes 8.. the second row
-// of split pairs.
-template <int P0>
-__device__ __forceinline__ void pair_sum_half(int sl, int wave, int lane, const uint4* rows, uint32_t (&acc)[kR / 2])
-{
-    constexpr int NP = kPairs / kPParts;
-    const pair_tables& T = ptab();
-    const int pp = P0 + (lane & (NP - 1));
-    const int2 en = T.ent[sl][wave * kPairs + pp];
-    const bool second = (lane & 8) != 0;
-    const int row = second ? ((en.x >> 8) & 0xff) : (en.x & 0xff);
-    // host-checked to lie in [0, kPairMaxOffset]; clamped so that a row read
-    // can never leave the buffer
-    const int e = min(max(second ? ((en.y >> 16) & 0xffff) : (en.y & 0xffff), 0), kPairMaxOffset);
-    const bool valid = (lane & 7) < NP && lane < 16 && row < kPRows;
-    pair_sum_classes<0>(valid, e & 7, row, e, lane, rows, acc);
-}
-
-// Part H: load part H of chunk k+1's build, sum pairs of part H of chunk k,
-// store the build part.
-template <int H>
-__device__ __forceinline__ void pair_part(const lds_params& p, long t0, int k, bool more, bool active, int sn,
-                                          int wave, int lane, const uint4* rows_cur, uint4* rows_next,
-                                          uint32_t (&acc16)[kR / 2])
-{
-    pair_build B;
-    if (more) pair_build_load<H>(p, t0, k + 1, sn, wave, lane, B);
-    if (active) pair_sum_half<H * (kPairs / kPParts)>(k % 3, wave, lane, rows_cur, acc16);
-    if (more) pair_build_store<H>(sn, wave, lane, B, rows_next);
-}
-
-__device__ __forceinline__ void pair_step(const lds_params& p, long t0, int d0, int k, int nchunks, bool active,
-                                          int wave, int lane, int tid, const uint4* rows_cur, uint4* rows_next,
-                                          uint32_t (&acc16)[kR / 2], uint32_t (&acc32)[kR])
-{
-    const bool more = k + 1 < nchunks;
-    const int sn = (k + 1) % 3;
-    pair_part<0>(p, t0, k, more, active, sn, wave, lane, rows_cur, rows_next, acc16);
-    pair_part<1>(p, t0, k, more, active, sn, wave, lane, rows_cur, rows_next, acc16);
-    if (active && ((k + 1) % kPFlushChunks == 0 || !more)) flush(acc16, acc32);
-    if (wave == 0 && k + 2 < nchunks) pair_table(p, d0, k + 2, (k + 2) % 3, lane);
-    __syncthreads();
-}
-
-__device__ __forceinline__ void process_pair_tile(const lds_params& p, int d0, long t0)
+// Software pipeline over the chunks: step k stores chunk k+1's rows from
+// the registers loaded during step k-1, issues the loads of chunk k+2, sums
+// chunk k, and ends with the chunk's one barrier; a row's loads thus have a
+// whole step to arrive.  NI: quads per lane of a row (the range's row width
+// rounded up to 64 quads).
+template <int NI>
+__device__ __forceinline__ void process_pair_tile(const lds_params& p, int dt, long t0)
 {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const bool active = d0 + wave < p.ndms;
+    const bool active = dt * kDms + wave < p.ndms;
     const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
-    uint32_t acc16[kR / 2];
-    uint32_t acc32[kR];
+    const pair_rec* recs = reinterpret_cast<const pair_rec*>(p.pair_recs) + (size_t)dt * nchunks;
+    uint32_t acc16[kPR / 2];
+    uint32_t acc32[kPR];
 #pragma unroll
-    for (int j = 0; j < kR / 2; ++j) acc16[j] = 0;
+    for (int j = 0; j < kPR / 2; ++j) acc16[j] = 0;
 #pragma unroll
-    for (int j = 0; j < kR; ++j) acc32[j] = 0;
+    for (int j = 0; j < kPR; ++j) acc32[j] = 0;
 
-    if (wave == 0) {
-        pair_table(p, d0, 0, 0, lane);
-        if (nchunks > 1) pair_table(p, d0, 1, 1, lane);
-    }
+    pair_build<NI> B;
+    pair_build_load(p, t0, recs, wave, lane, B);
+    if (wave < recs[0].nrows) pair_build_store(wave, lane, B, g_prow0);
+    const int k1 = min(1, nchunks - 1);
+    bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
+    pair_build_load(p, t0, recs + k1, wave, lane, B);
     __syncthreads();
-    {
-        pair_build B;
-        pair_build_load<0>(p, t0, 0, 0, wave, lane, B);
-        pair_build_store<0>(0, wave, lane, B, g_rows0);
-        pair_build_load<1>(p, t0, 0, 0, wave, lane, B);
-        pair_build_store<1>(0, wave, lane, B, g_rows0);
-    }
-    __syncthreads();
-    for (int k = 0; k < nchunks; k += 2) {
-        pair_step(p, t0, d0, k, nchunks, active, wave, lane, tid, g_rows0, g_rows1, acc16, acc32);
-        if (k + 1 >= nchunks) break;
-        pair_step(p, t0, d0, k + 1, nchunks, active, wave, lane, tid, g_rows1, g_rows0, acc16, acc32);
+    for (int k = 0; k < nchunks; ++k) {
+        const uint4* rows_cur = (k & 1) ? g_prow1 : g_prow0;
+        uint4* rows_next = (k & 1) ? g_prow0 : g_prow1;
+        if (have_next) pair_build_store(wave, lane, B, rows_next);
+        const int k2 = min(k + 2, nchunks - 1);
+        have_next = k + 2 < nchunks && wave < recs[k2].nrows;
+        pair_build_load(p, t0, recs + k2, wave, lane, B);
+        if (active) {
+            pair_sum(recs[k], wave, lane, rows_cur, acc16);
+            if ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks) pair_flush(acc16, acc32);
+        }
+        __syncthreads();
     }
 
     if (active) {
-        const int d = d0 + wave;
-        const long tl = t0 + (long)kR * lane;
+        const int d = dt * kDms + wave;
         float* orow = p.out + (size_t)d * p.out_pitch;
 #pragma unroll
-        for (int j = 0; j < kR; ++j) {
-            const long tt = tl + j;
-            if (tt < p.t_processed) orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[j], p.fnchans), p.fbin);
-        }
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int j = 0; j < kPR / 2; ++j) {
+                const long tt = t0 + h * kPSplit + lane + 64 * j;
+                if (tt < p.t_processed)
+                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * (kPR / 2) + j], p.fnchans), p.fbin);
+            }
     }
 }
 
+template <int NI>
 __global__ __launch_bounds__(kThreads) void dedisperse_pair_kernel(lds_params p)
 {
     const int ntiles = p.n_dm_tiles * p.n_t_tiles;
     const int per_xcd = (ntiles + 7) / 8;
     const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (w >= ntiles) return;  // whole work-group leaves before any barrier
-    process_pair_tile(p, (w % p.n_dm_tiles) * kDms, (long)(w / p.n_dm_tiles) * kT);
+    process_pair_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kPT);
 }
 
 // One range: a flat grid of tiles (DM-major within a time tile).
@@ -777,12 +666,13 @@ __global__ void batch_reset_kernel(int* counters)
 
 }  // namespace
 
-static bool pair_plan_fits(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep);
+static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
+                     std::vector<pair_rec>* recs);
 
 int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
 {
     if (variant < 0 || variant >= kLdsVariants) return -1;
-    if (variant == 2) return pair_plan_fits(h_dmshifts, nchans, ndms, mstartdm, mdmstep) ? kPRQ : -1;
+    if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
     // 16*floor(e/16) + 32*lane.
@@ -804,12 +694,34 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
     return quads > cap ? -1 : (int)quads;
 }
 
-// Pair shape: every window offset of every tile fits a row (the same
-// arithmetic as pair_table; rows per chunk never exceed 16 by construction).
-static bool pair_plan_fits(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
+// Pair shape plan: quads per row (window offsets up to e_max need dwords up
+// to e_max + 511), or -1 when some offset exceeds a row or a pair's relative
+// delay is negative.  With recs, also the records
+// of every (DM tile, chunk), DM-tile-major, for rows `pitch` bytes apart.
+// Delays follow the kernels' rule (truncation of
+// dmshift * fmaf(d, mdmstep, mstartdm) in float).
+static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
+                     std::vector<pair_rec>* recs)
 {
+    auto make_row = [pitch](int a, int base, int b, int vb) {
+        // channel a's window at base; channel b (or none: b < 0) at base + vb
+        pair_row r{};
+        r.offa = (long long)a * (long long)pitch + base;
+        const int bb = b >= 0 ? b : a;
+        const int wb = b >= 0 ? base + vb : base;
+        r.offb = (long long)bb * (long long)pitch + (wb & ~3);
+        r.sh = (uint32_t)(wb & 3);
+        r.mb = b >= 0 ? ~0u : 0u;
+        return r;
+    };
+    if (nchans <= 0 || ndms <= 0) return -1;
+    const int nchunks = (nchans + kPairChunk - 1) / kPairChunk;
+    const int ndt = (ndms + kDms - 1) / kDms;
+    if (recs) recs->assign((size_t)ndt * nchunks, pair_rec{});
     std::vector<int> dl((size_t)kDms * nchans), d0l((size_t)nchans);
-    for (int d0 = 0; d0 < ndms; d0 += kDms) {
+    int e_max = 0;
+    for (int dt = 0; dt < ndt; ++dt) {
+        const int d0 = dt * kDms;
         const float s0 = __builtin_fmaf((float)d0, mdmstep, mstartdm);
         for (int c = 0; c < nchans; ++c) d0l[(size_t)c] = (int)(h_dmshifts[c] * s0);
         for (int w = 0; w < kDms; ++w) {
@@ -817,17 +729,74 @@ static bool pair_plan_fits(const float* h_dmshifts, int nchans, int ndms, float 
             const float sd = __builtin_fmaf((float)d, mdmstep, mstartdm);
             for (int c = 0; c < nchans; ++c) dl[(size_t)w * nchans + c] = (int)(h_dmshifts[c] * sd);
         }
-        for (int c = 0; c < nchans; ++c) {
-            const int base = d0l[(size_t)c] & ~7;
-            for (int w = 0; w < kDms; ++w) {
-                const int e = dl[(size_t)w * nchans + c] - base;
-                if (e < 0 || e > kPairMaxOffset) return false;
-                if ((c & 1) == 0 && c + 1 < nchans && dl[(size_t)w * nchans + c + 1] < dl[(size_t)w * nchans + c])
-                    return false;   // relative delay of a pair must not be negative
+        auto D = [&](int w, int c) { return dl[(size_t)w * nchans + c]; };
+        for (int k = 0; k < nchunks; ++k) {
+            pair_rec* R = recs ? &(*recs)[(size_t)dt * nchunks + k] : nullptr;
+            int nrows = 0, npairs = 0, nsplit = 0;
+            int split_row[kPairs], split_e[kPairs][kDms];
+            for (int pp = 0; pp < kPairs; ++pp) {
+                const int a = k * kPairChunk + 2 * pp;
+                if (a >= nchans) break;
+                ++npairs;
+                const bool hasb = a + 1 < nchans;
+                const int basea = d0l[(size_t)a] & ~3;
+                int ea[kDms], v[kDms], vmin = 0, vmax = 0;
+                for (int w = 0; w < kDms; ++w) {
+                    ea[w] = D(w, a) - basea;
+                    if (ea[w] < 0 || ea[w] > kPairMaxOffset) return -1;
+                    e_max = std::max(e_max, ea[w]);
+                    v[w] = hasb ? D(w, a + 1) - D(w, a) : 0;
+                    if (v[w] < 0) return -1;   // relative delay of a pair must not be negative
+                    vmin = w ? std::min(vmin, v[w]) : v[w];
+                    vmax = w ? std::max(vmax, v[w]) : v[w];
+                }
+                const int nv = vmax - vmin + 1;
+                if (nv <= 2) {
+                    if (R) {
+                        for (int i = 0; i < nv; ++i)
+                            R->row[nrows + i] = make_row(a, basea, hasb ? a + 1 : -1, vmin + i);
+                        for (int w = 0; w < kDms; ++w)
+                            R->ent[w][pp] = (uint16_t)((nrows + v[w] - vmin) * kPRowDw + ea[w]);
+                    }
+                    nrows += nv;
+                } else {   // split: two single-channel rows
+                    const int baseb = d0l[(size_t)a + 1] & ~3;
+                    for (int w = 0; w < kDms; ++w) {
+                        split_e[nsplit][w] = D(w, a + 1) - baseb;
+                        if (split_e[nsplit][w] < 0 || split_e[nsplit][w] > kPairMaxOffset) return -1;
+                        e_max = std::max(e_max, split_e[nsplit][w]);
+                    }
+                    if (R) {
+                        R->row[nrows] = make_row(a, basea, -1, 0);
+                        R->row[nrows + 1] = make_row(a + 1, baseb, -1, 0);
+                        for (int w = 0; w < kDms; ++w) R->ent[w][pp] = (uint16_t)(nrows * kPRowDw + ea[w]);
+                    }
+                    split_row[nsplit++] = nrows + 1;
+                    nrows += 2;
+                }
+            }
+            if (R) {
+                for (int s = 0; s < nsplit; ++s)
+                    for (int w = 0; w < kDms; ++w)
+                        R->ent[w][npairs + s] = (uint16_t)(split_row[s] * kPRowDw + split_e[s][w]);
+                R->nrows = nrows;
+                R->nent = npairs + nsplit;
             }
         }
     }
-    return true;
+    return (e_max + kPSplit + 3) / 4;
+}
+
+int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
+                    std::vector<uint8_t>* bytes)
+{
+    std::vector<pair_rec> recs;
+    const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, bytes ? &recs : nullptr);
+    if (q > 0 && bytes) {
+        bytes->resize(recs.size() * sizeof(pair_rec));
+        std::memcpy(bytes->data(), recs.data(), bytes->size());
+    }
+    return q;
 }
 
 long lds_tile_samples() { return kT; }
@@ -837,10 +806,12 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
 {
     const int cap = variant == 0 ? shape_c32::kRowQuads : variant == 1 ? shape_c16::kRowQuads : kPRQ;
     if (variant < 0 || variant >= kLdsVariants || p.row_quads <= 0 || p.row_quads > cap) return hipErrorInvalidValue;
+    if (variant == 2 && !p.pair_recs) return hipErrorInvalidValue;
     if (p.ndms <= 0 || p.t_processed <= 0) return hipSuccess;
     lds_params q = p;
+    const long tile = variant == 2 ? kPT : kT;
     q.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
-    q.n_t_tiles = (int)((p.t_processed + kT - 1) / kT);
+    q.n_t_tiles = (int)((p.t_processed + tile - 1) / tile);
     const long ntiles = (long)q.n_dm_tiles * q.n_t_tiles;
     const long grid = 8 * ((ntiles + 7) / 8);
     if (grid > 0x7fffffffL) return hipErrorInvalidValue;
@@ -848,8 +819,19 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
         hipLaunchKernelGGL(dedisperse_lds_kernel<shape_c32>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
     else if (variant == 1)
         hipLaunchKernelGGL(dedisperse_lds_kernel<shape_c16>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
-    else
-        hipLaunchKernelGGL(dedisperse_pair_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+    else {
+        // quads per lane: the row width rounded up to whole 64-quad items
+        // (a row is at least kPSplit / 4 = 128 quads)
+        const int ni = (p.row_quads + 63) / 64;
+        if (ni <= 2)
+            hipLaunchKernelGGL(dedisperse_pair_kernel<2>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+        else if (ni == 3)
+            hipLaunchKernelGGL(dedisperse_pair_kernel<3>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+        else if (ni == 4)
+            hipLaunchKernelGGL(dedisperse_pair_kernel<4>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+        else
+            hipLaunchKernelGGL(dedisperse_pair_kernel<5>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+    }
     return hipGetLastError();
 }
 
