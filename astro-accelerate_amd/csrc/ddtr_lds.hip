@@ -418,8 +418,10 @@ struct pair_rec {
 static_assert(sizeof(pair_row) == 32 && sizeof(pair_rec) % 16 == 0, "records are 16-byte aligned");
 static_assert(kPRows * kPRowDw <= 65536, "entry offsets fit 16 bits");
 
-__shared__ uint4 g_prow0[kPRows * kPRQ];   // 2 x 80 KiB: the whole LDS
-__shared__ uint4 g_prow1[kPRows * kPRQ];
+// Two row buffers of 80 KiB (the whole LDS) in one array: buffer b starts
+// at quad b * kPRows * kPRQ.
+__shared__ uint4 g_prow[2 * kPRows * kPRQ];
+__device__ __forceinline__ uint4* pair_buf(int b) { return g_prow + b * (kPRows * kPRQ); }
 
 // Registers of one row in flight: quads q = lane + 64 i, i < NI.
 template <int NI>
@@ -428,16 +430,15 @@ struct pair_build {
     uint32_t sh, mb;
 };
 
-// Loads of row `wave` of a chunk (record rec), issued whatever the row's
-// use (no register merges on a branch): a wave without a row in the chunk
-// reads the zeroed record's window at the level start.  A lane over-reads at
+// Loads of one row (R: its record), issued whatever the row's use (no
+// register merges on a branch): a wave without a row in the chunk reads the
+// zeroed record's window at the level start.  A lane over-reads at
 // most 63 quads (252 bytes) past the window's needed width, inside the
 // 4096-byte slack every row has past its longest window.
 template <int NI>
-__device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, const pair_rec* rec, int wave,
-                                                int lane, pair_build<NI>& B)
+__device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, const pair_row& R, int lane,
+                                                pair_build<NI>& B)
 {
-    const pair_row& R = rec->row[wave];
     const uint8_t* ra = p.L + t0 + R.offa;
     const uint8_t* rb = p.L + t0 + R.offb;
     B.sh = R.sh;
@@ -500,29 +501,55 @@ __device__ __forceinline__ void pair_add1(const uint32_t* base, int o0, uint32_t
     for (int j = 0; j < kPR / 2; ++j) acc[j] += r0[64 * j];
 }
 
-// The entries of this wave's DM in chunk record rc, two per v_add3.  They
-// are read with scalar loads (wave-uniform addresses), which the compiler
-// waits for separately from the vector loads of the rows in flight.
-__device__ __forceinline__ void pair_sum(const pair_rec& rc, int wave, int lane, const uint4* rows,
-                                         uint32_t (&acc)[kPR / 2])
+// Four rows' dwords of a lane (entries e01, e23), read ahead of their adds.
+struct pair_x4 {
+    uint32_t x[4][kPR / 2];
+};
+
+__device__ __forceinline__ void pair_read4(const uint32_t* base, uint32_t e01, uint32_t e23, pair_x4& X)
+{
+    const uint32_t* r[4] = {base + (e01 & 0xffffu), base + (e01 >> 16), base + (e23 & 0xffffu), base + (e23 >> 16)};
+#pragma unroll
+    for (int j = 0; j < kPR / 2; ++j)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) X.x[m][j] = r[m][64 * j];
+}
+
+__device__ __forceinline__ void pair_acc4(const pair_x4& X, uint32_t (&acc)[kPR / 2])
+{
+#pragma unroll
+    for (int j = 0; j < kPR / 2; ++j) acc[j] += X.x[0][j] + X.x[1][j];
+#pragma unroll
+    for (int j = 0; j < kPR / 2; ++j) acc[j] += X.x[2][j] + X.x[3][j];
+}
+
+// Entries of one DM in one chunk (16-bit dword offsets, two per dword),
+// held in scalar registers.
+struct pair_ents {
+    uint32_t e[kPMaxEnt / 2];
+    int n;
+};
+
+__device__ __forceinline__ pair_ents pair_ents_of(const pair_rec& rc, int wave)
+{
+    pair_ents E;
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
+#pragma unroll
+    for (int i = 0; i < kPMaxEnt / 2; ++i) E.e[i] = ew[i];
+    E.n = rc.nent;
+    return E;
+}
+
+// All entries of this wave's DM (any count).
+__device__ __forceinline__ void pair_sum(const pair_ents& E, int lane, const uint4* rows, uint32_t (&acc)[kPR / 2])
 {
     const uint32_t* base = reinterpret_cast<const uint32_t*>(rows) + lane;
-    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);   // two entries per dword
-    const int n = rc.nent;
-    if (n == kPairs) {
-        uint32_t e[kPairs / 2];
-#pragma unroll
-        for (int i = 0; i < kPairs / 2; ++i) e[i] = ew[i];
-#pragma unroll
-        for (int i = 0; i < kPairs / 2; ++i) pair_add2(base, (int)(e[i] & 0xffffu), (int)(e[i] >> 16), acc);
-        return;
-    }
     int i = 0;
-    for (; i + 1 < n; i += 2) {
-        const uint32_t e = ew[i >> 1];
+    for (; i + 1 < E.n; i += 2) {
+        const uint32_t e = E.e[i >> 1];
         pair_add2(base, (int)(e & 0xffffu), (int)(e >> 16), acc);
     }
-    if (i < n) pair_add1(base, (int)(ew[i >> 1] & 0xffffu), acc);
+    if (i < E.n) pair_add1(base, (int)(E.e[i >> 1] & 0xffffu), acc);
 }
 
 __device__ __forceinline__ void pair_flush(uint32_t (&acc16)[kPR / 2], uint32_t (&acc32)[kPR])
@@ -555,24 +582,43 @@ __device__ __forceinline__ void process_pair_tile(const lds_params& p, int dt, l
 #pragma unroll
     for (int j = 0; j < kPR; ++j) acc32[j] = 0;
 
+    // Scalar prefetches: the row record of the chunk whose loads the next
+    // step issues, and the entries of the chunk the next step sums; read
+    // just before the barrier, whose wait covers their latency.
     pair_build<NI> B;
-    pair_build_load(p, t0, recs, wave, lane, B);
-    if (wave < recs[0].nrows) pair_build_store(wave, lane, B, g_prow0);
+    pair_build_load(p, t0, recs[0].row[wave], lane, B);
+    if (wave < recs[0].nrows) pair_build_store(wave, lane, B, pair_buf(0));
     const int k1 = min(1, nchunks - 1);
     bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
-    pair_build_load(p, t0, recs + k1, wave, lane, B);
+    pair_build_load(p, t0, recs[k1].row[wave], lane, B);
+    int kr = min(2, nchunks - 1);
+    pair_row rn = recs[kr].row[wave];                     // row record of chunk k+2
+    int nrows_n = recs[kr].nrows;
+    pair_ents E = pair_ents_of(recs[0], wave);
     __syncthreads();
+    const uint32_t* lane_dw = reinterpret_cast<const uint32_t*>(pair_buf(0)) + lane;
     for (int k = 0; k < nchunks; ++k) {
-        const uint4* rows_cur = (k & 1) ? g_prow1 : g_prow0;
-        uint4* rows_next = (k & 1) ? g_prow0 : g_prow1;
+        const uint32_t* base = lane_dw + ((k & 1) ? kPRows * kPRowDw : 0);   // rows of chunk k
+        uint4* rows_next = pair_buf((k + 1) & 1);
         if (have_next) pair_build_store(wave, lane, B, rows_next);
-        const int k2 = min(k + 2, nchunks - 1);
-        have_next = k + 2 < nchunks && wave < recs[k2].nrows;
-        pair_build_load(p, t0, recs + k2, wave, lane, B);
+        have_next = k + 2 < nchunks && wave < nrows_n;
+        pair_build_load(p, t0, rn, lane, B);
         if (active) {
-            pair_sum(recs[k], wave, lane, rows_cur, acc16);
-            if ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks) pair_flush(acc16, acc32);
+            if (E.n == kPairs) {   // the usual full chunk: straight-line code
+                pair_x4 X;
+                pair_read4(base, E.e[0], E.e[1], X);
+                pair_acc4(X, acc16);
+                pair_read4(base, E.e[2], E.e[3], X);
+                pair_acc4(X, acc16);
+            } else {
+                pair_sum(E, lane, pair_buf(k & 1), acc16);
+            }
         }
+        if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) pair_flush(acc16, acc32);
+        kr = min(k + 3, nchunks - 1);
+        rn = recs[kr].row[wave];
+        nrows_n = recs[kr].nrows;
+        E = pair_ents_of(recs[min(k + 1, nchunks - 1)], wave);
         __syncthreads();
     }
 
