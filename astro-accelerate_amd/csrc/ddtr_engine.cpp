@@ -73,7 +73,8 @@ int check_delays(const float *dmshifts, int nchans, int ndms, float mstartdm, fl
 
 lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant) {
     lds_choice c{-1, -1};
-    for (int v = 0; v < kLdsVariants; ++v) {
+    static const int order[kLdsVariants] = {2, 0, 1};
+    for (int v : order) {
         if (variant >= 0 && v != variant) continue;
         const int q = lds_row_quads(v, h_dmshifts, nchans, ndms, mstartdm, mdmstep);
         if (q > 0) {
@@ -337,6 +338,15 @@ int ddtr_engine::prepare_range(int r, int t_processed, float *d_out, size_t out_
     }
     rl->pair_recs = t;
     return AA_HIP_SUCCESS;
+}
+
+lds_choice ddtr_engine::byte_shape(int r) const {
+    float mstartdm, mdmstep;
+    range_scalars(r, &mstartdm, &mdmstep);
+    const int ndms = m_ranges[(size_t)r].ndms;
+    lds_choice c = choose_lds(m_dmshifts.data(), m_nchans, ndms, mstartdm, mdmstep, 0);
+    if (c.variant < 0) c = choose_lds(m_dmshifts.data(), m_nchans, ndms, mstartdm, mdmstep, 1);
+    return c;
 }
 
 int ddtr_engine::set_range_variant(int r, int variant) {

@@ -42,7 +42,8 @@ struct lds_choice {
     int variant;
     int row_quads;
 };
-// First shape (or only `variant` when >= 0) whose rows hold the range's windows.
+// First shape (or only `variant` when >= 0) whose rows hold the range's
+// windows, tried in the order 2 (pair pre-summation, the fastest), 0, 1.
 lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant);
 
 // One range, resolved to a kernel: variant >= 0 = LDS kernel with that chunk
@@ -94,9 +95,15 @@ public:
     // level on `s`, validation, kernel choice); the launch itself is left to
     // the caller (launch_range, or a batched launch).
     int prepare_range(int r, int t_processed, float *d_out, size_t out_pitch, hipStream_t s, range_launch *rl);
-    // LDS chunk shape preferred for range r (-1 = first that fits); used when
-    // it fits the range's delay spread.
+    // LDS chunk shape preferred for range r (-1 = first that fits, in the
+    // order pair shape, 32-channel, 16-channel); used when it fits the
+    // range's delay spread.
     int set_range_variant(int r, int variant);
+    int range_variant_pref(int r) const {
+        return r >= 0 && r < (int)m_variant_pref.size() ? m_variant_pref[(size_t)r] : -1;
+    }
+    // First of the byte-row shapes (32-channel, 16-channel) that fits range r.
+    lds_choice byte_shape(int r) const;
     // Times every LDS chunk shape that fits range r (reps launches each, after
     // one untimed launch; output written to d_out) and keeps the fastest as
     // the range's preference.  ms[v] = average launch time of shape v, or -1
