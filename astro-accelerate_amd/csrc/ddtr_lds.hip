@@ -540,16 +540,20 @@ __device__ __forceinline__ pair_ents pair_ents_of(const pair_rec& rc, int wave)
     return E;
 }
 
-// All entries of this wave's DM (any count).
-__device__ __forceinline__ void pair_sum(const pair_ents& E, int lane, const uint4* rows, uint32_t (&acc)[kPR / 2])
+// All n entries of DM slot `wave` in chunk record rc, read from the record
+// itself (a register array indexed at run time would live in scratch
+// memory), two rows per v_add3.
+__device__ __forceinline__ void pair_sum(const pair_rec& rc, int wave, int n, int lane, const uint4* rows,
+                                         uint32_t (&acc)[kPR / 2])
 {
     const uint32_t* base = reinterpret_cast<const uint32_t*>(rows) + lane;
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
     int i = 0;
-    for (; i + 1 < E.n; i += 2) {
-        const uint32_t e = E.e[i >> 1];
+    for (; i + 1 < n; i += 2) {
+        const uint32_t e = ew[i >> 1];
         pair_add2(base, (int)(e & 0xffffu), (int)(e >> 16), acc);
     }
-    if (i < E.n) pair_add1(base, (int)(E.e[i >> 1] & 0xffffu), acc);
+    if (i < n) pair_add1(base, (int)(ew[i >> 1] & 0xffffu), acc);
 }
 
 __device__ __forceinline__ void pair_flush(uint32_t (&acc16)[kPR / 2], uint32_t (&acc32)[kPR])
@@ -611,7 +615,7 @@ __device__ __forceinline__ void process_pair_tile(const lds_params& p, int dt, l
                 pair_read4(base, E.e[2], E.e[3], X);
                 pair_acc4(X, acc16);
             } else {
-                pair_sum(E, lane, pair_buf(k & 1), acc16);
+                pair_sum(recs[k], wave, E.n, lane, pair_buf(k & 1), acc16);
             }
         }
         if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) pair_flush(acc16, acc32);
