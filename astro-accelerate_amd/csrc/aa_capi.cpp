@@ -362,19 +362,20 @@ int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nch
     if (nchans <= 0 || ndms < 0 || t_processed < 0 || inBin <= 0 || row_len < 0 || (size_t)row_len > pitch ||
         out_pitch < (size_t)t_processed)
         return bad("invalid geometry");
-    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_PAIR) return bad("unknown kernel variant");
+    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_QUAD) return bad("unknown kernel variant");
     const int st = aa_hip::check_delays(h_dmshifts, nchans, ndms, mstartdm, mdmstep, t_processed, row_len);
     if (st != AA_HIP_SUCCESS) return st;
     aa_hip::range_launch rl;
     const int rs = aa_hip::resolve_launch(kernel, d_in, pitch, nchans, d_dmshifts, h_dmshifts, ndms, t_processed,
                                           mstartdm, mdmstep, inBin, d_out, out_pitch, aa_hip::lds_choice{-2, 0}, &rl);
     if (rs != AA_HIP_SUCCESS) return rs;
-    if (rl.variant != 2) return from_hip(aa_hip::launch_range(rl, (hipStream_t)stream), "dedisperse");
-    // Chunk shape 2 reads plan tables; this handle-free call builds them,
+    if (rl.variant < 2) return from_hip(aa_hip::launch_range(rl, (hipStream_t)stream), "dedisperse");
+    // Chunk shapes 2 and 3 read plan tables; this handle-free call builds them,
     // launches, waits for the stream and frees them (handles keep theirs).
     std::vector<uint8_t> bytes;
-    if (aa_hip::lds_pair_tables(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, &bytes) != rl.row_quads)
-        return bad("pair plan tables disagree with the chosen row width");
+    if (aa_hip::lds_plan_tables(rl.variant, h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, &bytes) !=
+        rl.row_quads)
+        return bad("plan tables disagree with the chosen row width");
     void *t = nullptr;
     hipError_t e = hipMalloc(&t, bytes.size());
     if (e == hipSuccess) e = hipMemcpy(t, bytes.data(), bytes.size(), hipMemcpyHostToDevice);

@@ -100,10 +100,10 @@ int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float
             if (rl.ndms <= 0 || rl.t_processed <= 0) continue;
             // The persistent kernel holds the two byte-row chunk shapes; a
             // range left to the automatic choice takes one of them when it
-            // fits.  Ranges tuned to the pair shape (whose registers would
-            // make every batched tile spill) and ranges no shape fits run as
-            // launches of their own.
-            if (rl.variant == 2 && e->range_variant_pref(r) < 0) {
+            // fits.  Ranges tuned to a pre-summation shape (whose registers
+            // would make every batched tile spill) and ranges no shape fits
+            // run as launches of their own.
+            if (rl.variant >= 2 && e->range_variant_pref(r) < 0) {
                 const lds_choice c = e->byte_shape(r);
                 if (c.variant >= 0) {
                     rl.variant = c.variant;

@@ -73,7 +73,7 @@ int check_delays(const float *dmshifts, int nchans, int ndms, float mstartdm, fl
 
 lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant) {
     lds_choice c{-1, -1};
-    static const int order[kLdsVariants] = {2, 0, 1};
+    static const int order[kLdsVariants] = {2, 3, 0, 1};
     for (int v : order) {
         if (variant >= 0 && v != variant) continue;
         const int q = lds_row_quads(v, h_dmshifts, nchans, ndms, mstartdm, mdmstep);
@@ -105,6 +105,7 @@ int resolve_launch(int kernel, const uint8_t *L, size_t pitch, int nchans, const
     const int forced = kernel == AA_HIP_KERNEL_LDS_C32    ? 0
                        : kernel == AA_HIP_KERNEL_LDS_C16  ? 1
                        : kernel == AA_HIP_KERNEL_LDS_PAIR ? 2
+                       : kernel == AA_HIP_KERNEL_LDS_QUAD ? 3
                                                           : -1;
     if (kernel != AA_HIP_KERNEL_DIRECT && (pitch % 16) == 0 && ((uintptr_t)L % 16) == 0) {
         if (forced >= 0) choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, forced);
@@ -219,12 +220,12 @@ ddtr_engine::~ddtr_engine() {
         if (lv.ptr) (void)hipFree(lv.ptr);
     if (m_d_dmshifts) (void)hipFree(m_d_dmshifts);
     if (m_d_err) (void)hipFree(m_d_err);
-    for (void *t : m_pair_recs)
-        if (t) (void)hipFree(t);
+    for (const plan_tables &t : m_tables)
+        if (t.ptr) (void)hipFree(t.ptr);
 }
 
 int ddtr_engine::set_kernel(int kernel) {
-    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_PAIR)
+    if (kernel < AA_HIP_KERNEL_AUTO || kernel > AA_HIP_KERNEL_LDS_QUAD)
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "unknown kernel variant");
     m_kernel = kernel;
     return AA_HIP_SUCCESS;
@@ -317,25 +318,35 @@ int ddtr_engine::prepare_range(int r, int t_processed, float *d_out, size_t out_
     }
     st = resolve_launch(m_kernel, lv.ptr, lv.pitch, m_nchans, m_d_dmshifts, m_dmshifts.data(), rd.ndms,
                         t_processed, mstartdm, mdmstep, rd.inBin, d_out, out_pitch, c, rl);
-    if (st != AA_HIP_SUCCESS || rl->variant != 2) return st;
-    // Chunk shape 2 reads plan tables; they depend on the range only, so
-    // they are built once (on the first call that needs them, never while a
-    // later call is being captured into a graph).
-    if (m_pair_recs.size() != m_ranges.size()) m_pair_recs.assign(m_ranges.size(), nullptr);
-    void *&t = m_pair_recs[(size_t)r];
-    if (!t) {
+    if (st != AA_HIP_SUCCESS || rl->variant < 2) return st;
+    // Chunk shapes 2 and 3 read plan tables; they depend on the range and
+    // shape only, so they are built once (on the first call that needs them,
+    // never while a later call is being captured into a graph).
+    if (m_tables.size() != m_ranges.size()) m_tables.assign(m_ranges.size(), plan_tables{});
+    plan_tables &pt = m_tables[(size_t)r];
+    if (pt.variant != rl->variant) {
+        if (pt.ptr) {
+            (void)hipFree(pt.ptr);
+            m_device_bytes -= pt.bytes;
+            pt = plan_tables{};
+        }
         std::vector<uint8_t> bytes;
-        if (lds_pair_tables(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, lv.pitch, &bytes) != rl->row_quads)
-            return fail(AA_HIP_ERROR_UNSUPPORTED, "pair plan tables disagree with the chosen row width");
+        if (lds_plan_tables(rl->variant, m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, lv.pitch, &bytes) !=
+            rl->row_quads)
+            return fail(AA_HIP_ERROR_UNSUPPORTED, "plan tables disagree with the chosen row width");
+        void *t = nullptr;
         hipError_t e = hipMalloc(&t, bytes.size());
         if (e == hipSuccess) e = hipMemcpy(t, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             if (t) (void)hipFree(t);
-            t = nullptr;
-            return hip_fail(e, "pair plan tables");
+            return hip_fail(e, "plan tables");
         }
+        pt.variant = rl->variant;
+        pt.ptr = t;
+        pt.bytes = bytes.size();
         m_device_bytes += bytes.size();
     }
+    void *t = pt.ptr;
     rl->pair_recs = t;
     return AA_HIP_SUCCESS;
 }

@@ -43,7 +43,8 @@ struct lds_choice {
     int row_quads;
 };
 // First shape (or only `variant` when >= 0) whose rows hold the range's
-// windows, tried in the order 2 (pair pre-summation, the fastest), 0, 1.
+// windows, tried in the order 2 (pair pre-summation, the fastest on the
+// 8-range 4096-channel plan), 3, 0, 1.
 lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant);
 
 // One range, resolved to a kernel: variant >= 0 = LDS kernel with that chunk
@@ -59,7 +60,7 @@ struct range_launch {
     size_t out_pitch = 0;
     int variant = -1;
     int row_quads = 0;
-    const void *pair_recs = nullptr;  // chunk shape 2: device plan tables
+    const void *pair_recs = nullptr;  // chunk shapes 2, 3: device plan tables
 };
 
 class ddtr_engine {
@@ -96,8 +97,8 @@ public:
     // the caller (launch_range, or a batched launch).
     int prepare_range(int r, int t_processed, float *d_out, size_t out_pitch, hipStream_t s, range_launch *rl);
     // LDS chunk shape preferred for range r (-1 = first that fits, in the
-    // order pair shape, 32-channel, 16-channel); used when it fits the
-    // range's delay spread.
+    // order pair shape, quad shape, 32-channel, 16-channel); used when it
+    // fits the range's delay spread.
     int set_range_variant(int r, int variant);
     int range_variant_pref(int r) const {
         return r >= 0 && r < (int)m_variant_pref.size() ? m_variant_pref[(size_t)r] : -1;
@@ -142,7 +143,12 @@ private:
     int m_resident = -1;  // highest level valid for the current chunk
     std::vector<lds_choice> m_lds;     // per range: chosen LDS shape (-2 = not computed, -1 = direct)
     std::vector<int> m_variant_pref;   // per range: preferred LDS shape, -1 = any
-    std::vector<void *> m_pair_recs;   // per range: chunk shape 2 plan tables (device), built on first use
+    struct plan_tables {
+        int variant = -1;  // chunk shape the tables are for
+        void *ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::vector<plan_tables> m_tables;  // per range: chunk shape 2/3 plan tables (device), built on first use
     size_t m_device_bytes = 0;
 };
 

@@ -13,8 +13,9 @@ namespace aa_hip {
 // 16-channel shape holds rows twice as long in the same buffers.
 constexpr int kLdsRowQuadsMax = 152;
 // Chunk shapes: 0 = 32 channels x 152 quads, 1 = 16 channels x 304 quads,
-// 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums).
-constexpr int kLdsVariants = 3;
+// 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums), 3 = 16
+// channels pre-summed in groups of four.
+constexpr int kLdsVariants = 4;
 
 struct lds_params {
     const uint8_t* L;        // nchans rows of u8 samples
@@ -28,7 +29,7 @@ struct lds_params {
     float* out;
     size_t out_pitch;        // floats between output rows
     int row_quads;           // staged row width in uint4 (from lds_row_quads)
-    const void* pair_recs;   // chunk shape 2: device plan tables (from lds_pair_tables)
+    const void* pair_recs;   // chunk shapes 2, 3: device plan tables (from lds_plan_tables)
     int n_dm_tiles, n_t_tiles;  // filled by the launcher / lds_job_tiles
 };
 
@@ -52,6 +53,11 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
 // width in quads, or -1 when the range does not fit the shape.  Host only.
 int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
                     std::vector<uint8_t>* bytes);
+
+// Plan tables of chunk shape 2 or 3 (as lds_pair_tables; -1 for other
+// shapes, which need none).
+int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep,
+                    size_t pitch, std::vector<uint8_t>* bytes);
 
 // Tile geometry: samples per tile and DM trials per tile.  Rows must hold
 // lds_tile_samples() readable bytes past the last processed sample plus the

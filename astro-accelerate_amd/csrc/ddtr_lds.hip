@@ -650,6 +650,222 @@ __global__ __launch_bounds__(kThreads) void dedisperse_pair_kernel(lds_params p)
     process_pair_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kPT);
 }
 
+// ---------------------------------------------------------------------------
+// Quad pre-summation (chunk shape 3).  As the pair shape, with rows that sum
+// the four channels of a group: in a 16-DM tile the three delays of channels
+// c+1, c+2, c+3 relative to c take a few combinations (about 3.4), and one
+// row per combination,
+//     S[i] = L[c][base + i] + sum_m L[c+m][base + v_m + i],
+// serves every DM whose combination it is, at offset e = delay(c, d) - base.
+// A DM then adds 4 rows per 16-channel chunk instead of 8: half the LDS
+// reads of the pair shape, for about twice its build work.  A group with
+// more than four combinations is built as two pairs (or single channels).
+// Rows are narrower (at most 192 quads: window offsets up to 256 samples),
+// so the two buffers take 96 KiB.  Accumulator layout, plan records (host
+// computed, one per DM tile and chunk) and the chunk pipeline are the pair
+// shape's.
+// ---------------------------------------------------------------------------
+constexpr int kQIt = 3;                        // quads per lane per row, at most
+constexpr int kQRQ = 64 * kQIt;                // 192 quads per row
+constexpr int kQRowDw = 4 * kQRQ;              // 768 dwords per row
+constexpr int kQRows = 16;                     // rows per buffer (one per wave)
+constexpr int kQMaxOffset = kQRowDw - kPSplit; // e + 511 < kQRowDw
+constexpr int kQGroups = kPairChunk / 4;       // 4-channel groups per chunk
+constexpr int kQMaxGroupRows = 4;
+static_assert(kQGroups * kQMaxGroupRows == kQRows && kQRows == kDms, "one row per wave");
+// Four rows of up to 4 x 255 per chunk: 4080 per 16-bit field, flushed
+// every kPFlushChunks chunks.
+static_assert(kPFlushChunks * 4 * 4 * 255 <= 65535, "16-bit fields do not overflow");
+
+struct quad_row {
+    long long off[4];   // channel m: bytes from the level start to the dword floor of its window
+    uint32_t sh;        // byte shift of channel m's window in bits 8m..8m+1 (channel 0: 0)
+    uint32_t nch;       // channels summed: 4, 2 or 1 (unused channels repeat channel 0's window)
+    int pad0, pad1;
+};
+struct quad_rec {
+    uint16_t ent[kDms][kPMaxEnt];  // per DM slot: dword offset (row * kQRowDw + e) of each entry
+    quad_row row[kQRows];
+    int nrows, nent, pad0, pad1;
+};
+static_assert(sizeof(quad_row) == 48 && sizeof(quad_rec) % 16 == 0, "records are 16-byte aligned");
+static_assert(kQRows * kQRowDw <= 65536, "entry offsets fit 16 bits");
+
+__shared__ uint4 g_qrow[2 * kQRows * kQRQ];   // 2 x 48 KiB
+__device__ __forceinline__ uint4* quad_buf(int b) { return g_qrow + b * (kQRows * kQRQ); }
+
+// Registers of one row in flight: channel 0's low and high dwords (its
+// window is dword aligned), and for channels 1-3 two dwords each around the
+// low and the high sample.
+template <int NI>
+struct quad_load {
+    uint32_t a[NI][2];
+    uint32_t x[NI][3][4];
+    uint32_t sh, nch;
+};
+
+// Loads of one row, issued whatever the row's use (no register merges on a
+// branch; a wave without a row reads its zeroed record's window at the level
+// start).  A lane over-reads at most 63 quads past the needed width, inside
+// the 4096-byte slack every row has past its longest window.
+template <int NI>
+__device__ __forceinline__ void quad_issue_loads(const lds_params& p, long t0, const quad_row& R, int lane,
+                                                 quad_load<NI>& B)
+{
+    const uint8_t* r[4] = {p.L + t0 + R.off[0], p.L + t0 + R.off[1], p.L + t0 + R.off[2], p.L + t0 + R.off[3]};
+    B.sh = R.sh;
+    B.nch = R.nch;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int q = lane + 64 * i;
+        const uint32_t* pa = reinterpret_cast<const uint32_t*>(r[0] + 4 * q);
+        B.a[i][0] = pa[0];
+        B.a[i][1] = pa[kPSplit / 4];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const uint32_t* pm = reinterpret_cast<const uint32_t*>(r[m + 1] + 4 * q);
+            B.x[i][m][0] = pm[0];
+            B.x[i][m][1] = pm[1];
+            B.x[i][m][2] = pm[kPSplit / 4];
+            B.x[i][m][3] = pm[kPSplit / 4 + 1];
+        }
+    }
+}
+
+// Row sums into `rows` (row `wave`): dword j of quad q is the 16-bit sums of
+// the channels' byte j of the low dword and of the high dword.  MASK: rows
+// of fewer than four channels (the unused ones are masked off).
+template <int NI, bool MASK>
+__device__ __forceinline__ void quad_store_row(int wave, int lane, const quad_load<NI>& B, uint4* rows)
+{
+    uint4* r = rows + wave * kQRQ + lane;
+    uint32_t mask[3];   // channel m + 1 summed
+#pragma unroll
+    for (int m = 0; m < 3; ++m) mask[m] = !MASK || B.nch > (uint32_t)(m + 1) ? ~0u : 0u;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        uint32_t lo[3], hi[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const uint32_t sh = (B.sh >> (8 * (m + 1))) & 3u;
+            const uint32_t mk = mask[m];
+            lo[m] = __builtin_amdgcn_alignbyte(B.x[i][m][1], B.x[i][m][0], sh) & mk;
+            hi[m] = __builtin_amdgcn_alignbyte(B.x[i][m][3], B.x[i][m][2], sh) & mk;
+        }
+        uint32_t o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t sel = 0x0c000c00u | ((uint32_t)(4 + j) << 16) | (uint32_t)j;
+            o[j] = __builtin_amdgcn_perm(B.a[i][1], B.a[i][0], sel) + __builtin_amdgcn_perm(hi[0], lo[0], sel) +
+                   __builtin_amdgcn_perm(hi[1], lo[1], sel) + __builtin_amdgcn_perm(hi[2], lo[2], sel);
+        }
+        r[64 * i] = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+}
+
+template <int NI>
+__device__ __forceinline__ void quad_store(int wave, int lane, const quad_load<NI>& B, uint4* rows)
+{
+    if (B.nch == 4) quad_store_row<NI, false>(wave, lane, B, rows);
+    else quad_store_row<NI, true>(wave, lane, B, rows);
+}
+
+// All n entries of DM slot `wave` in chunk record rc, read from the record.
+__device__ __forceinline__ void quad_sum_any(const quad_rec& rc, int wave, int n, const uint32_t* base,
+                                             uint32_t (&acc)[kPR / 2])
+{
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
+    for (int i = 0; i < n; ++i) {
+        const uint32_t e = ew[i >> 1];
+        pair_add1(base, (int)((i & 1) ? (e >> 16) : (e & 0xffffu)), acc);
+    }
+}
+
+__device__ __forceinline__ pair_ents quad_ents_of(const quad_rec& rc, int wave)
+{
+    pair_ents E;
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
+#pragma unroll
+    for (int i = 0; i < kPMaxEnt / 2; ++i) E.e[i] = ew[i];
+    E.n = rc.nent;
+    return E;
+}
+
+// The pair shape's pipeline: step k stores chunk k+1's rows (loaded during
+// step k-1), issues the loads of chunk k+2, sums chunk k, one barrier.
+template <int NI>
+__device__ __forceinline__ void process_quad_tile(const lds_params& p, int dt, long t0)
+{
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool active = dt * kDms + wave < p.ndms;
+    const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
+    const quad_rec* recs = reinterpret_cast<const quad_rec*>(p.pair_recs) + (size_t)dt * nchunks;
+    uint32_t acc16[kPR / 2];
+    uint32_t acc32[kPR];
+#pragma unroll
+    for (int j = 0; j < kPR / 2; ++j) acc16[j] = 0;
+#pragma unroll
+    for (int j = 0; j < kPR; ++j) acc32[j] = 0;
+
+    quad_load<NI> B;
+    quad_issue_loads(p, t0, recs[0].row[wave], lane, B);
+    if (wave < recs[0].nrows) quad_store(wave, lane, B, quad_buf(0));
+    const int k1 = min(1, nchunks - 1);
+    bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
+    quad_issue_loads(p, t0, recs[k1].row[wave], lane, B);
+    int kr = min(2, nchunks - 1);
+    quad_row rn = recs[kr].row[wave];                     // row record of chunk k+2
+    int nrows_n = recs[kr].nrows;
+    pair_ents E = quad_ents_of(recs[0], wave);
+    const uint32_t* lane_dw = reinterpret_cast<const uint32_t*>(quad_buf(0)) + lane;
+    __syncthreads();
+    for (int k = 0; k < nchunks; ++k) {
+        const uint32_t* base = lane_dw + ((k & 1) ? kQRows * kQRowDw : 0);   // rows of chunk k
+        if (have_next) quad_store(wave, lane, B, quad_buf((k + 1) & 1));
+        have_next = k + 2 < nchunks && wave < nrows_n;
+        quad_issue_loads(p, t0, rn, lane, B);
+        if (active) {
+            if (E.n == kQGroups) {   // the usual full chunk: four rows, one batch of reads
+                pair_x4 X;
+                pair_read4(base, E.e[0], E.e[1], X);
+                pair_acc4(X, acc16);
+            } else {
+                quad_sum_any(recs[k], wave, E.n, base, acc16);
+            }
+            if ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks) pair_flush(acc16, acc32);
+        }
+        kr = min(k + 3, nchunks - 1);
+        rn = recs[kr].row[wave];
+        nrows_n = recs[kr].nrows;
+        E = quad_ents_of(recs[min(k + 1, nchunks - 1)], wave);
+        __syncthreads();
+    }
+
+    if (active) {
+        const int d = dt * kDms + wave;
+        float* orow = p.out + (size_t)d * p.out_pitch;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int j = 0; j < kPR / 2; ++j) {
+                const long tt = t0 + h * kPSplit + lane + 64 * j;
+                if (tt < p.t_processed)
+                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * (kPR / 2) + j], p.fnchans), p.fbin);
+            }
+    }
+}
+
+template <int NI>
+__global__ __launch_bounds__(kThreads) void dedisperse_quad_kernel(lds_params p)
+{
+    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (w >= ntiles) return;  // whole work-group leaves before any barrier
+    process_quad_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kPT);
+}
+
 // One range: a flat grid of tiles (DM-major within a time tile).
 template <class S>
 __global__ __launch_bounds__(kThreads) void dedisperse_lds_kernel(lds_params p)
@@ -718,11 +934,14 @@ __global__ void batch_reset_kernel(int* counters)
 
 static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
                      std::vector<pair_rec>* recs);
+static int quad_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
+                     std::vector<quad_rec>* recs);
 
 int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
 {
     if (variant < 0 || variant >= kLdsVariants) return -1;
     if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
+    if (variant == 3) return quad_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
     // 16*floor(e/16) + 32*lane.
@@ -837,6 +1056,172 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
     return (e_max + kPSplit + 3) / 4;
 }
 
+// Quad shape plan (shape 3): as pair_plan, with 4-channel groups; a group
+// whose relative delays take more than four combinations in a tile is built
+// as two pairs (each as in the pair shape: one row per value of its relative
+// delay when there are at most two, else two single-channel rows).  Entry
+// slots: one per group in order, the further entries of split groups after
+// them.
+static int quad_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
+                     std::vector<quad_rec>* recs)
+{
+    if (nchans <= 0 || ndms <= 0) return -1;
+    const int nchunks = (nchans + kPairChunk - 1) / kPairChunk;
+    const int ndt = (ndms + kDms - 1) / kDms;
+    if (recs) recs->assign((size_t)ndt * nchunks, quad_rec{});
+    std::vector<int> dl((size_t)kDms * nchans), d0l((size_t)nchans);
+    int e_max = 0;
+    bool fits = true;
+    auto chan_off = [pitch](int c, int start) { return (long long)c * (long long)pitch + (start & ~3); };
+    for (int dt = 0; dt < ndt && fits; ++dt) {
+        const int d0 = dt * kDms;
+        const float s0 = __builtin_fmaf((float)d0, mdmstep, mstartdm);
+        for (int c = 0; c < nchans; ++c) d0l[(size_t)c] = (int)(h_dmshifts[c] * s0);
+        for (int w = 0; w < kDms; ++w) {
+            const int d = std::min(d0 + w, ndms - 1);
+            const float sd = __builtin_fmaf((float)d, mdmstep, mstartdm);
+            for (int c = 0; c < nchans; ++c) dl[(size_t)w * nchans + c] = (int)(h_dmshifts[c] * sd);
+        }
+        auto D = [&](int w, int c) { return dl[(size_t)w * nchans + c]; };
+        for (int k = 0; k < nchunks && fits; ++k) {
+            quad_rec* R = recs ? &(*recs)[(size_t)dt * nchunks + k] : nullptr;
+            int nrows = 0, ngroups = 0, nextra = 0;
+            int extra_ent[kPMaxEnt][kDms];   // further entries of split groups, in order
+            // One row: channels ch[0..nc) at windows base + rel[m] (rel[0] = 0).
+            auto add_row = [&](const int* ch, int nc, int base, const int* rel) {
+                if (R) {
+                    quad_row q{};
+                    q.nch = (uint32_t)nc;
+                    for (int m = 0; m < 4; ++m) {
+                        const int c = m < nc ? ch[m] : ch[0];
+                        const int start = m < nc ? base + rel[m] : base;
+                        q.off[m] = chan_off(c, start);
+                        q.sh |= (uint32_t)(start & 3) << (8 * m);
+                    }
+                    R->row[nrows] = q;
+                }
+                return nrows++;
+            };
+            auto offset_ok = [&](int e) {
+                if (e < 0 || e > kQMaxOffset) return false;
+                e_max = std::max(e_max, e);
+                return true;
+            };
+            // entries of a two-channel piece (first channel a, second b or -1):
+            // rows as in the pair shape; ent_out[w][0..n) the DM's entries.
+            auto add_pair = [&](int a, int b, int (*ent_out)[2], int* n_out) {
+                const int basea = d0l[(size_t)a] & ~3;
+                int v[kDms], vmin = 0, vmax = 0;
+                for (int w = 0; w < kDms; ++w) {
+                    if (!offset_ok(D(w, a) - basea)) return false;
+                    v[w] = b >= 0 ? D(w, b) - D(w, a) : 0;
+                    if (v[w] < 0) return false;
+                    vmin = w ? std::min(vmin, v[w]) : v[w];
+                    vmax = w ? std::max(vmax, v[w]) : v[w];
+                }
+                if (b < 0 || vmax - vmin <= 1) {
+                    int first = -1;
+                    for (int i = 0; i <= vmax - vmin; ++i) {
+                        const int ch[2] = {a, b};
+                        const int rel[2] = {0, vmin + i};
+                        const int r = add_row(ch, b >= 0 ? 2 : 1, basea, rel);
+                        if (first < 0) first = r;
+                    }
+                    for (int w = 0; w < kDms; ++w) ent_out[w][0] = (first + v[w] - vmin) * kQRowDw + D(w, a) - basea;
+                    *n_out = 1;
+                    return true;
+                }
+                const int baseb = d0l[(size_t)b] & ~3;
+                for (int w = 0; w < kDms; ++w)
+                    if (!offset_ok(D(w, b) - baseb)) return false;
+                const int rel0[1] = {0};
+                const int ra = add_row(&a, 1, basea, rel0);
+                const int rb = add_row(&b, 1, baseb, rel0);
+                for (int w = 0; w < kDms; ++w) {
+                    ent_out[w][0] = ra * kQRowDw + D(w, a) - basea;
+                    ent_out[w][1] = rb * kQRowDw + D(w, b) - baseb;
+                }
+                *n_out = 2;
+                return true;
+            };
+            for (int g = 0; g < kQGroups && fits; ++g) {
+                const int a = k * kPairChunk + 4 * g;
+                if (a >= nchans) break;
+                ++ngroups;
+                const int nc = std::min(4, nchans - a);
+                const int basea = d0l[(size_t)a] & ~3;
+                // combinations of relative delays over the tile's DMs
+                int combo[kDms], ncombo = 0, rels[kDms][4];
+                for (int w = 0; w < kDms && fits; ++w) {
+                    if (!offset_ok(D(w, a) - basea)) fits = false;
+                    rels[w][0] = 0;
+                    for (int m = 1; m < nc; ++m) {
+                        rels[w][m] = D(w, a + m) - D(w, a);
+                        if (rels[w][m] < 0) fits = false;   // relative delays must not be negative
+                    }
+                    combo[w] = -1;
+                    for (int u = 0; u < w && combo[w] < 0; ++u) {
+                        bool same = true;
+                        for (int m = 1; m < nc; ++m) same = same && rels[u][m] == rels[w][m];
+                        if (same) combo[w] = combo[u];
+                    }
+                    if (combo[w] < 0) combo[w] = ncombo++;
+                }
+                if (!fits) break;
+                if (ncombo <= kQMaxGroupRows) {
+                    int first = -1;
+                    for (int cb = 0; cb < ncombo; ++cb) {
+                        int w0 = 0;
+                        while (combo[w0] != cb) ++w0;
+                        const int ch[4] = {a, a + 1, a + 2, a + 3};
+                        const int r = add_row(ch, nc, basea, rels[w0]);
+                        if (first < 0) first = r;
+                    }
+                    if (R)
+                        for (int w = 0; w < kDms; ++w)
+                            R->ent[w][g] = (uint16_t)((first + combo[w]) * kQRowDw + D(w, a) - basea);
+                } else {   // two pairs
+                    int e1[kDms][2], e2[kDms][2], n1 = 0, n2 = 0;
+                    if (!add_pair(a, nc > 1 ? a + 1 : -1, e1, &n1)) { fits = false; break; }
+                    if (nc > 2 && !add_pair(a + 2, nc > 3 ? a + 3 : -1, e2, &n2)) { fits = false; break; }
+                    for (int w = 0; w < kDms; ++w) {
+                        if (R) R->ent[w][g] = (uint16_t)e1[w][0];
+                        int x = nextra;
+                        for (int i = 1; i < n1; ++i) extra_ent[x++][w] = e1[w][i];
+                        for (int i = 0; i < n2; ++i) extra_ent[x++][w] = e2[w][i];
+                    }
+                    nextra += (n1 - 1) + n2;
+                }
+            }
+            if (!fits) break;
+            if (R) {
+                for (int x = 0; x < nextra; ++x)
+                    for (int w = 0; w < kDms; ++w) R->ent[w][ngroups + x] = (uint16_t)extra_ent[x][w];
+                R->nrows = nrows;
+                R->nent = ngroups + nextra;
+            }
+        }
+    }
+    if (!fits) return -1;
+    return (e_max + kPSplit + 3) / 4;
+}
+
+int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep,
+                    size_t pitch, std::vector<uint8_t>* bytes)
+{
+    if (variant == 3) {
+        std::vector<quad_rec> recs;
+        const int q = quad_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, bytes ? &recs : nullptr);
+        if (q > 0 && bytes) {
+            bytes->resize(recs.size() * sizeof(quad_rec));
+            std::memcpy(bytes->data(), recs.data(), bytes->size());
+        }
+        return q;
+    }
+    if (variant == 2) return lds_pair_tables(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, bytes);
+    return -1;
+}
+
 int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
                     std::vector<uint8_t>* bytes)
 {
@@ -854,12 +1239,15 @@ int lds_tile_dms() { return kDms; }
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s)
 {
-    const int cap = variant == 0 ? shape_c32::kRowQuads : variant == 1 ? shape_c16::kRowQuads : kPRQ;
+    const int cap = variant == 0   ? shape_c32::kRowQuads
+                    : variant == 1 ? shape_c16::kRowQuads
+                    : variant == 2 ? kPRQ
+                                   : kQRQ;
     if (variant < 0 || variant >= kLdsVariants || p.row_quads <= 0 || p.row_quads > cap) return hipErrorInvalidValue;
-    if (variant == 2 && !p.pair_recs) return hipErrorInvalidValue;
+    if (variant >= 2 && !p.pair_recs) return hipErrorInvalidValue;
     if (p.ndms <= 0 || p.t_processed <= 0) return hipSuccess;
     lds_params q = p;
-    const long tile = variant == 2 ? kPT : kT;
+    const long tile = variant >= 2 ? kPT : kT;
     q.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
     q.n_t_tiles = (int)((p.t_processed + tile - 1) / tile);
     const long ntiles = (long)q.n_dm_tiles * q.n_t_tiles;
@@ -869,7 +1257,12 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
         hipLaunchKernelGGL(dedisperse_lds_kernel<shape_c32>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
     else if (variant == 1)
         hipLaunchKernelGGL(dedisperse_lds_kernel<shape_c16>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
-    else {
+    else if (variant == 3) {
+        if ((p.row_quads + 63) / 64 <= 2)
+            hipLaunchKernelGGL(dedisperse_quad_kernel<2>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+        else
+            hipLaunchKernelGGL(dedisperse_quad_kernel<3>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+    } else {
         // quads per lane: the row width rounded up to whole 64-quad items
         // (a row is at least kPSplit / 4 = 128 quads)
         const int ni = (p.row_quads + 63) / 64;

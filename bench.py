@@ -68,7 +68,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--tune-cache", default=None, help="config 4: JSON file of autotuned chunk shapes")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "direct", "lds", "c32", "c16", "pair"])
+    ap.add_argument("--kernel", default="auto", choices=["auto", "direct", "lds", "c32", "c16", "pair", "quad"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=16, help="DM rows per range in the CPU sample")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -93,7 +93,7 @@ def main():
     torch.cuda.set_device(dev)
     lib = aa.load_library()
     kernel = {"auto": aa.KERNEL_AUTO, "direct": aa.KERNEL_DIRECT, "lds": aa.KERNEL_LDS, "c32": aa.KERNEL_LDS_C32,
-              "c16": aa.KERNEL_LDS_C16, "pair": aa.KERNEL_LDS_PAIR}[args.kernel]
+              "c16": aa.KERNEL_LDS_C16, "pair": aa.KERNEL_LDS_PAIR, "quad": aa.KERNEL_LDS_QUAD}[args.kernel]
 
     if args.copy_calibration:
         src = torch.empty(1 << 30, dtype=torch.uint8, device=dev)

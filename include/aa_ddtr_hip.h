@@ -48,7 +48,8 @@ typedef enum {
     AA_HIP_KERNEL_LDS = 2,      /* LDS-staged windows, packed integer sums, first shape that fits */
     AA_HIP_KERNEL_LDS_C32 = 3,  /* LDS kernel, 32-channel chunks (rows up to 2432 bytes) */
     AA_HIP_KERNEL_LDS_C16 = 4,  /* LDS kernel, 16-channel chunks (rows up to 4864 bytes) */
-    AA_HIP_KERNEL_LDS_PAIR = 5  /* LDS kernel, 16-channel chunks pre-summed in channel pairs */
+    AA_HIP_KERNEL_LDS_PAIR = 5, /* LDS kernel, 16-channel chunks pre-summed in channel pairs */
+    AA_HIP_KERNEL_LDS_QUAD = 6  /* LDS kernel, 16-channel chunks pre-summed in groups of four */
 } aa_hip_kernel;
 
 /* Message for the calling thread's most recent aa_hip_* failure ("" if none). */
@@ -237,15 +238,15 @@ int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_o
 
 /* Tile tuning (no reference counterpart: the reference picks its kernel by a
  * fixed LDS-size test, src/aa_dedisperse.cpp:90-131).  The LDS kernel has
- * three channel-chunk shapes, 0 = 32 channels x 2432-byte rows, 1 = 16
- * channels x 4864-byte rows, 2 = 16 channels in pairs; set_range_variant
- * records a preference for
- * range r (-1 = first that fits), used whenever it fits; shape 2 sums
- * 16-channel chunks pre-added in channel pairs (rows of 16-bit pair sums,
- * one per distinct relative delay of the pair in a 16-DM tile).
+ * four channel-chunk shapes: 0 = 32 channels x 2432-byte rows, 1 = 16
+ * channels x 4864-byte rows, 2 = 16 channels pre-added in channel pairs
+ * (rows of 16-bit pair sums, one per distinct relative delay of the pair in
+ * a 16-DM tile), 3 = 16 channels pre-added in groups of four.
+ * set_range_variant records a preference for range r (-1 = first that
+ * fits, in the order 2, 3, 0, 1), used whenever it fits.
  * autotune_range times every shape that fits range r on `stream` (reps
  * launches each, writing d_out), keeps the fastest as the preference and
- * reports it: *best_variant (-1 if none fits) and variant_ms[0..2] (-1 =
+ * reports it: *best_variant (-1 if none fits) and variant_ms[0..3] (-1 =
  * does not fit).
  * The range's level must be resident (after ingest). */
 int aa_hip_ddtr_set_range_variant(aa_hip_ddtr *h, int range, int variant);

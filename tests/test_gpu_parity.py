@@ -12,8 +12,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 # AA_HIP_KERNEL_DIRECT, _LDS (first chunk shape that fits), _LDS_C32, _LDS_C16, _LDS_PAIR
-KERNELS = [1, 2, 3, 4, 5]
-KERNELS_FULL = [1, 2, 4, 5]
+KERNELS = [1, 2, 3, 4, 5, 6]
+KERNELS_FULL = [1, 2, 4, 5, 6]
 
 
 def say(msg):
@@ -103,7 +103,7 @@ def test_dedisperse_range(aa, oracle, cuda, geom, kernel):
                          out_pitch, kernel=kernel)
     except aa.AAHipError as e:
         # a forced chunk shape may not hold this geometry's windows: refused on the host
-        assert kernel in (3, 5) and e.status == aa.AA_HIP_ERROR_UNSUPPORTED, e
+        assert kernel in (3, 5, 6) and e.status == aa.AA_HIP_ERROR_UNSUPPORTED, e
         return
     got = out.cpu().numpy()
     np.testing.assert_array_equal(got[:, :tp], want)
