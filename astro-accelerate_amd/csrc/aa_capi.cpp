@@ -369,6 +369,17 @@ int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nch
     const int rs = aa_hip::resolve_launch(kernel, d_in, pitch, nchans, d_dmshifts, h_dmshifts, ndms, t_processed,
                                           mstartdm, mdmstep, inBin, d_out, out_pitch, aa_hip::lds_choice{-2, 0}, &rl);
     if (rs != AA_HIP_SUCCESS) return rs;
+    if (rl.variant >= 2 && kernel == AA_HIP_KERNEL_AUTO) {
+        // This handle-free call keeps no plan tables: left to the automatic
+        // choice, it takes a byte-row shape when one fits, which needs none
+        // and keeps the call asynchronous.
+        aa_hip::lds_choice c = aa_hip::choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0);
+        if (c.variant < 0) c = aa_hip::choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 1);
+        if (c.variant >= 0) {
+            rl.variant = c.variant;
+            rl.row_quads = c.row_quads;
+        }
+    }
     if (rl.variant < 2) return from_hip(aa_hip::launch_range(rl, (hipStream_t)stream), "dedisperse");
     // Chunk shapes 2 and 3 read plan tables; this handle-free call builds them,
     // launches, waits for the stream and frees them (handles keep theirs).

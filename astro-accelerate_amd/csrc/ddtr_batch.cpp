@@ -53,8 +53,10 @@ int ddtr_batch::create(int nbeams, ddtr_engine *const *beams, ddtr_batch **out) 
     hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->m_device);
     // one 1024-thread work-group per CU (its LDS use fills the CU)
     p->m_grid = 8 * std::max(1, (ncu + 7) / 8);
-    if (e == hipSuccess) e = hipMalloc(&p->m_d_bounds, 9 * sizeof(long));
-    if (e == hipSuccess) e = hipMalloc(&p->m_d_counters, 8 * sizeof(int));
+    for (auto &f : p->m_fam) {
+        if (e == hipSuccess) e = hipMalloc(&f.d_bounds, 9 * sizeof(long));
+        if (e == hipSuccess) e = hipMalloc(&f.d_counters, 8 * sizeof(int));
+    }
     if (e != hipSuccess) {
         const int st = hip_fail(e, "batch create");
         delete p;
@@ -67,9 +69,11 @@ int ddtr_batch::create(int nbeams, ddtr_engine *const *beams, ddtr_batch **out) 
 ddtr_batch::~ddtr_batch() {
     device_guard guard(m_device);
     release_graph();
-    if (m_d_jobs) (void)hipFree(m_d_jobs);
-    if (m_d_bounds) (void)hipFree(m_d_bounds);
-    if (m_d_counters) (void)hipFree(m_d_counters);
+    for (auto &f : m_fam) {
+        if (f.d_jobs) (void)hipFree(f.d_jobs);
+        if (f.d_bounds) (void)hipFree(f.d_bounds);
+        if (f.d_counters) (void)hipFree(f.d_counters);
+    }
     if (m_capture_stream) (void)hipStreamDestroy(m_capture_stream);
 }
 
@@ -81,12 +85,13 @@ void ddtr_batch::release_graph() {
     m_key.clear();
 }
 
-// Launches ingest and binning of every beam, resolves every range, and the
-// batched dedispersion.  Builds m_jobs; the caller uploads them (outside any
-// graph capture) before the batched kernel runs.
+// Launches ingest and binning of every beam and resolves every range into
+// the families' job lists (m_fam[*].jobs) and the separate launches
+// (m_direct); the caller uploads the job lists (outside any graph capture)
+// before the persistent kernels run.
 int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float *const *d_out,
                         const size_t *out_pitch, hipStream_t s) {
-    std::vector<lds_job> jobs;
+    std::vector<lds_job> jobs[2];
     std::vector<range_launch> direct;
     int flat = 0;
     for (size_t b = 0; b < m_beams.size(); ++b) {
@@ -98,41 +103,75 @@ int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float
             st = e->prepare_range(r, t_processed[flat], d_out[flat], out_pitch[flat], s, &rl);
             if (st != AA_HIP_SUCCESS) return st;
             if (rl.ndms <= 0 || rl.t_processed <= 0) continue;
-            // The persistent kernel holds the two byte-row chunk shapes; a
-            // range left to the automatic choice takes one of them when it
-            // fits.  Ranges tuned to a pre-summation shape (whose registers
-            // would make every batched tile spill) and ranges no shape fits
-            // run as launches of their own.
-            if (rl.variant >= 2 && e->range_variant_pref(r) < 0) {
-                const lds_choice c = e->byte_shape(r);
-                if (c.variant >= 0) {
-                    rl.variant = c.variant;
-                    rl.row_quads = c.row_quads;
-                }
-            }
-            if (rl.variant == 0 || rl.variant == 1) {
+            if (rl.variant == 0 || rl.variant == 1 || rl.variant == 2) {
                 lds_job j{};
                 j.p = lds_params_of(rl);
                 j.variant = rl.variant;
-                jobs.push_back(j);
+                lds_job_tiles(j.p, rl.variant);
+                jobs[rl.variant == 2 ? 1 : 0].push_back(j);
             } else {
                 direct.push_back(rl);
             }
         }
     }
-    // Largest tiles (most channels) first: the costliest work starts early
-    // and the queue tails are made of the cheapest tiles.
-    std::stable_sort(jobs.begin(), jobs.end(),
-                     [](const lds_job &a, const lds_job &b) { return a.p.nchans > b.p.nchans; });
-    long begin = 0;
-    for (auto &j : jobs) {
-        j.tile_begin = begin;
-        begin += (long)j.p.n_dm_tiles * j.p.n_t_tiles;
+    for (int f = 0; f < 2; ++f) {
+        // Largest tiles (most channels) first: the costliest work starts
+        // early and the queue tails are made of the cheapest tiles.
+        std::stable_sort(jobs[f].begin(), jobs[f].end(),
+                         [](const lds_job &a, const lds_job &b) { return a.p.nchans > b.p.nchans; });
+        long begin = 0;
+        for (auto &j : jobs[f]) {
+            j.tile_begin = begin;
+            begin += (long)j.p.n_dm_tiles * j.p.n_t_tiles;
+        }
+        if (begin >= 0x7fffffffL) return fail(AA_HIP_ERROR_UNSUPPORTED, "batch holds more than 2^31 tiles");
+        m_fam[f].jobs.swap(jobs[f]);
+        m_fam[f].ntiles = begin;
     }
-    m_jobs.swap(jobs);
     m_direct.swap(direct);
-    m_ntiles = begin;
     return AA_HIP_SUCCESS;
+}
+
+namespace {
+
+// Queue bounds: 8 contiguous slices of a family's tile list of about equal
+// work (tiles weighted by channel count).
+std::vector<long> queue_bounds(const std::vector<lds_job> &jobs, long ntiles) {
+    std::vector<long> bounds(9, ntiles);
+    double total = 0.0;
+    for (const auto &j : jobs) total += (double)j.p.nchans * j.p.n_dm_tiles * j.p.n_t_tiles;
+    bounds[0] = 0;
+    double acc = 0.0;
+    int x = 1;
+    for (const auto &j : jobs) {
+        const long n = (long)j.p.n_dm_tiles * j.p.n_t_tiles;
+        const double w = (double)j.p.nchans;
+        while (x < 8 && acc + w * n >= total * x / 8.0) {
+            const long cut = (long)std::ceil((total * x / 8.0 - acc) / w);
+            bounds[(size_t)x] = j.tile_begin + std::min(std::max(cut, 0L), n);
+            ++x;
+        }
+        acc += w * n;
+    }
+    for (; x < 8; ++x) bounds[(size_t)x] = ntiles;
+    for (int i = 1; i < 9; ++i) bounds[(size_t)i] = std::max(bounds[(size_t)i], bounds[(size_t)i - 1]);
+    return bounds;
+}
+
+}  // namespace
+
+// The persistent launch of each family that has jobs, then the separate
+// launches.  Reads only device-side job tables: capturable.
+hipError_t ddtr_batch::launch_families(hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (!m_fam[1].jobs.empty())
+        e = launch_dedisperse_pair_batch(m_fam[1].d_jobs, (int)m_fam[1].jobs.size(), m_fam[1].d_bounds,
+                                         m_fam[1].d_counters, m_grid, s);
+    if (e == hipSuccess && !m_fam[0].jobs.empty())
+        e = launch_dedisperse_lds_batch(m_fam[0].d_jobs, (int)m_fam[0].jobs.size(), m_fam[0].d_bounds,
+                                        m_fam[0].d_counters, m_grid, s);
+    for (size_t i = 0; e == hipSuccess && i < m_direct.size(); ++i) e = launch_range(m_direct[i], s);
+    return e;
 }
 
 int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float *const *d_out,
@@ -152,56 +191,36 @@ int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float
         const hipError_t e = hipGraphLaunch(m_exec, s);
         return e == hipSuccess ? AA_HIP_SUCCESS : hip_fail(e, "batch graph launch");
     }
-    release_graph();  // the eager pass rewrites the job table a captured graph reads
+    release_graph();  // the eager pass rewrites the job tables a captured graph reads
 
-    // Eager pass: builds and uploads the job list, and computes this call.
+    // Eager pass: builds and uploads the job lists, and computes this call.
     int st = enqueue(inputs, t_processed, d_out, out_pitch, s);
     if (st != AA_HIP_SUCCESS) return st;
-    if (m_jobs.size() > m_jobs_cap) {
-        if (m_d_jobs) (void)hipFree(m_d_jobs);
-        m_d_jobs = nullptr;
-        m_jobs_cap = 0;
-        const hipError_t e = hipMalloc(&m_d_jobs, m_jobs.size() * sizeof(lds_job));
-        if (e != hipSuccess) return hip_fail(e, "batch job table");
-        m_jobs_cap = m_jobs.size();
-    }
-    // Queue bounds: 8 contiguous slices of the tile list of about equal work
-    // (tiles weighted by channel count).
-    std::vector<long> bounds(9, m_ntiles);
-    {
-        double total = 0.0;
-        for (const auto &j : m_jobs) total += (double)j.p.nchans * j.p.n_dm_tiles * j.p.n_t_tiles;
-        bounds[0] = 0;
-        double acc = 0.0;
-        int x = 1;
-        for (const auto &j : m_jobs) {
-            const long n = (long)j.p.n_dm_tiles * j.p.n_t_tiles;
-            const double w = (double)j.p.nchans;
-            while (x < 8 && acc + w * n >= total * x / 8.0) {
-                const long cut = (long)std::ceil((total * x / 8.0 - acc) / w);
-                bounds[(size_t)x] = j.tile_begin + std::min(std::max(cut, 0L), n);
-                ++x;
-            }
-            acc += w * n;
-        }
-        for (; x < 8; ++x) bounds[(size_t)x] = m_ntiles;
-        for (int i = 1; i < 9; ++i) bounds[(size_t)i] = std::max(bounds[(size_t)i], bounds[(size_t)i - 1]);
-    }
     hipError_t e = hipSuccess;
-    if (!m_jobs.empty()) {
-        e = hipMemcpyAsync(m_d_jobs, m_jobs.data(), m_jobs.size() * sizeof(lds_job), hipMemcpyHostToDevice, s);
+    std::vector<long> bounds[2];
+    for (int f = 0; f < 2 && e == hipSuccess; ++f) {
+        family &F = m_fam[f];
+        if (F.jobs.empty()) continue;
+        if (F.jobs.size() > F.jobs_cap) {
+            if (F.d_jobs) (void)hipFree(F.d_jobs);
+            F.d_jobs = nullptr;
+            F.jobs_cap = 0;
+            e = hipMalloc(&F.d_jobs, F.jobs.size() * sizeof(lds_job));
+            if (e != hipSuccess) return hip_fail(e, "batch job table");
+            F.jobs_cap = F.jobs.size();
+        }
+        bounds[f] = queue_bounds(F.jobs, F.ntiles);
+        e = hipMemcpyAsync(F.d_jobs, F.jobs.data(), F.jobs.size() * sizeof(lds_job), hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(m_d_bounds, bounds.data(), 9 * sizeof(long), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);  // host vectors may change before the copies run
-        if (e == hipSuccess)
-            e = launch_dedisperse_lds_batch(m_d_jobs, (int)m_jobs.size(), m_d_bounds, m_d_counters, m_grid, s);
+            e = hipMemcpyAsync(F.d_bounds, bounds[f].data(), 9 * sizeof(long), hipMemcpyHostToDevice, s);
     }
-    for (size_t i = 0; e == hipSuccess && i < m_direct.size(); ++i) e = launch_range(m_direct[i], s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // host vectors may change before the copies run
+    if (e == hipSuccess) e = launch_families(s);
     if (e != hipSuccess) return hip_fail(e, "batch launch");
     if (!use_graph) return AA_HIP_SUCCESS;
 
-    // Capture the same sequence (the job table is already on the device and
-    // is the same for these buffers); later calls replay it.
+    // Capture the same sequence (the job tables are already on the device
+    // and are the same for these buffers); later calls replay it.
     // The legacy default stream cannot be captured: capture on a stream of
     // the batch's own (the graph is launched on the caller's stream).
     hipStream_t cs = s;
@@ -214,15 +233,13 @@ int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float
         if (e != hipSuccess) return hip_fail(e, "batch capture");
         cs = m_capture_stream;
     }
-    const size_t njobs = m_jobs.size();
+    const size_t n0 = m_fam[0].jobs.size(), n1 = m_fam[1].jobs.size(), nd = m_direct.size();
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return hip_fail(e, "batch capture");
     st = enqueue(inputs, t_processed, d_out, out_pitch, cs);
-    if (st == AA_HIP_SUCCESS && m_jobs.size() != njobs) st = fail(AA_HIP_ERROR_DEVICE, "batch changed in capture");
-    if (st == AA_HIP_SUCCESS && njobs)
-        e = launch_dedisperse_lds_batch(m_d_jobs, (int)njobs, m_d_bounds, m_d_counters, m_grid, cs);
-    for (size_t i = 0; st == AA_HIP_SUCCESS && e == hipSuccess && i < m_direct.size(); ++i)
-        e = launch_range(m_direct[i], cs);
+    if (st == AA_HIP_SUCCESS && (m_fam[0].jobs.size() != n0 || m_fam[1].jobs.size() != n1 || m_direct.size() != nd))
+        st = fail(AA_HIP_ERROR_DEVICE, "batch changed in capture");
+    if (st == AA_HIP_SUCCESS) e = launch_families(cs);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(cs, &g);
     if (st != AA_HIP_SUCCESS) {

@@ -1,14 +1,16 @@
 // Batched dedispersion of many beams (each its own plan: channel count,
-// length, DM ranges) in one persistent LDS-kernel launch, optionally
-// replayed from a captured hipGraph.  No counterpart in the reference, whose
+// length, DM ranges) in persistent LDS-kernel launches, optionally replayed
+// from a captured hipGraph.  No counterpart in the reference, whose
 // pipeline runs one beam per process (include/aa_pipeline_api.hpp:113-117).
 //
 // Per call: every beam's chunk is corner-turned into its engine's level 0 and
-// binned as its ranges need (the engines' own kernels), then all (beam,
-// range) jobs whose geometry fits an LDS chunk shape go to one persistent
-// launch (dedisperse_lds_batch_kernel) over a shared tile list; the rest
-// (if any) run the direct kernel afterwards.  Results are bit-identical to
-// running each engine's execute_chunk.
+// binned as its ranges need (the engines' own kernels), then the (beam,
+// range) jobs run in one persistent launch per kernel family over a shared
+// tile list: the pair pre-summation shape (dedisperse_pair_batch_kernel) and
+// the two byte-row shapes (dedisperse_lds_batch_kernel).  Ranges on the quad
+// shape or on the direct kernel (no LDS shape fits) run as launches of their
+// own afterwards.  Results are bit-identical to running each engine's
+// execute_chunk.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,8 +35,10 @@ public:
     int execute(const void *const *inputs, const int *t_processed, float *const *d_out, const size_t *out_pitch,
                 bool use_graph, hipStream_t s);
 
-    int njobs() const { return (int)m_jobs.size(); }
-    long ntiles() const { return m_ntiles; }
+    // jobs (ranges) and tiles in the persistent launches, ranges launched
+    // on their own
+    int njobs() const { return (int)(m_fam[0].jobs.size() + m_fam[1].jobs.size()); }
+    long ntiles() const { return m_fam[0].ntiles + m_fam[1].ntiles; }
     int ndirect() const { return (int)m_direct.size(); }
     int nranges_total() const { return m_nranges_total; }
 
@@ -42,19 +46,24 @@ private:
     ddtr_batch() = default;
     int enqueue(const void *const *inputs, const int *t_processed, float *const *d_out, const size_t *out_pitch,
                 hipStream_t s);
+    hipError_t launch_families(hipStream_t s);
     void release_graph();
 
     int m_device = 0;
     int m_grid = 0;
     std::vector<ddtr_engine *> m_beams;
     int m_nranges_total = 0;
-    std::vector<lds_job> m_jobs;
+    // One persistent launch per kernel family: 0 = byte-row shapes, 1 = pair shape.
+    struct family {
+        std::vector<lds_job> jobs;
+        long ntiles = 0;
+        lds_job *d_jobs = nullptr;
+        size_t jobs_cap = 0;
+        long *d_bounds = nullptr;  // 9 queue bounds
+        int *d_counters = nullptr; // 8 queue heads
+    };
+    family m_fam[2];
     std::vector<range_launch> m_direct;
-    long m_ntiles = 0;
-    lds_job *m_d_jobs = nullptr;
-    size_t m_jobs_cap = 0;
-    long *m_d_bounds = nullptr;
-    int *m_d_counters = nullptr;
     hipStream_t m_capture_stream = nullptr;  // for callers on the legacy default stream
     hipGraph_t m_graph = nullptr;
     hipGraphExec_t m_exec = nullptr;

@@ -138,7 +138,7 @@ lds_params lds_params_of(const range_launch &rl) {
     p.out_pitch = rl.out_pitch;
     p.row_quads = rl.row_quads;
     p.pair_recs = rl.pair_recs;
-    lds_job_tiles(p);
+    lds_job_tiles(p, rl.variant);
     return p;
 }
 
@@ -349,15 +349,6 @@ int ddtr_engine::prepare_range(int r, int t_processed, float *d_out, size_t out_
     void *t = pt.ptr;
     rl->pair_recs = t;
     return AA_HIP_SUCCESS;
-}
-
-lds_choice ddtr_engine::byte_shape(int r) const {
-    float mstartdm, mdmstep;
-    range_scalars(r, &mstartdm, &mdmstep);
-    const int ndms = m_ranges[(size_t)r].ndms;
-    lds_choice c = choose_lds(m_dmshifts.data(), m_nchans, ndms, mstartdm, mdmstep, 0);
-    if (c.variant < 0) c = choose_lds(m_dmshifts.data(), m_nchans, ndms, mstartdm, mdmstep, 1);
-    return c;
 }
 
 int ddtr_engine::set_range_variant(int r, int variant) {

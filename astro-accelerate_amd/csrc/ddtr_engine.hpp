@@ -100,11 +100,6 @@ public:
     // order pair shape, quad shape, 32-channel, 16-channel); used when it
     // fits the range's delay spread.
     int set_range_variant(int r, int variant);
-    int range_variant_pref(int r) const {
-        return r >= 0 && r < (int)m_variant_pref.size() ? m_variant_pref[(size_t)r] : -1;
-    }
-    // First of the byte-row shapes (32-channel, 16-channel) that fits range r.
-    lds_choice byte_shape(int r) const;
     // Times every LDS chunk shape that fits range r (reps launches each, after
     // one untimed launch; output written to d_out) and keeps the fastest as
     // the range's preference.  ms[v] = average launch time of shape v, or -1

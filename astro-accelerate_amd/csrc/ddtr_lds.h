@@ -37,7 +37,7 @@ struct lds_params {
 // tile_begin + n_dm_tiles * n_t_tiles - 1 of the batch's tile list.
 struct lds_job {
     lds_params p;
-    int variant;             // chunk shape (0 or 1: the shapes the batched kernel holds)
+    int variant;             // chunk shape: 0, 1 (dedisperse_lds_batch) or 2 (dedisperse_pair_batch)
     int pad;
     long tile_begin;
 };
@@ -67,8 +67,8 @@ int lds_tile_dms();
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s);
 
-// Fills p.n_dm_tiles and p.n_t_tiles.
-void lds_job_tiles(lds_params& p);
+// Fills p.n_dm_tiles and p.n_t_tiles for chunk shape `variant`.
+void lds_job_tiles(lds_params& p, int variant);
 
 // Persistent launch over jobs[0..njobs) (device array, sorted by tile_begin,
 // the first at 0); d_bounds[0..8] splits the tile list into 8 queues (one per
@@ -76,5 +76,9 @@ void lds_job_tiles(lds_params& p);
 // kernel, so the sequence may be captured in a graph).  grid: multiple of 8.
 hipError_t launch_dedisperse_lds_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
                                        int grid, hipStream_t s);
+// The same for jobs of chunk shape 2 (p.pair_recs set, tiles of the pair
+// shape); the tile count must stay below 2^31.
+hipError_t launch_dedisperse_pair_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
+                                        int grid, hipStream_t s);
 
 }  // namespace aa_hip

@@ -650,6 +650,55 @@ __global__ __launch_bounds__(kThreads) void dedisperse_pair_kernel(lds_params p)
     process_pair_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kPT);
 }
 
+// Pair-shape jobs of many ranges (of many beams) in one persistent launch,
+// scheduled as dedisperse_lds_batch_kernel schedules the byte-row shapes
+// (per-XCD queues of one shared tile list, stealing when a queue is empty).
+// The tile index is handed to the work-group through the last dword of the
+// row buffers, which no tile reads between tiles (every tile ends with a
+// barrier after its last LDS read, and writes that row last).
+__global__ __launch_bounds__(kThreads) void dedisperse_pair_batch_kernel(const lds_job* __restrict__ jobs, int njobs,
+                                                                         const long* __restrict__ bounds,
+                                                                         int* __restrict__ counters)
+{
+    int* slot = reinterpret_cast<int*>(g_prow + 2 * kPRows * kPRQ) - 1;
+    const int xcd = (int)(blockIdx.x % 8);
+    int q = 0;  // queues found empty so far (thread 0)
+    for (;;) {
+        if (threadIdx.x == 0) {
+            long item = -1;
+            while (q < 8) {
+                const int x = (xcd + q) & 7;
+                const long i = bounds[x] + (long)atomicAdd(&counters[x], 1);
+                if (i < bounds[x + 1]) {
+                    item = i;
+                    break;
+                }
+                ++q;
+            }
+            *slot = (int)item;   // the host keeps the tile count below 2^31
+        }
+        __syncthreads();
+        const long item = *slot;
+        __syncthreads();
+        if (item < 0) break;
+        int lo = 0, hi = njobs - 1;  // last job with tile_begin <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (jobs[mid].tile_begin <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const lds_job& J = jobs[lo];
+        const long local = item - J.tile_begin;
+        const int dt = (int)(local % J.p.n_dm_tiles);
+        const long t0 = (local / J.p.n_dm_tiles) * kPT;
+        const int ni = (J.p.row_quads + 63) / 64;
+        if (ni <= 2) process_pair_tile<2>(J.p, dt, t0);
+        else if (ni == 3) process_pair_tile<3>(J.p, dt, t0);
+        else if (ni == 4) process_pair_tile<4>(J.p, dt, t0);
+        else process_pair_tile<5>(J.p, dt, t0);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Quad pre-summation (chunk shape 3).  As the pair shape, with rows that sum
 // the four channels of a group: in a 16-DM tile the three delays of channels
@@ -1278,10 +1327,22 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
     return hipGetLastError();
 }
 
-void lds_job_tiles(lds_params& p)
+void lds_job_tiles(lds_params& p, int variant)
 {
+    const long tile = variant == 2 || variant == 3 ? kPT : kT;   // 16 DMs per tile in every shape
     p.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
-    p.n_t_tiles = (int)((p.t_processed + kT - 1) / kT);
+    p.n_t_tiles = (int)((p.t_processed + tile - 1) / tile);
+}
+
+hipError_t launch_dedisperse_pair_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
+                                        int grid, hipStream_t s)
+{
+    if (njobs <= 0) return hipSuccess;
+    if (grid <= 0 || grid % 8 != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, d_counters);
+    hipLaunchKernelGGL(dedisperse_pair_batch_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, d_jobs, njobs,
+                       d_bounds, d_counters);
+    return hipGetLastError();
 }
 
 hipError_t launch_dedisperse_lds_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,

@@ -260,9 +260,10 @@ int aa_hip_ddtr_autotune_range(aa_hip_ddtr *h, int range, int t_processed, float
  * it and share one device.  execute: per beam b, d_inputs_tc[b] is its
  * [t][c] u8 chunk; t_processed, d_out and out_pitch are flattened
  * beam-major over each beam's ranges (as execute_chunk takes them per beam).
- * Every beam is corner-turned and binned, then all ranges run in one
- * persistent LDS-kernel launch over a shared tile list (ranges on the pair
- * shape, or that no chunk shape fits, run launches of their own after it).  With use_graph != 0 the
+ * Every beam is corner-turned and binned, then the ranges run in one
+ * persistent launch per LDS kernel family over a shared tile list (the pair
+ * pre-summation shape; the two byte-row shapes); ranges on the quad shape,
+ * or that no chunk shape fits, run launches of their own after them.  With use_graph != 0 the
  * sequence is captured into a hipGraph on the first call for a given set of
  * buffers and replayed by later calls with the same buffers.  Results are
  * bit-identical to aa_hip_ddtr_execute_chunk per beam.  Calls on one batch
@@ -273,8 +274,8 @@ int aa_hip_ddtr_batch_destroy(aa_hip_ddtr_batch *h);
 int aa_hip_ddtr_batch_execute(aa_hip_ddtr_batch *h, const void *const *d_inputs_tc,
                               const int *t_processed, float *const *d_out,
                               const size_t *out_pitch, int use_graph, void *stream);
-/* Jobs (ranges) in the persistent launch, tiles, and ranges launched on their
- * own (direct kernel or pair shape) in the last execute. */
+/* Jobs (ranges) in the persistent launches, their tiles, and ranges launched
+ * on their own (direct kernel or quad shape) in the last execute. */
 int aa_hip_ddtr_batch_stats(const aa_hip_ddtr_batch *h, int *njobs, long *ntiles, int *ndirect);
 
 /* Replaces corner_turn_SM_inplace (src/aa_corner_turn.cpp:118-121) for the
@@ -298,7 +299,11 @@ int aa_hip_bin2_u8(const uint8_t *d_in, size_t in_pitch, int nchans, int n_out, 
  * launch, that every delay lies in [0, row_len - t_processed]; d_dmshifts
  * (device) is what the kernel reads.  row_len is the number of valid
  * samples per row (<= pitch); rows must also hold at least 4096 readable
- * bytes past row_len (the LDS kernel reads whole windows). */
+ * bytes past row_len (the LDS kernel reads whole windows).  This call keeps
+ * no state: with AA_HIP_KERNEL_AUTO it uses a byte-row LDS shape when one
+ * fits (asynchronous on `stream`); the pre-summation kernels (PAIR, QUAD,
+ * or AUTO when only they fit) build their plan tables for the call and
+ * synchronise `stream` before returning. */
 int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nchans,
                          const float *h_dmshifts, const float *d_dmshifts, int ndms, int t_processed,
                          float mstartdm, float mdmstep, int inBin, float *d_out, size_t out_pitch,

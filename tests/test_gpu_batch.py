@@ -1,10 +1,12 @@
 """Batched multi-beam dedispersion (aa_hip_ddtr_batch_*) on the GPU.
 
 Every beam of a batch is checked bit for bit against the CPU oracle run on
-the same seeded input, and the batched launch (eager and replayed from its
-hipGraph) against the per-beam launches.  The beams mix channel counts,
-lengths, DM plans, forced LDS chunk shapes and a range no shape fits (which
-the batch hands to the direct kernel).
+the same seeded input, and the batched launches (eager and replayed from
+their hipGraph) against the per-beam launches.  The beams mix channel
+counts, lengths, DM plans, forced byte-row chunk shapes (the byte-row
+persistent kernel), ranges left to the automatic choice (the pair-shape
+persistent kernel) and a range no shape fits (which the batch hands to the
+direct kernel).
 """
 import sys
 
