@@ -91,7 +91,7 @@ void ddtr_batch::release_graph() {
 // before the persistent kernels run.
 int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float *const *d_out,
                         const size_t *out_pitch, hipStream_t s) {
-    std::vector<lds_job> jobs[2];
+    std::vector<lds_job> jobs[kFamilies];
     std::vector<range_launch> direct;
     int flat = 0;
     for (size_t b = 0; b < m_beams.size(); ++b) {
@@ -103,71 +103,51 @@ int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float
             st = e->prepare_range(r, t_processed[flat], d_out[flat], out_pitch[flat], s, &rl);
             if (st != AA_HIP_SUCCESS) return st;
             if (rl.ndms <= 0 || rl.t_processed <= 0) continue;
-            if (rl.variant == 0 || rl.variant == 1 || rl.variant == 2) {
+            // A persistent kernel holding the pair shape beside the byte-row
+            // shapes, or by itself, measured 10 % slower per tile than its
+            // launches per range: those ranges keep their own launches.
+            if (rl.variant == 0 || rl.variant == 1) {
                 lds_job j{};
                 j.p = lds_params_of(rl);
                 j.variant = rl.variant;
                 lds_job_tiles(j.p, rl.variant);
-                jobs[rl.variant == 2 ? 1 : 0].push_back(j);
+                jobs[0].push_back(j);
             } else {
                 direct.push_back(rl);
             }
         }
     }
-    for (int f = 0; f < 2; ++f) {
+    for (int f = 0; f < kFamilies; ++f) {
         // Largest tiles (most channels) first: the costliest work starts
-        // early and the queue tails are made of the cheapest tiles.
+        // early and the queue tails are made of the cheapest tiles.  Queue x
+        // holds the x-th eighth of every job (lds_job).
         std::stable_sort(jobs[f].begin(), jobs[f].end(),
                          [](const lds_job &a, const lds_job &b) { return a.p.nchans > b.p.nchans; });
-        long begin = 0;
+        long qlen[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        long total = 0;
         for (auto &j : jobs[f]) {
-            j.tile_begin = begin;
-            begin += (long)j.p.n_dm_tiles * j.p.n_t_tiles;
+            const long n = (long)j.p.n_dm_tiles * j.p.n_t_tiles;
+            for (int x = 0; x < 8; ++x) {
+                qlen[x] += (x + 1) * n / 8 - x * n / 8;
+                j.qend[x] = qlen[x];
+            }
+            total += n;
         }
-        if (begin >= 0x7fffffffL) return fail(AA_HIP_ERROR_UNSUPPORTED, "batch holds more than 2^31 tiles");
+        if (total >= 0x7fffffffL) return fail(AA_HIP_ERROR_UNSUPPORTED, "batch holds more than 2^31 tiles");
         m_fam[f].jobs.swap(jobs[f]);
-        m_fam[f].ntiles = begin;
+        m_fam[f].ntiles = total;
+        m_fam[f].bounds.assign(9, 0);
+        for (int x = 0; x < 8; ++x) m_fam[f].bounds[(size_t)x + 1] = m_fam[f].bounds[(size_t)x] + qlen[x];
     }
     m_direct.swap(direct);
     return AA_HIP_SUCCESS;
 }
 
-namespace {
-
-// Queue bounds: 8 contiguous slices of a family's tile list of about equal
-// work (tiles weighted by channel count).
-std::vector<long> queue_bounds(const std::vector<lds_job> &jobs, long ntiles) {
-    std::vector<long> bounds(9, ntiles);
-    double total = 0.0;
-    for (const auto &j : jobs) total += (double)j.p.nchans * j.p.n_dm_tiles * j.p.n_t_tiles;
-    bounds[0] = 0;
-    double acc = 0.0;
-    int x = 1;
-    for (const auto &j : jobs) {
-        const long n = (long)j.p.n_dm_tiles * j.p.n_t_tiles;
-        const double w = (double)j.p.nchans;
-        while (x < 8 && acc + w * n >= total * x / 8.0) {
-            const long cut = (long)std::ceil((total * x / 8.0 - acc) / w);
-            bounds[(size_t)x] = j.tile_begin + std::min(std::max(cut, 0L), n);
-            ++x;
-        }
-        acc += w * n;
-    }
-    for (; x < 8; ++x) bounds[(size_t)x] = ntiles;
-    for (int i = 1; i < 9; ++i) bounds[(size_t)i] = std::max(bounds[(size_t)i], bounds[(size_t)i - 1]);
-    return bounds;
-}
-
-}  // namespace
-
-// The persistent launch of each family that has jobs, then the separate
-// launches.  Reads only device-side job tables: capturable.
+// The persistent launch (when it has jobs), then the separate launches.
+// Reads only device-side job tables: capturable.
 hipError_t ddtr_batch::launch_families(hipStream_t s) {
     hipError_t e = hipSuccess;
-    if (!m_fam[1].jobs.empty())
-        e = launch_dedisperse_pair_batch(m_fam[1].d_jobs, (int)m_fam[1].jobs.size(), m_fam[1].d_bounds,
-                                         m_fam[1].d_counters, m_grid, s);
-    if (e == hipSuccess && !m_fam[0].jobs.empty())
+    if (!m_fam[0].jobs.empty())
         e = launch_dedisperse_lds_batch(m_fam[0].d_jobs, (int)m_fam[0].jobs.size(), m_fam[0].d_bounds,
                                         m_fam[0].d_counters, m_grid, s);
     for (size_t i = 0; e == hipSuccess && i < m_direct.size(); ++i) e = launch_range(m_direct[i], s);
@@ -197,8 +177,7 @@ int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float
     int st = enqueue(inputs, t_processed, d_out, out_pitch, s);
     if (st != AA_HIP_SUCCESS) return st;
     hipError_t e = hipSuccess;
-    std::vector<long> bounds[2];
-    for (int f = 0; f < 2 && e == hipSuccess; ++f) {
+    for (int f = 0; f < kFamilies && e == hipSuccess; ++f) {
         family &F = m_fam[f];
         if (F.jobs.empty()) continue;
         if (F.jobs.size() > F.jobs_cap) {
@@ -209,10 +188,9 @@ int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float
             if (e != hipSuccess) return hip_fail(e, "batch job table");
             F.jobs_cap = F.jobs.size();
         }
-        bounds[f] = queue_bounds(F.jobs, F.ntiles);
         e = hipMemcpyAsync(F.d_jobs, F.jobs.data(), F.jobs.size() * sizeof(lds_job), hipMemcpyHostToDevice, s);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(F.d_bounds, bounds[f].data(), 9 * sizeof(long), hipMemcpyHostToDevice, s);
+            e = hipMemcpyAsync(F.d_bounds, F.bounds.data(), 9 * sizeof(long), hipMemcpyHostToDevice, s);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // host vectors may change before the copies run
     if (e == hipSuccess) e = launch_families(s);
@@ -233,12 +211,15 @@ int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float
         if (e != hipSuccess) return hip_fail(e, "batch capture");
         cs = m_capture_stream;
     }
-    const size_t n0 = m_fam[0].jobs.size(), n1 = m_fam[1].jobs.size(), nd = m_direct.size();
+    size_t nj[kFamilies];
+    for (int f = 0; f < kFamilies; ++f) nj[f] = m_fam[f].jobs.size();
+    const size_t nd = m_direct.size();
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return hip_fail(e, "batch capture");
     st = enqueue(inputs, t_processed, d_out, out_pitch, cs);
-    if (st == AA_HIP_SUCCESS && (m_fam[0].jobs.size() != n0 || m_fam[1].jobs.size() != n1 || m_direct.size() != nd))
-        st = fail(AA_HIP_ERROR_DEVICE, "batch changed in capture");
+    bool same = m_direct.size() == nd;
+    for (int f = 0; f < kFamilies; ++f) same = same && m_fam[f].jobs.size() == nj[f];
+    if (st == AA_HIP_SUCCESS && !same) st = fail(AA_HIP_ERROR_DEVICE, "batch changed in capture");
     if (st == AA_HIP_SUCCESS) e = launch_families(cs);
     hipGraph_t g = nullptr;
     const hipError_t ec = hipStreamEndCapture(cs, &g);

@@ -1,16 +1,15 @@
 // Batched dedispersion of many beams (each its own plan: channel count,
-// length, DM ranges) in persistent LDS-kernel launches, optionally replayed
-// from a captured hipGraph.  No counterpart in the reference, whose
-// pipeline runs one beam per process (include/aa_pipeline_api.hpp:113-117).
+// length, DM ranges), optionally replayed from a captured hipGraph.  No
+// counterpart in the reference, whose pipeline runs one beam per process
+// (include/aa_pipeline_api.hpp:113-117).
 //
 // Per call: every beam's chunk is corner-turned into its engine's level 0 and
-// binned as its ranges need (the engines' own kernels), then the (beam,
-// range) jobs run in one persistent launch per kernel family over a shared
-// tile list: the pair pre-summation shape (dedisperse_pair_batch_kernel) and
-// the two byte-row shapes (dedisperse_lds_batch_kernel).  Ranges on the quad
-// shape or on the direct kernel (no LDS shape fits) run as launches of their
-// own afterwards.  Results are bit-identical to running each engine's
-// execute_chunk.
+// binned as its ranges need (the engines' own kernels).  The (beam, range)
+// jobs on the two byte-row chunk shapes run in one persistent launch over a
+// shared tile list (dedisperse_lds_batch_kernel); ranges on the pair or quad
+// pre-summation shapes, or on the direct kernel (no LDS shape fits), run as
+// launches of their own after it (captured in the same graph).  Results are
+// bit-identical to running each engine's execute_chunk.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,8 +36,16 @@ public:
 
     // jobs (ranges) and tiles in the persistent launches, ranges launched
     // on their own
-    int njobs() const { return (int)(m_fam[0].jobs.size() + m_fam[1].jobs.size()); }
-    long ntiles() const { return m_fam[0].ntiles + m_fam[1].ntiles; }
+    int njobs() const {
+        size_t n = 0;
+        for (const auto &f : m_fam) n += f.jobs.size();
+        return (int)n;
+    }
+    long ntiles() const {
+        long n = 0;
+        for (const auto &f : m_fam) n += f.ntiles;
+        return n;
+    }
     int ndirect() const { return (int)m_direct.size(); }
     int nranges_total() const { return m_nranges_total; }
 
@@ -53,16 +60,18 @@ private:
     int m_grid = 0;
     std::vector<ddtr_engine *> m_beams;
     int m_nranges_total = 0;
-    // One persistent launch per kernel family: 0 = byte-row shapes, 1 = pair shape.
+    // The persistent launch's jobs (byte-row shapes).
     struct family {
         std::vector<lds_job> jobs;
+        std::vector<long> bounds;  // 9 queue bounds (host copy)
         long ntiles = 0;
         lds_job *d_jobs = nullptr;
         size_t jobs_cap = 0;
         long *d_bounds = nullptr;  // 9 queue bounds
         int *d_counters = nullptr; // 8 queue heads
     };
-    family m_fam[2];
+    static constexpr int kFamilies = 1;
+    family m_fam[kFamilies];
     std::vector<range_launch> m_direct;
     hipStream_t m_capture_stream = nullptr;  // for callers on the legacy default stream
     hipGraph_t m_graph = nullptr;

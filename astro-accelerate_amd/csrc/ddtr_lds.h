@@ -33,13 +33,16 @@ struct lds_params {
     int n_dm_tiles, n_t_tiles;  // filled by the launcher / lds_job_tiles
 };
 
-// One range of a batched launch: its tiles are items tile_begin ..
-// tile_begin + n_dm_tiles * n_t_tiles - 1 of the batch's tile list.
+// One range of a batched launch.  The launch has 8 queues (one per XCD);
+// queue x holds, job after job, the x-th eighth of every job's tiles (tiles
+// [x n / 8, (x+1) n / 8) of a job of n = n_dm_tiles * n_t_tiles tiles), so
+// the XCDs work through the jobs side by side.  qend[x]: end of this job's
+// part in queue x (queue positions from 0).
 struct lds_job {
     lds_params p;
-    int variant;             // chunk shape: 0, 1 (dedisperse_lds_batch) or 2 (dedisperse_pair_batch)
+    int variant;             // chunk shape 0 or 1 (the shapes dedisperse_lds_batch holds)
     int pad;
-    long tile_begin;
+    long qend[8];
 };
 
 // Staged row width (uint4) this geometry needs with chunk shape `variant`,
@@ -70,15 +73,13 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
 // Fills p.n_dm_tiles and p.n_t_tiles for chunk shape `variant`.
 void lds_job_tiles(lds_params& p, int variant);
 
-// Persistent launch over jobs[0..njobs) (device array, sorted by tile_begin,
-// the first at 0); d_bounds[0..8] splits the tile list into 8 queues (one per
-// XCD), d_counters[0..7] is scratch (reset by the launch itself, with a
-// kernel, so the sequence may be captured in a graph).  grid: multiple of 8.
+// Persistent launch over jobs[0..njobs) (device array, in queue order);
+// queue x is items d_bounds[x] .. d_bounds[x+1]-1 of the launch (d_bounds[x]
+// = the lengths of queues 0..x-1 added), d_counters[0..7] is scratch (reset
+// by the launch itself, with a kernel, so the sequence may be captured in a
+// graph).  grid: multiple of 8.
 hipError_t launch_dedisperse_lds_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
                                        int grid, hipStream_t s);
-// The same for jobs of chunk shape 2 (p.pair_recs set, tiles of the pair
-// shape); the tile count must stay below 2^31.
-hipError_t launch_dedisperse_pair_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
-                                        int grid, hipStream_t s);
+
 
 }  // namespace aa_hip

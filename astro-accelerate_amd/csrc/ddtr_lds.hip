@@ -650,53 +650,29 @@ __global__ __launch_bounds__(kThreads) void dedisperse_pair_kernel(lds_params p)
     process_pair_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kPT);
 }
 
-// Pair-shape jobs of many ranges (of many beams) in one persistent launch,
-// scheduled as dedisperse_lds_batch_kernel schedules the byte-row shapes
-// (per-XCD queues of one shared tile list, stealing when a queue is empty).
-// The tile index is handed to the work-group through the last dword of the
-// row buffers, which no tile reads between tiles (every tile ends with a
-// barrier after its last LDS read, and writes that row last).
-__global__ __launch_bounds__(kThreads) void dedisperse_pair_batch_kernel(const lds_job* __restrict__ jobs, int njobs,
-                                                                         const long* __restrict__ bounds,
-                                                                         int* __restrict__ counters)
+// The XCD this work-group runs on (hardware register XCC_ID, bits 0-3), so
+// that it starts from the queue whose tiles share its XCD's L2.
+__device__ __forceinline__ int batch_xcd()
 {
-    int* slot = reinterpret_cast<int*>(g_prow + 2 * kPRows * kPRQ) - 1;
-    const int xcd = (int)(blockIdx.x % 8);
-    int q = 0;  // queues found empty so far (thread 0)
-    for (;;) {
-        if (threadIdx.x == 0) {
-            long item = -1;
-            while (q < 8) {
-                const int x = (xcd + q) & 7;
-                const long i = bounds[x] + (long)atomicAdd(&counters[x], 1);
-                if (i < bounds[x + 1]) {
-                    item = i;
-                    break;
-                }
-                ++q;
-            }
-            *slot = (int)item;   // the host keeps the tile count below 2^31
-        }
-        __syncthreads();
-        const long item = *slot;
-        __syncthreads();
-        if (item < 0) break;
-        int lo = 0, hi = njobs - 1;  // last job with tile_begin <= item
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].tile_begin <= item) lo = mid;
-            else hi = mid - 1;
-        }
-        const lds_job& J = jobs[lo];
-        const long local = item - J.tile_begin;
-        const int dt = (int)(local % J.p.n_dm_tiles);
-        const long t0 = (local / J.p.n_dm_tiles) * kPT;
-        const int ni = (J.p.row_quads + 63) / 64;
-        if (ni <= 2) process_pair_tile<2>(J.p, dt, t0);
-        else if (ni == 3) process_pair_tile<3>(J.p, dt, t0);
-        else if (ni == 4) process_pair_tile<4>(J.p, dt, t0);
-        else process_pair_tile<5>(J.p, dt, t0);
+    return (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u);
+}
+
+// Job and tile of position `pos` of queue x (see lds_job): the first job
+// whose part in queue x ends after pos.
+__device__ __forceinline__ const lds_job& batch_job_of(const lds_job* __restrict__ jobs, int njobs, int x, long pos,
+                                                       long* local)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (jobs[mid].qend[x] > pos) hi = mid;
+        else lo = mid + 1;
     }
+    const lds_job& J = jobs[lo];
+    const long n = (long)J.p.n_dm_tiles * J.p.n_t_tiles;
+    const long b = x * n / 8, e = (x + 1) * n / 8;
+    *local = b + pos - (J.qend[x] - (e - b));
+    return J;
 }
 
 // ---------------------------------------------------------------------------
@@ -929,44 +905,44 @@ __global__ __launch_bounds__(kThreads) void dedisperse_lds_kernel(lds_params p)
     process_tile<S>(p, (w % p.n_dm_tiles) * kDms, (long)(w / p.n_dm_tiles) * kT);
 }
 
-// Many ranges (of many beams) in one persistent launch.  The tiles of all
-// jobs form one list, split into 8 contiguous queues of about equal work, one
-// per XCD; a work-group takes tiles from its own XCD's queue (neighbouring
-// tiles share input rows in that XCD's L2) and, once it is empty, from the
-// others in turn.  Every work-group leaves when all 8 queues are empty.
+// Many ranges (of many beams) in one persistent launch.  Each of the 8
+// queues (one per XCD) holds the x-th eighth of every job, job after job
+// (lds_job), so the XCDs work through the jobs side by side, as a launch per
+// range would; within an XCD, neighbouring tiles share input rows in its L2.
+// A work-group takes tiles from its own XCD's queue and, once it is empty,
+// from the others in turn.  Every work-group leaves when all 8 queues are
+// empty.
 __global__ __launch_bounds__(kThreads) void dedisperse_lds_batch_kernel(const lds_job* __restrict__ jobs, int njobs,
                                                                         const long* __restrict__ bounds,
                                                                         int* __restrict__ counters)
 {
-    __shared__ long s_item;
-    const int xcd = (int)(blockIdx.x % 8);
+    __shared__ int s_q[2];   // queue, position
+    const int xcd = batch_xcd();
     int q = 0;  // queues found empty so far (thread 0)
     for (;;) {
         if (threadIdx.x == 0) {
-            long item = -1;
+            int qx = -1, pos = 0;
             while (q < 8) {
                 const int x = (xcd + q) & 7;
                 const long i = bounds[x] + (long)atomicAdd(&counters[x], 1);
                 if (i < bounds[x + 1]) {
-                    item = i;
+                    qx = x;
+                    pos = (int)(i - bounds[x]);   // the host keeps the tile count below 2^31
                     break;
                 }
                 ++q;
             }
-            s_item = item;
+            s_q[0] = qx;
+            s_q[1] = pos;
         }
         __syncthreads();
-        const long item = s_item;
+        // wave-uniform (scalar) from here on: the tile's indices and addresses
+        const int qx = __builtin_amdgcn_readfirstlane(s_q[0]);
+        const long pos = __builtin_amdgcn_readfirstlane(s_q[1]);
         __syncthreads();
-        if (item < 0) break;
-        int lo = 0, hi = njobs - 1;  // last job with tile_begin <= item
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (jobs[mid].tile_begin <= item) lo = mid;
-            else hi = mid - 1;
-        }
-        const lds_job& J = jobs[lo];
-        const long local = item - J.tile_begin;
+        if (qx < 0) break;
+        long local;
+        const lds_job& J = batch_job_of(jobs, njobs, qx, pos, &local);
         const int d0 = (int)(local % J.p.n_dm_tiles) * kDms;
         const long t0 = (local / J.p.n_dm_tiles) * kT;
         if (J.variant == 0) process_tile<shape_c32>(J.p, d0, t0);
@@ -1332,17 +1308,6 @@ void lds_job_tiles(lds_params& p, int variant)
     const long tile = variant == 2 || variant == 3 ? kPT : kT;   // 16 DMs per tile in every shape
     p.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
     p.n_t_tiles = (int)((p.t_processed + tile - 1) / tile);
-}
-
-hipError_t launch_dedisperse_pair_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
-                                        int grid, hipStream_t s)
-{
-    if (njobs <= 0) return hipSuccess;
-    if (grid <= 0 || grid % 8 != 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, d_counters);
-    hipLaunchKernelGGL(dedisperse_pair_batch_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, d_jobs, njobs,
-                       d_bounds, d_counters);
-    return hipGetLastError();
 }
 
 hipError_t launch_dedisperse_lds_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,

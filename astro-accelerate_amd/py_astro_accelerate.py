@@ -587,8 +587,9 @@ class DdtrPlan:
 
 
 class DdtrBatch:
-    """Many beams (DdtrPlan handles) dedispersed in one persistent launch per LDS kernel
-    family, optionally replayed from a hipGraph (aa_hip_ddtr_batch_*)."""
+    """Many beams (DdtrPlan handles) dedispersed together: byte-row ranges in one persistent
+    launch, the others in launches of their own, optionally replayed from a hipGraph
+    (aa_hip_ddtr_batch_*)."""
 
     def __init__(self, plans: Sequence["DdtrPlan"]):
         lib = _lib_or_raise()

@@ -303,9 +303,10 @@ def bench_frac(args):
 def bench_multibeam(args):
     """48 beams (1024/2048/4096 channels, 2^16..2^19 samples, DM max 500/1000/
     3000 in 1-4 AA-style ranges, seeded u8 noise + 2 pulses each): all beams
-    in one persistent launch replayed from a hipGraph (DdtrBatch), against
-    the same beams launched one after another (execute_chunk per beam).
-    Chunk shapes are autotuned per shape class first."""
+    batched and replayed from a hipGraph (DdtrBatch), against the same beams
+    launched one after another (execute_chunk per beam) on the same tuned
+    shapes, and on the byte-row shapes.  Chunk shapes are autotuned per
+    shape class first."""
     import torch
     import py_astro_accelerate as aa
     import multibeam as mb
@@ -363,6 +364,25 @@ def bench_multibeam(args):
         log(f"[bench] per-beam launches: {ms_per_beam:.2f} ms/step")
         ms_batched = timed(batched, args.steps)
         log(f"[bench] batched (graph, persistent): {ms_batched:.2f} ms/step -> speedup {ms_per_beam / ms_batched:.3f}x")
+        # The same per-beam launches on the byte-row shapes (each range's
+        # faster of the 32- and 16-channel shapes where one fits; ranges no
+        # byte-row shape fits keep the automatic choice): the path a port of
+        # the reference's one-beam-at-a-time kernel would run.
+        for b in beams:
+            for r in range(len(b.tps)):
+                ms = cache[repr(mb.shape_class(b, r))]["ms"]
+                fit = [v for v in (0, 1) if ms[v] >= 0]
+                b.plan.set_range_variant(r, min(fit, key=lambda v: ms[v]) if fit else -1)
+        for _ in range(max(1, args.warmup)):
+            per_beam()
+        stream.synchronize()
+        exact_bytes = mb.output_digest(beams) == want
+        ms_per_beam_bytes = timed(per_beam, args.steps)
+        log(f"[bench] per-beam launches, byte-row shapes: {ms_per_beam_bytes:.2f} ms/step (bit-identical: "
+            f"{exact_bytes}) -> batched speedup {ms_per_beam_bytes / ms_batched:.3f}x")
+        for b in beams:   # back to the tuned shapes
+            for r in range(len(b.tps)):
+                b.plan.set_range_variant(r, int(cache[repr(mb.shape_class(b, r))]["variant"]))
     achieved = adds / (ms_batched * 1e-3) / 1e12
     result = {
         "metric": "DM×sample/s bit-exact, % FP32 add roofline; batched vs per-beam speedup",
@@ -377,9 +397,12 @@ def bench_multibeam(args):
         "per_beam_ms_per_step": ms_per_beam,
         "speedup_vs_per_beam": ms_per_beam / ms_batched,
         "bit_exact_vs_per_beam": exact,
+        "per_beam_byte_rows_ms_per_step": ms_per_beam_bytes,
+        "speedup_vs_per_beam_byte_rows": ms_per_beam_bytes / ms_batched,
+        "bit_exact_vs_per_beam_byte_rows": exact_bytes,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": ADD_ROOFLINE_T, "unit": "Tadd/s",
                      "frac": achieved / ADD_ROOFLINE_T, "traffic": None,
-                     "kernel": "batched step (corner turns + binning + one persistent dedispersion launch)"},
+                     "kernel": "batched step (corner turns + binning + dedispersion launches, one graph)"},
     }
     print(json.dumps(result), flush=True)
 
