@@ -43,8 +43,8 @@ struct lds_choice {
     int row_quads;
 };
 // First shape (or only `variant` when >= 0) whose rows hold the range's
-// windows, tried in the order 2 (pair pre-summation, the fastest on the
-// 8-range 4096-channel plan), 3, 0, 1.
+// windows, tried in the order 4, 2 (pair pre-summation with 2048- and
+// 1024-sample tiles, the fastest on the 8-range 4096-channel plan), 3, 0, 1.
 lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant);
 
 // One range, resolved to a kernel: variant >= 0 = LDS kernel with that chunk
@@ -60,7 +60,7 @@ struct range_launch {
     size_t out_pitch = 0;
     int variant = -1;
     int row_quads = 0;
-    const void *pair_recs = nullptr;  // chunk shapes 2, 3: device plan tables
+    const void *pair_recs = nullptr;  // chunk shapes 2-4: device plan tables
 };
 
 class ddtr_engine {
@@ -97,8 +97,7 @@ public:
     // the caller (launch_range, or a batched launch).
     int prepare_range(int r, int t_processed, float *d_out, size_t out_pitch, hipStream_t s, range_launch *rl);
     // LDS chunk shape preferred for range r (-1 = first that fits, in the
-    // order pair shape, quad shape, 32-channel, 16-channel); used when it
-    // fits the range's delay spread.
+    // order of choose_lds); used when it fits the range's delay spread.
     int set_range_variant(int r, int variant);
     // Times every LDS chunk shape that fits range r (reps launches each, after
     // one untimed launch; output written to d_out) and keeps the fastest as
@@ -143,7 +142,7 @@ private:
         void *ptr = nullptr;
         size_t bytes = 0;
     };
-    std::vector<plan_tables> m_tables;  // per range: chunk shape 2/3 plan tables (device), built on first use
+    std::vector<plan_tables> m_tables;  // per range: chunk shape 2-4 plan tables (device), built on first use
     size_t m_device_bytes = 0;
 };
 

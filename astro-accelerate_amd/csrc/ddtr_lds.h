@@ -13,9 +13,10 @@ namespace aa_hip {
 // 16-channel shape holds rows twice as long in the same buffers.
 constexpr int kLdsRowQuadsMax = 152;
 // Chunk shapes: 0 = 32 channels x 152 quads, 1 = 16 channels x 304 quads,
-// 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums), 3 = 16
-// channels pre-summed in groups of four.
-constexpr int kLdsVariants = 4;
+// 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums, 1024-sample
+// tiles), 3 = 16 channels pre-summed in groups of four, 4 = shape 2 with
+// 2048-sample tiles (window offsets up to 256 samples).
+constexpr int kLdsVariants = 5;
 
 struct lds_params {
     const uint8_t* L;        // nchans rows of u8 samples
@@ -29,7 +30,7 @@ struct lds_params {
     float* out;
     size_t out_pitch;        // floats between output rows
     int row_quads;           // staged row width in uint4 (from lds_row_quads)
-    const void* pair_recs;   // chunk shapes 2, 3: device plan tables (from lds_plan_tables)
+    const void* pair_recs;   // chunk shapes 2-4: device plan tables (from lds_plan_tables)
     int n_dm_tiles, n_t_tiles;  // filled by the launcher / lds_job_tiles
 };
 
@@ -57,15 +58,16 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
 int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
                     std::vector<uint8_t>* bytes);
 
-// Plan tables of chunk shape 2 or 3 (as lds_pair_tables; -1 for other
+// Plan tables of chunk shape 2, 3 or 4 (as lds_pair_tables; -1 for other
 // shapes, which need none).
 int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep,
                     size_t pitch, std::vector<uint8_t>* bytes);
 
-// Tile geometry: samples per tile and DM trials per tile.  Rows must hold
-// lds_tile_samples() readable bytes past the last processed sample plus the
-// maximum delay (the engine pads by 4 KiB).
-long lds_tile_samples();
+// Tile geometry: samples per tile of chunk shape `variant` and DM trials
+// per tile (16 in every shape).  Rows must hold the tile's samples readable
+// past the last processed sample plus the maximum delay (the engine pads by
+// 4 KiB).
+long lds_tile_samples(int variant);
 int lds_tile_dms();
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s);

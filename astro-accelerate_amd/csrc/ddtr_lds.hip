@@ -385,9 +385,13 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
 // registers: the loads of chunk k+1 are issued before chunk k is summed and
 // stored after it; one barrier per chunk.
 // ---------------------------------------------------------------------------
-constexpr int kPT = 1024;                      // samples per tile
-constexpr int kPR = kPT / 64;                  // 16 samples per lane
-constexpr int kPSplit = kPT / 2;               // samples between the halves of a dword
+// Tile widths: T samples (1024 for shape 2, 2048 for shape 4), R = T / 64
+// samples per lane, halves of a dword T / 2 samples apart.
+template <int T>
+struct pair_geom {
+    static constexpr int R = T / 64;
+    static constexpr int SPLIT = T / 2;
+};
 constexpr int kPairs = 8;                      // pairs per chunk
 constexpr int kPairChunk = 2 * kPairs;         // 16 channels
 constexpr int kPIt = 5;                        // quads per lane per row
@@ -396,9 +400,9 @@ constexpr int kPRowDw = 4 * kPRQ;              // 1280 dwords per row
 constexpr int kPRows = 16;                     // rows per buffer (one per wave)
 constexpr int kPMaxEnt = 2 * kPairs;           // row entries per DM per chunk
 constexpr int kPFlushChunks = 256 / kPairChunk;   // 128 pair sums (<= 510 each) per 16-bit field
-constexpr int kPairMaxOffset = kPRowDw - kPSplit; // e + 511 < kPRowDw
+// window offsets e with e + T/2 - 1 < kPRowDw
+__host__ __device__ constexpr int pair_max_offset(int T) { return kPRowDw - T / 2; }
 static_assert(kPRows == kDms, "one row per wave is built");
-static_assert(kPSplit == 64 * (kPR / 2), "a lane's eight dwords span half a tile");
 
 // Plan record of one (DM tile, chunk).  A pair whose relative delay takes
 // more than two values in the tile is split into two single-channel rows
@@ -435,7 +439,7 @@ struct pair_build {
 // zeroed record's window at the level start.  A lane over-reads at
 // most 63 quads (252 bytes) past the window's needed width, inside the
 // 4096-byte slack every row has past its longest window.
-template <int NI>
+template <int NI, int T>
 __device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, const pair_row& R, int lane,
                                                 pair_build<NI>& B)
 {
@@ -449,11 +453,11 @@ __device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, co
         const uint32_t* pa = reinterpret_cast<const uint32_t*>(ra + 4 * q);
         const uint32_t* pb = reinterpret_cast<const uint32_t*>(rb + 4 * q);
         B.al[i] = pa[0];
-        B.ah[i] = pa[kPSplit / 4];
+        B.ah[i] = pa[pair_geom<T>::SPLIT / 4];
         B.bl[i][0] = pb[0];
         B.bl[i][1] = pb[1];
-        B.bh[i][0] = pb[kPSplit / 4];
-        B.bh[i][1] = pb[kPSplit / 4 + 1];
+        B.bh[i][0] = pb[pair_geom<T>::SPLIT / 4];
+        B.bh[i][1] = pb[pair_geom<T>::SPLIT / 4 + 1];
     }
 }
 
@@ -480,47 +484,52 @@ __device__ __forceinline__ void pair_build_store(int wave, int lane, const pair_
 
 // acc16[j] += row dword k = e + L + 64 j of the entries at dword offsets o0
 // (and o1): two rows per v_add3.
-__device__ __forceinline__ void pair_add2(const uint32_t* base, int o0, int o1, uint32_t (&acc)[kPR / 2])
+template <int R>
+__device__ __forceinline__ void pair_add2(const uint32_t* base, int o0, int o1, uint32_t (&acc)[R / 2])
 {
     const uint32_t* r0 = base + o0;
     const uint32_t* r1 = base + o1;
-    uint32_t x0[kPR / 2], x1[kPR / 2];
+    uint32_t x0[R / 2], x1[R / 2];
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j) {
+    for (int j = 0; j < R / 2; ++j) {
         x0[j] = r0[64 * j];
         x1[j] = r1[64 * j];
     }
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j) acc[j] += x0[j] + x1[j];
+    for (int j = 0; j < R / 2; ++j) acc[j] += x0[j] + x1[j];
 }
 
-__device__ __forceinline__ void pair_add1(const uint32_t* base, int o0, uint32_t (&acc)[kPR / 2])
+template <int R>
+__device__ __forceinline__ void pair_add1(const uint32_t* base, int o0, uint32_t (&acc)[R / 2])
 {
     const uint32_t* r0 = base + o0;
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j) acc[j] += r0[64 * j];
+    for (int j = 0; j < R / 2; ++j) acc[j] += r0[64 * j];
 }
 
 // Four rows' dwords of a lane (entries e01, e23), read ahead of their adds.
+template <int R>
 struct pair_x4 {
-    uint32_t x[4][kPR / 2];
+    uint32_t x[4][R / 2];
 };
 
-__device__ __forceinline__ void pair_read4(const uint32_t* base, uint32_t e01, uint32_t e23, pair_x4& X)
+template <int R>
+__device__ __forceinline__ void pair_read4(const uint32_t* base, uint32_t e01, uint32_t e23, pair_x4<R>& X)
 {
     const uint32_t* r[4] = {base + (e01 & 0xffffu), base + (e01 >> 16), base + (e23 & 0xffffu), base + (e23 >> 16)};
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j)
+    for (int j = 0; j < R / 2; ++j)
 #pragma unroll
         for (int m = 0; m < 4; ++m) X.x[m][j] = r[m][64 * j];
 }
 
-__device__ __forceinline__ void pair_acc4(const pair_x4& X, uint32_t (&acc)[kPR / 2])
+template <int R>
+__device__ __forceinline__ void pair_acc4(const pair_x4<R>& X, uint32_t (&acc)[R / 2])
 {
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j) acc[j] += X.x[0][j] + X.x[1][j];
+    for (int j = 0; j < R / 2; ++j) acc[j] += X.x[0][j] + X.x[1][j];
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j) acc[j] += X.x[2][j] + X.x[3][j];
+    for (int j = 0; j < R / 2; ++j) acc[j] += X.x[2][j] + X.x[3][j];
 }
 
 // Entries of one DM in one chunk (16-bit dword offsets, two per dword),
@@ -543,25 +552,27 @@ __device__ __forceinline__ pair_ents pair_ents_of(const pair_rec& rc, int wave)
 // All n entries of DM slot `wave` in chunk record rc, read from the record
 // itself (a register array indexed at run time would live in scratch
 // memory), two rows per v_add3.
+template <int R>
 __device__ __forceinline__ void pair_sum(const pair_rec& rc, int wave, int n, int lane, const uint4* rows,
-                                         uint32_t (&acc)[kPR / 2])
+                                         uint32_t (&acc)[R / 2])
 {
     const uint32_t* base = reinterpret_cast<const uint32_t*>(rows) + lane;
     const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
     int i = 0;
     for (; i + 1 < n; i += 2) {
         const uint32_t e = ew[i >> 1];
-        pair_add2(base, (int)(e & 0xffffu), (int)(e >> 16), acc);
+        pair_add2<R>(base, (int)(e & 0xffffu), (int)(e >> 16), acc);
     }
-    if (i < n) pair_add1(base, (int)(ew[i >> 1] & 0xffffu), acc);
+    if (i < n) pair_add1<R>(base, (int)(ew[i >> 1] & 0xffffu), acc);
 }
 
-__device__ __forceinline__ void pair_flush(uint32_t (&acc16)[kPR / 2], uint32_t (&acc32)[kPR])
+template <int R>
+__device__ __forceinline__ void pair_flush(uint32_t (&acc16)[R / 2], uint32_t (&acc32)[R])
 {
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j) {
+    for (int j = 0; j < R / 2; ++j) {
         acc32[j] += acc16[j] & 0xFFFFu;
-        acc32[kPR / 2 + j] += acc16[j] >> 16;
+        acc32[R / 2 + j] += acc16[j] >> 16;
         acc16[j] = 0;
     }
 }
@@ -570,31 +581,32 @@ __device__ __forceinline__ void pair_flush(uint32_t (&acc16)[kPR / 2], uint32_t 
 // the registers loaded during step k-1, issues the loads of chunk k+2, sums
 // chunk k, and ends with the chunk's one barrier; a row's loads thus have a
 // whole step to arrive.  NI: quads per lane of a row (the range's row width
-// rounded up to 64 quads).
-template <int NI>
+// rounded up to 64 quads); T: samples per tile.
+template <int NI, int T>
 __device__ __forceinline__ void process_pair_tile(const lds_params& p, int dt, long t0)
 {
+    constexpr int R = pair_geom<T>::R;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool active = dt * kDms + wave < p.ndms;
     const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
     const pair_rec* recs = reinterpret_cast<const pair_rec*>(p.pair_recs) + (size_t)dt * nchunks;
-    uint32_t acc16[kPR / 2];
-    uint32_t acc32[kPR];
+    uint32_t acc16[R / 2];
+    uint32_t acc32[R];
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j) acc16[j] = 0;
+    for (int j = 0; j < R / 2; ++j) acc16[j] = 0;
 #pragma unroll
-    for (int j = 0; j < kPR; ++j) acc32[j] = 0;
+    for (int j = 0; j < R; ++j) acc32[j] = 0;
 
     // Scalar prefetches: the row record of the chunk whose loads the next
     // step issues, and the entries of the chunk the next step sums; read
     // just before the barrier, whose wait covers their latency.
     pair_build<NI> B;
-    pair_build_load(p, t0, recs[0].row[wave], lane, B);
+    pair_build_load<NI, T>(p, t0, recs[0].row[wave], lane, B);
     if (wave < recs[0].nrows) pair_build_store(wave, lane, B, pair_buf(0));
     const int k1 = min(1, nchunks - 1);
     bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
-    pair_build_load(p, t0, recs[k1].row[wave], lane, B);
+    pair_build_load<NI, T>(p, t0, recs[k1].row[wave], lane, B);
     int kr = min(2, nchunks - 1);
     pair_row rn = recs[kr].row[wave];                     // row record of chunk k+2
     int nrows_n = recs[kr].nrows;
@@ -606,19 +618,25 @@ __device__ __forceinline__ void process_pair_tile(const lds_params& p, int dt, l
         uint4* rows_next = pair_buf((k + 1) & 1);
         if (have_next) pair_build_store(wave, lane, B, rows_next);
         have_next = k + 2 < nchunks && wave < nrows_n;
-        pair_build_load(p, t0, rn, lane, B);
+        pair_build_load<NI, T>(p, t0, rn, lane, B);
         if (active) {
             if (E.n == kPairs) {   // the usual full chunk: straight-line code
-                pair_x4 X;
-                pair_read4(base, E.e[0], E.e[1], X);
-                pair_acc4(X, acc16);
-                pair_read4(base, E.e[2], E.e[3], X);
-                pair_acc4(X, acc16);
+                if constexpr (R <= 16) {   // four rows per batch of reads
+                    pair_x4<R> X;
+                    pair_read4<R>(base, E.e[0], E.e[1], X);
+                    pair_acc4<R>(X, acc16);
+                    pair_read4<R>(base, E.e[2], E.e[3], X);
+                    pair_acc4<R>(X, acc16);
+                } else {                   // two (the registers of wider tiles)
+#pragma unroll
+                    for (int i = 0; i < kPairs / 2; ++i)
+                        pair_add2<R>(base, (int)(E.e[i] & 0xffffu), (int)(E.e[i] >> 16), acc16);
+                }
             } else {
-                pair_sum(recs[k], wave, E.n, lane, pair_buf(k & 1), acc16);
+                pair_sum<R>(recs[k], wave, E.n, lane, pair_buf(k & 1), acc16);
             }
         }
-        if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) pair_flush(acc16, acc32);
+        if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) pair_flush<R>(acc16, acc32);
         kr = min(k + 3, nchunks - 1);
         rn = recs[kr].row[wave];
         nrows_n = recs[kr].nrows;
@@ -632,22 +650,22 @@ __device__ __forceinline__ void process_pair_tile(const lds_params& p, int dt, l
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int j = 0; j < kPR / 2; ++j) {
-                const long tt = t0 + h * kPSplit + lane + 64 * j;
+            for (int j = 0; j < R / 2; ++j) {
+                const long tt = t0 + h * pair_geom<T>::SPLIT + lane + 64 * j;
                 if (tt < p.t_processed)
-                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * (kPR / 2) + j], p.fnchans), p.fbin);
+                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * (R / 2) + j], p.fnchans), p.fbin);
             }
     }
 }
 
-template <int NI>
+template <int NI, int T>
 __global__ __launch_bounds__(kThreads) void dedisperse_pair_kernel(lds_params p)
 {
     const int ntiles = p.n_dm_tiles * p.n_t_tiles;
     const int per_xcd = (ntiles + 7) / 8;
     const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (w >= ntiles) return;  // whole work-group leaves before any barrier
-    process_pair_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kPT);
+    process_pair_tile<NI, T>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * T);
 }
 
 // The XCD this work-group runs on (hardware register XCC_ID, bits 0-3), so
@@ -694,7 +712,10 @@ constexpr int kQIt = 3;                        // quads per lane per row, at mos
 constexpr int kQRQ = 64 * kQIt;                // 192 quads per row
 constexpr int kQRowDw = 4 * kQRQ;              // 768 dwords per row
 constexpr int kQRows = 16;                     // rows per buffer (one per wave)
-constexpr int kQMaxOffset = kQRowDw - kPSplit; // e + 511 < kQRowDw
+constexpr int kQT = 1024;                      // samples per tile
+constexpr int kQR = pair_geom<kQT>::R;         // 16 samples per lane
+constexpr int kQSplit = pair_geom<kQT>::SPLIT;
+constexpr int kQMaxOffset = kQRowDw - kQSplit; // e + 511 < kQRowDw
 constexpr int kQGroups = kPairChunk / 4;       // 4-channel groups per chunk
 constexpr int kQMaxGroupRows = 4;
 static_assert(kQGroups * kQMaxGroupRows == kQRows && kQRows == kDms, "one row per wave");
@@ -745,14 +766,14 @@ __device__ __forceinline__ void quad_issue_loads(const lds_params& p, long t0, c
         const int q = lane + 64 * i;
         const uint32_t* pa = reinterpret_cast<const uint32_t*>(r[0] + 4 * q);
         B.a[i][0] = pa[0];
-        B.a[i][1] = pa[kPSplit / 4];
+        B.a[i][1] = pa[kQSplit / 4];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             const uint32_t* pm = reinterpret_cast<const uint32_t*>(r[m + 1] + 4 * q);
             B.x[i][m][0] = pm[0];
             B.x[i][m][1] = pm[1];
-            B.x[i][m][2] = pm[kPSplit / 4];
-            B.x[i][m][3] = pm[kPSplit / 4 + 1];
+            B.x[i][m][2] = pm[kQSplit / 4];
+            B.x[i][m][3] = pm[kQSplit / 4 + 1];
         }
     }
 }
@@ -797,12 +818,12 @@ __device__ __forceinline__ void quad_store(int wave, int lane, const quad_load<N
 
 // All n entries of DM slot `wave` in chunk record rc, read from the record.
 __device__ __forceinline__ void quad_sum_any(const quad_rec& rc, int wave, int n, const uint32_t* base,
-                                             uint32_t (&acc)[kPR / 2])
+                                             uint32_t (&acc)[kQR / 2])
 {
     const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
     for (int i = 0; i < n; ++i) {
         const uint32_t e = ew[i >> 1];
-        pair_add1(base, (int)((i & 1) ? (e >> 16) : (e & 0xffffu)), acc);
+        pair_add1<kQR>(base, (int)((i & 1) ? (e >> 16) : (e & 0xffffu)), acc);
     }
 }
 
@@ -826,12 +847,12 @@ __device__ __forceinline__ void process_quad_tile(const lds_params& p, int dt, l
     const bool active = dt * kDms + wave < p.ndms;
     const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
     const quad_rec* recs = reinterpret_cast<const quad_rec*>(p.pair_recs) + (size_t)dt * nchunks;
-    uint32_t acc16[kPR / 2];
-    uint32_t acc32[kPR];
+    uint32_t acc16[kQR / 2];
+    uint32_t acc32[kQR];
 #pragma unroll
-    for (int j = 0; j < kPR / 2; ++j) acc16[j] = 0;
+    for (int j = 0; j < kQR / 2; ++j) acc16[j] = 0;
 #pragma unroll
-    for (int j = 0; j < kPR; ++j) acc32[j] = 0;
+    for (int j = 0; j < kQR; ++j) acc32[j] = 0;
 
     quad_load<NI> B;
     quad_issue_loads(p, t0, recs[0].row[wave], lane, B);
@@ -852,13 +873,13 @@ __device__ __forceinline__ void process_quad_tile(const lds_params& p, int dt, l
         quad_issue_loads(p, t0, rn, lane, B);
         if (active) {
             if (E.n == kQGroups) {   // the usual full chunk: four rows, one batch of reads
-                pair_x4 X;
-                pair_read4(base, E.e[0], E.e[1], X);
-                pair_acc4(X, acc16);
+                pair_x4<kQR> X;
+                pair_read4<kQR>(base, E.e[0], E.e[1], X);
+                pair_acc4<kQR>(X, acc16);
             } else {
                 quad_sum_any(recs[k], wave, E.n, base, acc16);
             }
-            if ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks) pair_flush(acc16, acc32);
+            if ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks) pair_flush<kQR>(acc16, acc32);
         }
         kr = min(k + 3, nchunks - 1);
         rn = recs[kr].row[wave];
@@ -873,10 +894,10 @@ __device__ __forceinline__ void process_quad_tile(const lds_params& p, int dt, l
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int j = 0; j < kPR / 2; ++j) {
-                const long tt = t0 + h * kPSplit + lane + 64 * j;
+            for (int j = 0; j < kQR / 2; ++j) {
+                const long tt = t0 + h * kQSplit + lane + 64 * j;
                 if (tt < p.t_processed)
-                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * (kPR / 2) + j], p.fnchans), p.fbin);
+                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * (kQR / 2) + j], p.fnchans), p.fbin);
             }
     }
 }
@@ -888,7 +909,7 @@ __global__ __launch_bounds__(kThreads) void dedisperse_quad_kernel(lds_params p)
     const int per_xcd = (ntiles + 7) / 8;
     const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (w >= ntiles) return;  // whole work-group leaves before any barrier
-    process_quad_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kPT);
+    process_quad_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kQT);
 }
 
 // One range: a flat grid of tiles (DM-major within a time tile).
@@ -958,14 +979,15 @@ __global__ void batch_reset_kernel(int* counters)
 }  // namespace
 
 static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
-                     std::vector<pair_rec>* recs);
+                     int tile, std::vector<pair_rec>* recs);
 static int quad_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
                      std::vector<quad_rec>* recs);
 
 int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
 {
     if (variant < 0 || variant >= kLdsVariants) return -1;
-    if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
+    if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, 1024, nullptr);
+    if (variant == 4) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, 2048, nullptr);
     if (variant == 3) return quad_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
@@ -988,15 +1010,17 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
     return quads > cap ? -1 : (int)quads;
 }
 
-// Pair shape plan: quads per row (window offsets up to e_max need dwords up
-// to e_max + 511), or -1 when some offset exceeds a row or a pair's relative
-// delay is negative.  With recs, also the records
+// Pair shape plan for tiles of `tile` samples (1024: shape 2, 2048: shape
+// 4): quads per row (window offsets up to e_max need dwords up to
+// e_max + tile/2 - 1), or -1 when some offset exceeds a row or a pair's
+// relative delay is negative.  With recs, also the records
 // of every (DM tile, chunk), DM-tile-major, for rows `pitch` bytes apart.
 // Delays follow the kernels' rule (truncation of
 // dmshift * fmaf(d, mdmstep, mstartdm) in float).
 static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
-                     std::vector<pair_rec>* recs)
+                     int tile, std::vector<pair_rec>* recs)
 {
+    const int max_off = pair_max_offset(tile);
     auto make_row = [pitch](int a, int base, int b, int vb) {
         // channel a's window at base; channel b (or none: b < 0) at base + vb
         pair_row r{};
@@ -1037,7 +1061,7 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
                 int ea[kDms], v[kDms], vmin = 0, vmax = 0;
                 for (int w = 0; w < kDms; ++w) {
                     ea[w] = D(w, a) - basea;
-                    if (ea[w] < 0 || ea[w] > kPairMaxOffset) return -1;
+                    if (ea[w] < 0 || ea[w] > max_off) return -1;
                     e_max = std::max(e_max, ea[w]);
                     v[w] = hasb ? D(w, a + 1) - D(w, a) : 0;
                     if (v[w] < 0) return -1;   // relative delay of a pair must not be negative
@@ -1057,7 +1081,7 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
                     const int baseb = d0l[(size_t)a + 1] & ~3;
                     for (int w = 0; w < kDms; ++w) {
                         split_e[nsplit][w] = D(w, a + 1) - baseb;
-                        if (split_e[nsplit][w] < 0 || split_e[nsplit][w] > kPairMaxOffset) return -1;
+                        if (split_e[nsplit][w] < 0 || split_e[nsplit][w] > max_off) return -1;
                         e_max = std::max(e_max, split_e[nsplit][w]);
                     }
                     if (R) {
@@ -1078,7 +1102,7 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
             }
         }
     }
-    return (e_max + kPSplit + 3) / 4;
+    return (e_max + tile / 2 + 3) / 4;
 }
 
 // Quad shape plan (shape 3): as pair_plan, with 4-channel groups; a group
@@ -1228,7 +1252,7 @@ static int quad_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
         }
     }
     if (!fits) return -1;
-    return (e_max + kPSplit + 3) / 4;
+    return (e_max + kQSplit + 3) / 4;
 }
 
 int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep,
@@ -1243,7 +1267,16 @@ int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, 
         }
         return q;
     }
-    if (variant == 2) return lds_pair_tables(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, bytes);
+    if (variant == 2 || variant == 4) {
+        std::vector<pair_rec> recs;
+        const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, variant == 4 ? 2048 : 1024,
+                                bytes ? &recs : nullptr);
+        if (q > 0 && bytes) {
+            bytes->resize(recs.size() * sizeof(pair_rec));
+            std::memcpy(bytes->data(), recs.data(), bytes->size());
+        }
+        return q;
+    }
     return -1;
 }
 
@@ -1251,7 +1284,7 @@ int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartd
                     std::vector<uint8_t>* bytes)
 {
     std::vector<pair_rec> recs;
-    const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, bytes ? &recs : nullptr);
+    const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, 1024, bytes ? &recs : nullptr);
     if (q > 0 && bytes) {
         bytes->resize(recs.size() * sizeof(pair_rec));
         std::memcpy(bytes->data(), recs.data(), bytes->size());
@@ -1259,20 +1292,20 @@ int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartd
     return q;
 }
 
-long lds_tile_samples() { return kT; }
+long lds_tile_samples(int variant) { return variant == 2 || variant == 3 ? 1024 : kT; }
 int lds_tile_dms() { return kDms; }
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s)
 {
     const int cap = variant == 0   ? shape_c32::kRowQuads
                     : variant == 1 ? shape_c16::kRowQuads
-                    : variant == 2 ? kPRQ
-                                   : kQRQ;
+                    : variant == 3 ? kQRQ
+                                   : kPRQ;
     if (variant < 0 || variant >= kLdsVariants || p.row_quads <= 0 || p.row_quads > cap) return hipErrorInvalidValue;
     if (variant >= 2 && !p.pair_recs) return hipErrorInvalidValue;
     if (p.ndms <= 0 || p.t_processed <= 0) return hipSuccess;
     lds_params q = p;
-    const long tile = variant >= 2 ? kPT : kT;
+    const long tile = lds_tile_samples(variant);
     q.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
     q.n_t_tiles = (int)((p.t_processed + tile - 1) / tile);
     const long ntiles = (long)q.n_dm_tiles * q.n_t_tiles;
@@ -1289,23 +1322,28 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
             hipLaunchKernelGGL(dedisperse_quad_kernel<3>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
     } else {
         // quads per lane: the row width rounded up to whole 64-quad items
-        // (a row is at least kPSplit / 4 = 128 quads)
+        // (a row is at least half a tile wide)
         const int ni = (p.row_quads + 63) / 64;
-        if (ni <= 2)
-            hipLaunchKernelGGL(dedisperse_pair_kernel<2>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
-        else if (ni == 3)
-            hipLaunchKernelGGL(dedisperse_pair_kernel<3>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
-        else if (ni == 4)
-            hipLaunchKernelGGL(dedisperse_pair_kernel<4>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
-        else
-            hipLaunchKernelGGL(dedisperse_pair_kernel<5>, dim3((unsigned)grid), dim3(kThreads), 0, s, q);
+        const dim3 g((unsigned)grid), b(kThreads);
+        if (variant == 4) {   // 2048-sample tiles: rows of at least 256 quads
+            if (ni <= 4) hipLaunchKernelGGL((dedisperse_pair_kernel<4, 2048>), g, b, 0, s, q);
+            else hipLaunchKernelGGL((dedisperse_pair_kernel<5, 2048>), g, b, 0, s, q);
+        } else if (ni <= 2) {
+            hipLaunchKernelGGL((dedisperse_pair_kernel<2, 1024>), g, b, 0, s, q);
+        } else if (ni == 3) {
+            hipLaunchKernelGGL((dedisperse_pair_kernel<3, 1024>), g, b, 0, s, q);
+        } else if (ni == 4) {
+            hipLaunchKernelGGL((dedisperse_pair_kernel<4, 1024>), g, b, 0, s, q);
+        } else {
+            hipLaunchKernelGGL((dedisperse_pair_kernel<5, 1024>), g, b, 0, s, q);
+        }
     }
     return hipGetLastError();
 }
 
 void lds_job_tiles(lds_params& p, int variant)
 {
-    const long tile = variant == 2 || variant == 3 ? kPT : kT;   // 16 DMs per tile in every shape
+    const long tile = lds_tile_samples(variant);   // 16 DMs per tile in every shape
     p.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
     p.n_t_tiles = (int)((p.t_processed + tile - 1) / tile);
 }
