@@ -400,8 +400,19 @@ constexpr int kPRowDw = 4 * kPRQ;              // 1280 dwords per row
 constexpr int kPRows = 16;                     // rows per buffer (one per wave)
 constexpr int kPMaxEnt = 2 * kPairs;           // row entries per DM per chunk
 constexpr int kPFlushChunks = 256 / kPairChunk;   // 128 pair sums (<= 510 each) per 16-bit field
-// window offsets e with e + T/2 - 1 < kPRowDw
-__host__ __device__ constexpr int pair_max_offset(int T) { return kPRowDw - T / 2; }
+// Tiles of the 16-byte-read variant (below): 2 x 4 blocks of 252 dwords.
+constexpr int kWBlk = 252;                     // dwords per block: 63 lanes x 4
+constexpr int kWJ = 4;                         // blocks per row read
+constexpr int kWSplit = kWJ * kWBlk;           // 1008 dwords: halves 1008 samples apart
+constexpr int kWT = 2 * kWSplit;               // 2016 samples per tile
+constexpr int kWAcc = 4 * kWJ;                 // packed accumulators per lane
+// Largest window offset e of a tile of T samples: reads of a DM end at dword
+// e + T/2 - 1 (T = kWT: at most e + kWSplit + 3, lane 63's last quad) of an
+// 1280-dword row.
+__host__ __device__ constexpr int pair_max_offset(int T)
+{
+    return T == kWT ? kPRowDw - (kWSplit + 4) : kPRowDw - T / 2;
+}
 static_assert(kPRows == kDms, "one row per wave is built");
 
 // Plan record of one (DM tile, chunk).  A pair whose relative delay takes
@@ -666,6 +677,245 @@ __global__ __launch_bounds__(kThreads) void dedisperse_pair_kernel(lds_params p)
     const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (w >= ntiles) return;  // whole work-group leaves before any barrier
     process_pair_tile<NI, T>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * T);
+}
+
+// ---------------------------------------------------------------------------
+// Pair pre-summation with 16-byte LDS reads (chunk shape 4).  Rows, plan
+// records and the chunk pipeline are those above; what changes is how a DM
+// reads a row.  ds_read_b128 moves about twice the bytes per clock of the
+// dword forms on this part, but only from 16-byte-aligned addresses, and a
+// DM's row offset e is arbitrary.  So lane L reads the aligned quads at
+// A + 252 j + 4 L (A = e rounded down to a multiple of 4 dwords, r = e - A,
+// j < 4) and needs the dwords e + 252 j + 4 L + q (q < 4): the last 4 - r of
+// its own quad and the first r of lane L+1's, which a DPP wave shift (lane L
+// reads lane L+1) brings over inside the add.  Lane 63 reads the quad after
+// lane 62's (the first of block j+1) and owns no samples, so a tile is
+// 2 x 4 x 252 = 2016 samples: dword k = 252 j + 4 L + q of lane L < 63 holds
+// samples t0 + k (low half) and t0 + 1008 + k (high half).  r is the same for
+// the whole wave (one DM per wave): each pair of entries branches, on the
+// scalar unit, to the add code of its two r values.
+// ---------------------------------------------------------------------------
+
+// Lane L's x of lane L + 1 (lane 63 gets 0): DPP wave_shl:1.
+__device__ __forceinline__ uint32_t wide_next(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x130, 0xf, 0xf, true);
+}
+
+// The lane's four quads of the row entry at dword offset e (from the lane's
+// base: row buffer + 4 * lane): D[4j + m] = dword (e & ~3) + 252 j + 4 L + m.
+__device__ __forceinline__ void wide_read(const uint32_t* lane_q, uint32_t e, uint32_t (&D)[kWAcc])
+{
+    const uint32_t* p = lane_q + (e & ~3u);
+#pragma unroll
+    for (int j = 0; j < kWJ; ++j) {
+        const u32x4 v = *reinterpret_cast<const u32x4*>(p + kWBlk * j);
+        D[4 * j] = v.x; D[4 * j + 1] = v.y; D[4 * j + 2] = v.z; D[4 * j + 3] = v.w;
+    }
+}
+
+// acc += lane L+1's x, as one v_add_u32 with the DPP shift on its source:
+// the empty asm keeps the compiler from merging this add with the previous
+// one into a v_add3 (which cannot take a DPP operand).
+__device__ __forceinline__ void wide_add_next(uint32_t& acc, uint32_t x)
+{
+    asm("" : "+v"(acc));
+    acc += wide_next(x);
+}
+
+// acc[4j + q] += dword e + 252 j + 4 L + q of two entries (RA, RB: their e & 3):
+// one v_add3 where both dwords are the lane's own, else one add each.
+template <int RA, int RB>
+__device__ __forceinline__ void wide_acc2(const uint32_t (&Da)[kWAcc], const uint32_t (&Db)[kWAcc],
+                                          uint32_t (&acc)[kWAcc])
+{
+#pragma unroll
+    for (int j = 0; j < kWJ; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int x = 4 * j + q;
+            if (q + RA < 4 && q + RB < 4) {
+                acc[x] += Da[x + RA] + Db[x + RB];
+            } else if (q + RA < 4) {
+                acc[x] += Da[x + RA];
+                wide_add_next(acc[x], Db[x + RB - 4]);
+            } else if (q + RB < 4) {
+                acc[x] += Db[x + RB];
+                wide_add_next(acc[x], Da[x + RA - 4]);
+            } else {
+                wide_add_next(acc[x], Da[x + RA - 4]);
+                wide_add_next(acc[x], Db[x + RB - 4]);
+            }
+        }
+}
+
+template <int RA>
+__device__ __forceinline__ void wide_acc1(const uint32_t (&Da)[kWAcc], uint32_t (&acc)[kWAcc])
+{
+#pragma unroll
+    for (int j = 0; j < kWJ; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int x = 4 * j + q;
+            if (q + RA < 4) acc[x] += Da[x + RA];
+            else wide_add_next(acc[x], Da[x + RA - 4]);
+        }
+}
+
+// Two entries packed in one dword (low, high 16 bits).
+__device__ __forceinline__ void wide_add2(const uint32_t* lane_q, uint32_t e2, uint32_t (&acc)[kWAcc])
+{
+    uint32_t Da[kWAcc], Db[kWAcc];
+    wide_read(lane_q, e2 & 0xffffu, Da);
+    wide_read(lane_q, e2 >> 16, Db);
+    switch (((e2 & 3u) << 2) | ((e2 >> 16) & 3u)) {
+    case 0: wide_acc2<0, 0>(Da, Db, acc); break;
+    case 1: wide_acc2<0, 1>(Da, Db, acc); break;
+    case 2: wide_acc2<0, 2>(Da, Db, acc); break;
+    case 3: wide_acc2<0, 3>(Da, Db, acc); break;
+    case 4: wide_acc2<1, 0>(Da, Db, acc); break;
+    case 5: wide_acc2<1, 1>(Da, Db, acc); break;
+    case 6: wide_acc2<1, 2>(Da, Db, acc); break;
+    case 7: wide_acc2<1, 3>(Da, Db, acc); break;
+    case 8: wide_acc2<2, 0>(Da, Db, acc); break;
+    case 9: wide_acc2<2, 1>(Da, Db, acc); break;
+    case 10: wide_acc2<2, 2>(Da, Db, acc); break;
+    case 11: wide_acc2<2, 3>(Da, Db, acc); break;
+    case 12: wide_acc2<3, 0>(Da, Db, acc); break;
+    case 13: wide_acc2<3, 1>(Da, Db, acc); break;
+    case 14: wide_acc2<3, 2>(Da, Db, acc); break;
+    default: wide_acc2<3, 3>(Da, Db, acc); break;
+    }
+}
+
+__device__ __forceinline__ void wide_add1(const uint32_t* lane_q, uint32_t e, uint32_t (&acc)[kWAcc])
+{
+    uint32_t Da[kWAcc];
+    wide_read(lane_q, e, Da);
+    switch (e & 3u) {
+    case 0: wide_acc1<0>(Da, acc); break;
+    case 1: wide_acc1<1>(Da, acc); break;
+    case 2: wide_acc1<2>(Da, acc); break;
+    default: wide_acc1<3>(Da, acc); break;
+    }
+}
+
+// acc += the entry whose quads are in D (r: its e & 3).
+__device__ __forceinline__ void wide_acc_r(const uint32_t (&D)[kWAcc], uint32_t r, uint32_t (&acc)[kWAcc])
+{
+    switch (r) {
+    case 0: wide_acc1<0>(D, acc); break;
+    case 1: wide_acc1<1>(D, acc); break;
+    case 2: wide_acc1<2>(D, acc); break;
+    default: wide_acc1<3>(D, acc); break;
+    }
+}
+
+// The 8 entries of a full chunk (two per dword of e), one at a time, each
+// entry's reads issued before the previous entry's adds.
+__device__ __forceinline__ void wide_sum8(const uint32_t* lane_q, const uint32_t (&e)[kPMaxEnt / 2],
+                                          uint32_t (&acc)[kWAcc])
+{
+    uint32_t D0[kWAcc], D1[kWAcc];
+    wide_read(lane_q, e[0] & 0xffffu, D0);
+#pragma unroll
+    for (int i = 0; i < kPairs / 2; ++i) {
+        wide_read(lane_q, e[i] >> 16, D1);
+        wide_acc_r(D0, e[i] & 3u, acc);
+        if (i + 1 < kPairs / 2) wide_read(lane_q, e[i + 1] & 0xffffu, D0);
+        wide_acc_r(D1, (e[i] >> 16) & 3u, acc);
+    }
+}
+
+// All n entries of DM slot `wave` in chunk record rc, read from the record
+// (see pair_sum).
+__device__ __forceinline__ void wide_sum(const pair_rec& rc, int wave, int n, const uint32_t* lane_q,
+                                         uint32_t (&acc)[kWAcc])
+{
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
+    int i = 0;
+    for (; i + 1 < n; i += 2) wide_add2(lane_q, ew[i >> 1], acc);
+    if (i < n) wide_add1(lane_q, ew[i >> 1] & 0xffffu, acc);
+}
+
+// The chunk pipeline of process_pair_tile, with the 16-byte-read sums.
+template <int NI>
+__device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, long t0)
+{
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool active = dt * kDms + wave < p.ndms;
+    const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
+    const pair_rec* recs = reinterpret_cast<const pair_rec*>(p.pair_recs) + (size_t)dt * nchunks;
+    uint32_t acc16[kWAcc];
+    uint32_t acc32[2 * kWAcc];
+#pragma unroll
+    for (int j = 0; j < kWAcc; ++j) acc16[j] = 0;
+#pragma unroll
+    for (int j = 0; j < 2 * kWAcc; ++j) acc32[j] = 0;
+
+    pair_build<NI> B;
+    pair_build_load<NI, kWT>(p, t0, recs[0].row[wave], lane, B);
+    if (wave < recs[0].nrows) pair_build_store(wave, lane, B, pair_buf(0));
+    const int k1 = min(1, nchunks - 1);
+    bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
+    pair_build_load<NI, kWT>(p, t0, recs[k1].row[wave], lane, B);
+    int kr = min(2, nchunks - 1);
+    pair_row rn = recs[kr].row[wave];                     // row record of chunk k+2
+    int nrows_n = recs[kr].nrows;
+    pair_ents E = pair_ents_of(recs[0], wave);
+    __syncthreads();
+    const uint32_t* lane_q = reinterpret_cast<const uint32_t*>(pair_buf(0)) + 4 * lane;
+    for (int k = 0; k < nchunks; ++k) {
+        const uint32_t* base = lane_q + ((k & 1) ? kPRows * kPRowDw : 0);   // rows of chunk k
+        uint4* rows_next = pair_buf((k + 1) & 1);
+        if (have_next) pair_build_store(wave, lane, B, rows_next);
+        have_next = k + 2 < nchunks && wave < nrows_n;
+        pair_build_load<NI, kWT>(p, t0, rn, lane, B);
+        if (active) {
+            if (E.n == kPairs) {   // the usual full chunk: straight-line code
+                wide_sum8(base, E.e, acc16);
+            } else {
+                wide_sum(recs[k], wave, E.n, base, acc16);
+            }
+        }
+        if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) {
+#pragma unroll
+            for (int j = 0; j < kWAcc; ++j) {
+                acc32[j] += acc16[j] & 0xFFFFu;
+                acc32[kWAcc + j] += acc16[j] >> 16;
+                acc16[j] = 0;
+            }
+        }
+        kr = min(k + 3, nchunks - 1);
+        rn = recs[kr].row[wave];
+        nrows_n = recs[kr].nrows;
+        E = pair_ents_of(recs[min(k + 1, nchunks - 1)], wave);
+        __syncthreads();
+    }
+
+    if (active && lane < 63) {
+        const int d = dt * kDms + wave;
+        float* orow = p.out + (size_t)d * p.out_pitch;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int x = 0; x < kWAcc; ++x) {
+                const long tt = t0 + h * kWSplit + kWBlk * (x >> 2) + 4 * lane + (x & 3);
+                if (tt < p.t_processed)
+                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * kWAcc + x], p.fnchans), p.fbin);
+            }
+    }
+}
+
+template <int NI>
+__global__ __launch_bounds__(kThreads) void dedisperse_wide_kernel(lds_params p)
+{
+    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (w >= ntiles) return;  // whole work-group leaves before any barrier
+    process_wide_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kWT);
 }
 
 // The XCD this work-group runs on (hardware register XCC_ID, bits 0-3), so
@@ -987,7 +1237,7 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
 {
     if (variant < 0 || variant >= kLdsVariants) return -1;
     if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, 1024, nullptr);
-    if (variant == 4) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, 2048, nullptr);
+    if (variant == 4) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT, nullptr);
     if (variant == 3) return quad_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
@@ -1269,7 +1519,7 @@ int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, 
     }
     if (variant == 2 || variant == 4) {
         std::vector<pair_rec> recs;
-        const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, variant == 4 ? 2048 : 1024,
+        const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, variant == 4 ? kWT : 1024,
                                 bytes ? &recs : nullptr);
         if (q > 0 && bytes) {
             bytes->resize(recs.size() * sizeof(pair_rec));
@@ -1292,7 +1542,7 @@ int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartd
     return q;
 }
 
-long lds_tile_samples(int variant) { return variant == 2 || variant == 3 ? 1024 : kT; }
+long lds_tile_samples(int variant) { return variant == 2 || variant == 3 ? 1024 : variant == 4 ? kWT : kT; }
 int lds_tile_dms() { return kDms; }
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s)
@@ -1325,9 +1575,9 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
         // (a row is at least half a tile wide)
         const int ni = (p.row_quads + 63) / 64;
         const dim3 g((unsigned)grid), b(kThreads);
-        if (variant == 4) {   // 2048-sample tiles: rows of at least 256 quads
-            if (ni <= 4) hipLaunchKernelGGL((dedisperse_pair_kernel<4, 2048>), g, b, 0, s, q);
-            else hipLaunchKernelGGL((dedisperse_pair_kernel<5, 2048>), g, b, 0, s, q);
+        if (variant == 4) {   // 2016-sample tiles: rows of at least 252 quads
+            if (ni <= 4) hipLaunchKernelGGL(dedisperse_wide_kernel<4>, g, b, 0, s, q);
+            else hipLaunchKernelGGL(dedisperse_wide_kernel<5>, g, b, 0, s, q);
         } else if (ni <= 2) {
             hipLaunchKernelGGL((dedisperse_pair_kernel<2, 1024>), g, b, 0, s, q);
         } else if (ni == 3) {
