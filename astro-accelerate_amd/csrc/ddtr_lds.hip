@@ -442,7 +442,7 @@ __device__ __forceinline__ uint4* pair_buf(int b) { return g_prow + b * (kPRows 
 template <int NI>
 struct pair_build {
     uint32_t al[NI], ah[NI], bl[NI], bh[NI];
-    uint32_t mb;
+    uint32_t mb;   // ~0: pair row; 0: single-channel row (b's bytes select zero)
 };
 
 // Loads of one row (R: its record), issued whatever the row's use (no
@@ -477,16 +477,18 @@ template <int NI>
 __device__ __forceinline__ void pair_build_store(int wave, int lane, const pair_build<NI>& B, uint4* rows)
 {
     uint4* r = rows + wave * kPRQ + lane;
+    // byte m of the low word -> bits 0..7, byte m of the high word -> bits
+    // 16..23; channel b's selectors pick zero bytes (0x0c) in a single row
+    const uint32_t z = 0x0c0c0c0cu;
+    const uint32_t sb0 = B.mb ? 0x0c040c00u : z, sb1 = B.mb ? 0x0c050c01u : z;
+    const uint32_t sb2 = B.mb ? 0x0c060c02u : z, sb3 = B.mb ? 0x0c070c03u : z;
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-        const uint32_t bl = B.bl[i] & B.mb;
-        const uint32_t bh = B.bh[i] & B.mb;
-        // byte m of the low word -> bits 0..7, byte m of the high word -> bits 16..23
         uint4 o;
-        o.x = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c040c00u) + __builtin_amdgcn_perm(bh, bl, 0x0c040c00u);
-        o.y = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c050c01u) + __builtin_amdgcn_perm(bh, bl, 0x0c050c01u);
-        o.z = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c060c02u) + __builtin_amdgcn_perm(bh, bl, 0x0c060c02u);
-        o.w = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c070c03u) + __builtin_amdgcn_perm(bh, bl, 0x0c070c03u);
+        o.x = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c040c00u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], sb0);
+        o.y = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c050c01u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], sb1);
+        o.z = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c060c02u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], sb2);
+        o.w = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c070c03u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], sb3);
         r[64 * i] = o;
     }
 }
