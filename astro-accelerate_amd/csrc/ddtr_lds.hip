@@ -812,14 +812,14 @@ __device__ __forceinline__ void wide_acc_r(const uint32_t (&D)[kWAcc], uint32_t 
 }
 
 // The 8 entries of a full chunk (two per dword of e), one at a time, each
-// entry's reads issued before the previous entry's adds.  (A ring of five
+// entry's reads issued before the previous entry's adds (D0: the first
+// entry's quads, read by the caller).  (A ring of five
 // half-entry slots, reading each entry up to two entries ahead, measured
 // 1 % slower.)
 __device__ __forceinline__ void wide_sum8(const uint32_t* lane_q, const uint32_t (&e)[kPMaxEnt / 2],
-                                          uint32_t (&acc)[kWAcc])
+                                          uint32_t (&D0)[kWAcc], uint32_t (&acc)[kWAcc])
 {
-    uint32_t D0[kWAcc], D1[kWAcc];
-    wide_read(lane_q, e[0] & 0xffffu, D0);
+    uint32_t D1[kWAcc];
 #pragma unroll
     for (int i = 0; i < kPairs / 2; ++i) {
         wide_read(lane_q, e[i] >> 16, D1);
@@ -871,12 +871,15 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
     for (int k = 0; k < nchunks; ++k) {
         const uint32_t* base = lane_q + ((k & 1) ? kPRows * kPRowDw : 0);   // rows of chunk k
         uint4* rows_next = pair_buf((k + 1) & 1);
+        // the first entry's reads go ahead of the row stores in the LDS queue
+        uint32_t D0[kWAcc];
+        wide_read(base, E.e[0] & 0xffffu, D0);
         if (have_next) pair_build_store(wave, lane, B, rows_next);
         have_next = k + 2 < nchunks && wave < nrows_n;
         pair_build_load<NI, kWT>(p, t0, rn, lane, B);
         if (active) {
             if (E.n == kPairs) {   // the usual full chunk: straight-line code
-                wide_sum8(base, E.e, acc16);
+                wide_sum8(base, E.e, D0, acc16);
             } else {
                 wide_sum(recs[k], wave, E.n, base, acc16);
             }
