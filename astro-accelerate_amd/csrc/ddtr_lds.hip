@@ -453,6 +453,18 @@ struct pair_build {
 // quads (252 bytes) past the window's needed width, inside the 4096-byte
 // slack every row has past its longest window.
 template <int NI, int T>
+__device__ __forceinline__ void pair_load_item(int i, const uint8_t* ra, const uint8_t* rb, int lane,
+                                               pair_build<NI>& B)
+{
+    const int q = lane + 64 * i;
+    const uint32_t* pa = reinterpret_cast<const uint32_t*>(ra + 4 * q);
+    B.al[i] = pa[0];
+    B.ah[i] = pa[pair_geom<T>::SPLIT / 4];
+    __builtin_memcpy(&B.bl[i], rb + 4 * q, 4);
+    __builtin_memcpy(&B.bh[i], rb + 4 * q + pair_geom<T>::SPLIT, 4);
+}
+
+template <int NI, int T>
 __device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, const pair_row& R, int lane,
                                                 pair_build<NI>& B)
 {
@@ -460,37 +472,42 @@ __device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, co
     const uint8_t* rb = p.L + t0 + R.offb + R.sh;
     B.mb = R.mb;
 #pragma unroll
-    for (int i = 0; i < NI; ++i) {
-        const int q = lane + 64 * i;
-        const uint32_t* pa = reinterpret_cast<const uint32_t*>(ra + 4 * q);
-        B.al[i] = pa[0];
-        B.ah[i] = pa[pair_geom<T>::SPLIT / 4];
-        __builtin_memcpy(&B.bl[i], rb + 4 * q, 4);
-        __builtin_memcpy(&B.bh[i], rb + 4 * q + pair_geom<T>::SPLIT, 4);
-    }
+    for (int i = 0; i < NI; ++i) pair_load_item<NI, T>(i, ra, rb, lane, B);
 }
 
 // Pair sums of row `wave` into `rows`: dword m of quad q is
 // (a[4q+m] + b[4q+m]) | (a[T/2+4q+m] + b[T/2+4q+m]) << 16.  Whole items are
-// stored (rows are 320 quads wide).
+// stored (rows are 320 quads wide).  Byte m of the low word goes to bits
+// 0..7, byte m of the high word to bits 16..23; channel b's selectors pick
+// zero bytes (0x0c) in a single-channel row.
+struct pair_sel {
+    uint32_t s[4];
+};
+
+__device__ __forceinline__ pair_sel pair_sel_of(uint32_t mb)
+{
+    const uint32_t z = 0x0c0c0c0cu;
+    return pair_sel{{mb ? 0x0c040c00u : z, mb ? 0x0c050c01u : z, mb ? 0x0c060c02u : z, mb ? 0x0c070c03u : z}};
+}
+
+template <int NI>
+__device__ __forceinline__ void pair_store_item(int i, const pair_build<NI>& B, const pair_sel& S, uint4* r)
+{
+    uint4 o;
+    o.x = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c040c00u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], S.s[0]);
+    o.y = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c050c01u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], S.s[1]);
+    o.z = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c060c02u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], S.s[2]);
+    o.w = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c070c03u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], S.s[3]);
+    r[64 * i] = o;
+}
+
 template <int NI>
 __device__ __forceinline__ void pair_build_store(int wave, int lane, const pair_build<NI>& B, uint4* rows)
 {
     uint4* r = rows + wave * kPRQ + lane;
-    // byte m of the low word -> bits 0..7, byte m of the high word -> bits
-    // 16..23; channel b's selectors pick zero bytes (0x0c) in a single row
-    const uint32_t z = 0x0c0c0c0cu;
-    const uint32_t sb0 = B.mb ? 0x0c040c00u : z, sb1 = B.mb ? 0x0c050c01u : z;
-    const uint32_t sb2 = B.mb ? 0x0c060c02u : z, sb3 = B.mb ? 0x0c070c03u : z;
+    const pair_sel S = pair_sel_of(B.mb);
 #pragma unroll
-    for (int i = 0; i < NI; ++i) {
-        uint4 o;
-        o.x = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c040c00u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], sb0);
-        o.y = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c050c01u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], sb1);
-        o.z = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c060c02u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], sb2);
-        o.w = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c070c03u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], sb3);
-        r[64 * i] = o;
-    }
+    for (int i = 0; i < NI; ++i) pair_store_item<NI>(i, B, S, r);
 }
 
 // acc16[j] += row dword k = e + L + 64 j of the entries at dword offsets o0
@@ -829,6 +846,37 @@ __device__ __forceinline__ void wide_sum8(const uint32_t* lane_q, const uint32_t
     }
 }
 
+// wide_sum8 with the row build spread over the entries: after entry j's
+// adds, item j of this wave's row of the next chunk is stored (if `store`)
+// and item j of the row after that loaded into the freed registers.  The
+// build's VALU work and LDS writes thus fill the waits on the entries'
+// reads, and the global loads still have a whole chunk to arrive.
+template <int NI>
+__device__ __forceinline__ void wide_sum8_build(const uint32_t* lane_q, const uint32_t (&e)[kPMaxEnt / 2],
+                                                uint32_t (&D0)[kWAcc], uint32_t (&acc)[kWAcc], pair_build<NI>& B,
+                                                bool store, uint4* r, const uint8_t* ra, const uint8_t* rb,
+                                                uint32_t mb_next, int lane)
+{
+    const pair_sel S = pair_sel_of(B.mb);
+    uint32_t D1[kWAcc];
+    auto item = [&](int j) {
+        if (j < NI) {
+            if (store) pair_store_item<NI>(j, B, S, r);
+            pair_load_item<NI, kWT>(j, ra, rb, lane, B);
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < kPairs / 2; ++i) {
+        wide_read(lane_q, e[i] >> 16, D1);
+        wide_acc_r(D0, e[i] & 3u, acc);
+        item(2 * i);
+        if (i + 1 < kPairs / 2) wide_read(lane_q, e[i + 1] & 0xffffu, D0);
+        wide_acc_r(D1, (e[i] >> 16) & 3u, acc);
+        item(2 * i + 1);
+    }
+    B.mb = mb_next;
+}
+
 // All n entries of DM slot `wave` in chunk record rc, read from the record
 // (see pair_sum).
 __device__ __forceinline__ void wide_sum(const pair_rec& rc, int wave, int n, const uint32_t* lane_q,
@@ -874,15 +922,15 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
         // the first entry's reads go ahead of the row stores in the LDS queue
         uint32_t D0[kWAcc];
         wide_read(base, E.e[0] & 0xffffu, D0);
-        if (have_next) pair_build_store(wave, lane, B, rows_next);
+        const bool store = have_next;
         have_next = k + 2 < nchunks && wave < nrows_n;
-        pair_build_load<NI, kWT>(p, t0, rn, lane, B);
-        if (active) {
-            if (E.n == kPairs) {   // the usual full chunk: straight-line code
-                wide_sum8(base, E.e, D0, acc16);
-            } else {
-                wide_sum(recs[k], wave, E.n, base, acc16);
-            }
+        if (active && E.n == kPairs) {   // the usual full chunk: straight-line code
+            wide_sum8_build<NI>(base, E.e, D0, acc16, B, store, rows_next + wave * kPRQ + lane,
+                                p.L + t0 + rn.offa, p.L + t0 + rn.offb + rn.sh, rn.mb, lane);
+        } else {
+            if (store) pair_build_store(wave, lane, B, rows_next);
+            pair_build_load<NI, kWT>(p, t0, rn, lane, B);
+            if (active) wide_sum(recs[k], wave, E.n, base, acc16);
         }
         if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) {
 #pragma unroll
