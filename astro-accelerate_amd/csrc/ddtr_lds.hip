@@ -766,17 +766,19 @@ __device__ __forceinline__ void wide_acc2(const uint32_t (&Da)[kWAcc], const uin
         }
 }
 
+// The lane's own dwords first, then the shifted ones: a DPP add reading an
+// accumulator written by the instruction just before it needs wait states.
 template <int RA>
 __device__ __forceinline__ void wide_acc1(const uint32_t (&Da)[kWAcc], uint32_t (&acc)[kWAcc])
 {
 #pragma unroll
     for (int j = 0; j < kWJ; ++j)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int x = 4 * j + q;
-            if (q + RA < 4) acc[x] += Da[x + RA];
-            else wide_add_next(acc[x], Da[x + RA - 4]);
-        }
+        for (int q = 0; q < 4 - RA; ++q) acc[4 * j + q] += Da[4 * j + q + RA];
+#pragma unroll
+    for (int j = 0; j < kWJ; ++j)
+#pragma unroll
+        for (int q = 4 - RA; q < 4; ++q) wide_add_next(acc[4 * j + q], Da[4 * j + q + RA - 4]);
 }
 
 // Two entries packed in one dword (low, high 16 bits).
