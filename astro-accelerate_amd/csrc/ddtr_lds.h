@@ -15,7 +15,8 @@ constexpr int kLdsRowQuadsMax = 152;
 // Chunk shapes: 0 = 32 channels x 152 quads, 1 = 16 channels x 304 quads,
 // 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums, 1024-sample
 // tiles), 3 = 16 channels pre-summed in groups of four, 4 = shape 2 with
-// 2048-sample tiles (window offsets up to 256 samples).
+// 2016-sample tiles read with 16-byte LDS reads (window offsets up to 268
+// samples).
 constexpr int kLdsVariants = 5;
 
 struct lds_params {

@@ -385,8 +385,9 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
 // registers: the loads of chunk k+1 are issued before chunk k is summed and
 // stored after it; one barrier per chunk.
 // ---------------------------------------------------------------------------
-// Tile widths: T samples (1024 for shape 2, 2048 for shape 4), R = T / 64
-// samples per lane, halves of a dword T / 2 samples apart.
+// Tile widths: T samples (1024 for shape 2; shape 4's 2016-sample tiles use
+// only SPLIT), R = T / 64 samples per lane, halves of a dword T / 2 samples
+// apart.
 template <int T>
 struct pair_geom {
     static constexpr int R = T / 64;
@@ -648,18 +649,12 @@ __device__ __forceinline__ void process_pair_tile(const lds_params& p, int dt, l
         have_next = k + 2 < nchunks && wave < nrows_n;
         pair_build_load<NI, T>(p, t0, rn, lane, B);
         if (active) {
-            if (E.n == kPairs) {   // the usual full chunk: straight-line code
-                if constexpr (R <= 16) {   // four rows per batch of reads
-                    pair_x4<R> X;
-                    pair_read4<R>(base, E.e[0], E.e[1], X);
-                    pair_acc4<R>(X, acc16);
-                    pair_read4<R>(base, E.e[2], E.e[3], X);
-                    pair_acc4<R>(X, acc16);
-                } else {                   // two (the registers of wider tiles)
-#pragma unroll
-                    for (int i = 0; i < kPairs / 2; ++i)
-                        pair_add2<R>(base, (int)(E.e[i] & 0xffffu), (int)(E.e[i] >> 16), acc16);
-                }
+            if (E.n == kPairs) {   // the usual full chunk: straight-line code, four rows per batch of reads
+                pair_x4<R> X;
+                pair_read4<R>(base, E.e[0], E.e[1], X);
+                pair_acc4<R>(X, acc16);
+                pair_read4<R>(base, E.e[2], E.e[3], X);
+                pair_acc4<R>(X, acc16);
             } else {
                 pair_sum<R>(recs[k], wave, E.n, lane, pair_buf(k & 1), acc16);
             }
@@ -1319,7 +1314,7 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
     return quads > cap ? -1 : (int)quads;
 }
 
-// Pair shape plan for tiles of `tile` samples (1024: shape 2, 2048: shape
+// Pair shape plan for tiles of `tile` samples (1024: shape 2, 2016: shape
 // 4): quads per row (window offsets up to e_max need dwords up to
 // e_max + tile/2 - 1), or -1 when some offset exceeds a row or a pair's
 // relative delay is negative.  With recs, also the records

@@ -573,7 +573,7 @@ class DdtrPlan:
     def set_range_variant(self, r: int, variant: int):
         """Preferred LDS chunk shape of range r (0: 32 channels, 1: 16 channels, 2: 16 channels in
         pre-summed pairs on 1024-sample tiles, 3: 16 channels in pre-summed groups of four, 4: shape 2
-        on 2048-sample tiles, -1: first that fits)."""
+        on 2016-sample tiles with 16-byte LDS reads, -1: first that fits)."""
         check(_lib_or_raise().aa_hip_ddtr_set_range_variant(self.handle, int(r), int(variant)), "set_range_variant")
 
     def autotune_range(self, r: int, t_processed: int, out, reps: int = 2, stream=None):

@@ -48,13 +48,13 @@ CONFIGS = {
 ADD_ROOFLINE_T = 78.64   # T adds/s, packed FP32 VALU adds
 # HBM bytes moved by the dedisperse launches of one step, from rocprofv3
 # FETCH_SIZE (x2.000, calibrated on a 1 GiB device copy) + WRITE_SIZE (x1.000)
-# summed over the 8 launches of the pair kernel on 2048-sample tiles (the
-# automatic choice); recorded in
-# profiles/dedisperse_pair2048_8range_4096ch_1048576_hbm_traffic.txt.
+# summed over the 8 launches of chunk shape 4 (pair rows on 2016-sample
+# tiles, the automatic choice); recorded in
+# profiles/dedisperse_wide_8range_4096ch_1048576_hbm_traffic.txt.
 # Counters cannot be read live, so the figure is tied to the kernel it was
 # taken on.
-MEASURED_TRAFFIC = {2: {"bytes": 153.0e9, "kernel": "pair",
-                        "profile": "profiles/dedisperse_pair2048_8range_4096ch_1048576_hbm_traffic.txt"}}
+MEASURED_TRAFFIC = {2: {"bytes": 173.5e9, "kernel": "pair",
+                        "profile": "profiles/dedisperse_wide_8range_4096ch_1048576_hbm_traffic.txt"}}
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
 

@@ -49,7 +49,8 @@ typedef enum {
     AA_HIP_KERNEL_LDS_C32 = 3,  /* LDS kernel, 32-channel chunks (rows up to 2432 bytes) */
     AA_HIP_KERNEL_LDS_C16 = 4,  /* LDS kernel, 16-channel chunks (rows up to 4864 bytes) */
     AA_HIP_KERNEL_LDS_PAIR = 5, /* LDS kernel, 16-channel chunks pre-summed in channel pairs
-                                   (2048-sample tiles where they fit, else 1024) */
+                                   (2016-sample tiles read with 16-byte LDS reads
+                                   where they fit, else 1024-sample tiles) */
     AA_HIP_KERNEL_LDS_QUAD = 6  /* LDS kernel, 16-channel chunks pre-summed in groups of four */
 } aa_hip_kernel;
 
@@ -243,7 +244,8 @@ int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_o
  * channels x 4864-byte rows, 2 = 16 channels pre-added in channel pairs
  * (rows of 16-bit pair sums, one per distinct relative delay of the pair in
  * a 16-DM tile) on 1024-sample tiles, 3 = 16 channels pre-added in groups of
- * four, 4 = shape 2 on 2048-sample tiles (window offsets up to 256 samples).
+ * four, 4 = shape 2 on 2016-sample tiles read with 16-byte LDS reads (window
+ * offsets up to 268 samples).
  * set_range_variant records a preference for range r (-1 = first that
  * fits, in the order 4, 2, 3, 0, 1), used whenever it fits.
  * autotune_range times every shape that fits range r on `stream` (reps
