@@ -13,7 +13,7 @@ namespace aa_hip {
 // 16-channel shape holds rows twice as long in the same buffers.
 constexpr int kLdsRowQuadsMax = 152;
 // Chunk shapes: 0 = 32 channels x 152 quads, 1 = 16 channels x 304 quads,
-// 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums, 1024-sample
+// 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums, 1008-sample
 // tiles), 3 = 16 channels pre-summed in groups of four, 4 = shape 2 with
 // 2016-sample tiles read with 16-byte LDS reads (window offsets up to 268
 // samples).

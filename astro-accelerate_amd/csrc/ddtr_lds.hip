@@ -359,10 +359,10 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
 }
 
 // ---------------------------------------------------------------------------
-// Pair pre-summation (chunk shape 2).  For a 16-DM tile the relative delay
-// v = delay(c+1, d) - delay(c, d) of two adjacent channels takes only a few
-// values (about two: the truncations of nearly equal delays differ by k or
-// k+1).  Per chunk of 16 channels (8 pairs) the work-group builds, for each
+// Pair pre-summation (chunk shapes 2 and 4).  For a 16-DM tile the relative
+// delay v = delay(c+1, d) - delay(c, d) of two adjacent channels takes only a
+// few values (about two: the truncations of nearly equal delays differ by k
+// or k+1).  Per chunk of 16 channels (8 pairs) the work-group builds, for each
 // pair and each of its values v, one row of 16-bit sums
 //     S_v[i] = L[c][base + i] + L[c+1][base + v + i],
 // base = the pair's delay at the tile's first DM rounded down to 4 samples.
@@ -370,24 +370,22 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
 // two channels per 16-bit field, exactly the sum of the single-channel
 // kernel.
 //
-// Tiles are 16 DMs x 1024 samples; each wave sums one DM.  Transposed
-// accumulators: lane L holds output samples t0 + L + 64 j (low half of
-// acc16[j]) and t0 + 512 + L + 64 j (high half), j < 8, and a row stores
-// dword k = S[k] | S[k + 512] << 16.  The eight dwords a lane adds for
-// offset e are then k = e + L + 64 j: every read is stride-1 across the
-// wave whatever e is (two dwords per ds_read2st64_b32), a DM's delay is only
-// a base address, and two rows are summed per v_add3.
+// Tiles are 16 DMs x T samples; each wave sums one DM.  Transposed rows: a
+// row stores dword k = S[k] | S[k + T/2] << 16, so a DM adding its row at
+// offset e reads dwords e + k whatever e is, and each 16-bit half of an
+// accumulator holds one output sample (t0 + k low, t0 + T/2 + k high).  How
+// a lane reads those dwords (16-byte reads, realigned across lanes) is
+// described with dedisperse_wide_kernel below.
 //
 // Which rows a chunk needs and where each DM reads them depend only on the
-// plan, not on the data: the host computes them once per range
-// (pair_build_records: one pair_rec per DM tile and chunk, read by every
-// time tile through L2).  Rows are built from global memory through
-// registers: the loads of chunk k+1 are issued before chunk k is summed and
-// stored after it; one barrier per chunk.
+// plan, not on the data: the host computes them once per range (pair_plan:
+// one pair_rec per DM tile and chunk, read by every time tile through L2).
+// Rows are built from global memory through registers; one barrier per
+// chunk.
 // ---------------------------------------------------------------------------
-// Tile widths: T samples (1024 for shape 2; shape 4's 2016-sample tiles use
-// only SPLIT), R = T / 64 samples per lane, halves of a dword T / 2 samples
-// apart.
+// Tile geometry for T samples: R = T / 64 samples per lane, halves of a
+// dword T / 2 samples apart (the quad shape's 1024-sample tiles; the pair
+// shapes use SPLIT only).
 template <int T>
 struct pair_geom {
     static constexpr int R = T / 64;
@@ -401,18 +399,17 @@ constexpr int kPRowDw = 4 * kPRQ;              // 1280 dwords per row
 constexpr int kPRows = 16;                     // rows per buffer (one per wave)
 constexpr int kPMaxEnt = 2 * kPairs;           // row entries per DM per chunk
 constexpr int kPFlushChunks = 256 / kPairChunk;   // 128 pair sums (<= 510 each) per 16-bit field
-// Tiles of the 16-byte-read variant (below): 2 x 4 blocks of 252 dwords.
+// Tiles of the 16-byte-read kernel (below): 2 x J blocks of 252 dwords,
+// J = 4 (shape 4, 2016 samples) or 2 (shape 2, 1008 samples).
 constexpr int kWBlk = 252;                     // dwords per block: 63 lanes x 4
-constexpr int kWJ = 4;                         // blocks per row read
-constexpr int kWSplit = kWJ * kWBlk;           // 1008 dwords: halves 1008 samples apart
-constexpr int kWT = 2 * kWSplit;               // 2016 samples per tile
-constexpr int kWAcc = 4 * kWJ;                 // packed accumulators per lane
+constexpr int kWT4 = 2 * 4 * kWBlk;            // 2016 samples per shape-4 tile
+constexpr int kWT2 = 2 * 2 * kWBlk;            // 1008 samples per shape-2 tile
 // Largest window offset e of a tile of T samples: reads of a DM end at dword
-// e + T/2 - 1 (T = kWT: at most e + kWSplit + 3, lane 63's last quad) of an
-// 1280-dword row.
+// e + T/2 - 1, or for the 16-byte-read tiles at e + T/2 + 3 (lane 63's last
+// quad), of an 1280-dword row.
 __host__ __device__ constexpr int pair_max_offset(int T)
 {
-    return T == kWT ? kPRowDw - (kWSplit + 4) : kPRowDw - T / 2;
+    return T % (2 * kWBlk) == 0 ? kPRowDw - (T / 2 + 4) : kPRowDw - T / 2;
 }
 static_assert(kPRows == kDms, "one row per wave is built");
 
@@ -511,23 +508,7 @@ __device__ __forceinline__ void pair_build_store(int wave, int lane, const pair_
     for (int i = 0; i < NI; ++i) pair_store_item<NI>(i, B, S, r);
 }
 
-// acc16[j] += row dword k = e + L + 64 j of the entries at dword offsets o0
-// (and o1): two rows per v_add3.
-template <int R>
-__device__ __forceinline__ void pair_add2(const uint32_t* base, int o0, int o1, uint32_t (&acc)[R / 2])
-{
-    const uint32_t* r0 = base + o0;
-    const uint32_t* r1 = base + o1;
-    uint32_t x0[R / 2], x1[R / 2];
-#pragma unroll
-    for (int j = 0; j < R / 2; ++j) {
-        x0[j] = r0[64 * j];
-        x1[j] = r1[64 * j];
-    }
-#pragma unroll
-    for (int j = 0; j < R / 2; ++j) acc[j] += x0[j] + x1[j];
-}
-
+// acc16[j] += row dword k = e + L + 64 j of the entry at dword offset o0.
 template <int R>
 __device__ __forceinline__ void pair_add1(const uint32_t* base, int o0, uint32_t (&acc)[R / 2])
 {
@@ -578,23 +559,6 @@ __device__ __forceinline__ pair_ents pair_ents_of(const pair_rec& rc, int wave)
     return E;
 }
 
-// All n entries of DM slot `wave` in chunk record rc, read from the record
-// itself (a register array indexed at run time would live in scratch
-// memory), two rows per v_add3.
-template <int R>
-__device__ __forceinline__ void pair_sum(const pair_rec& rc, int wave, int n, int lane, const uint4* rows,
-                                         uint32_t (&acc)[R / 2])
-{
-    const uint32_t* base = reinterpret_cast<const uint32_t*>(rows) + lane;
-    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
-    int i = 0;
-    for (; i + 1 < n; i += 2) {
-        const uint32_t e = ew[i >> 1];
-        pair_add2<R>(base, (int)(e & 0xffffu), (int)(e >> 16), acc);
-    }
-    if (i < n) pair_add1<R>(base, (int)(ew[i >> 1] & 0xffffu), acc);
-}
-
 template <int R>
 __device__ __forceinline__ void pair_flush(uint32_t (&acc16)[R / 2], uint32_t (&acc32)[R])
 {
@@ -606,106 +570,21 @@ __device__ __forceinline__ void pair_flush(uint32_t (&acc16)[R / 2], uint32_t (&
     }
 }
 
-// Software pipeline over the chunks: step k stores chunk k+1's rows from
-// the registers loaded during step k-1, issues the loads of chunk k+2, sums
-// chunk k, and ends with the chunk's one barrier; a row's loads thus have a
-// whole step to arrive.  NI: quads per lane of a row (the range's row width
-// rounded up to 64 quads); T: samples per tile.
-template <int NI, int T>
-__device__ __forceinline__ void process_pair_tile(const lds_params& p, int dt, long t0)
-{
-    constexpr int R = pair_geom<T>::R;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const bool active = dt * kDms + wave < p.ndms;
-    const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
-    const pair_rec* recs = reinterpret_cast<const pair_rec*>(p.pair_recs) + (size_t)dt * nchunks;
-    uint32_t acc16[R / 2];
-    uint32_t acc32[R];
-#pragma unroll
-    for (int j = 0; j < R / 2; ++j) acc16[j] = 0;
-#pragma unroll
-    for (int j = 0; j < R; ++j) acc32[j] = 0;
-
-    // Scalar prefetches: the row record of the chunk whose loads the next
-    // step issues, and the entries of the chunk the next step sums; read
-    // just before the barrier, whose wait covers their latency.
-    pair_build<NI> B;
-    pair_build_load<NI, T>(p, t0, recs[0].row[wave], lane, B);
-    if (wave < recs[0].nrows) pair_build_store(wave, lane, B, pair_buf(0));
-    const int k1 = min(1, nchunks - 1);
-    bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
-    pair_build_load<NI, T>(p, t0, recs[k1].row[wave], lane, B);
-    int kr = min(2, nchunks - 1);
-    pair_row rn = recs[kr].row[wave];                     // row record of chunk k+2
-    int nrows_n = recs[kr].nrows;
-    pair_ents E = pair_ents_of(recs[0], wave);
-    __syncthreads();
-    const uint32_t* lane_dw = reinterpret_cast<const uint32_t*>(pair_buf(0)) + lane;
-    for (int k = 0; k < nchunks; ++k) {
-        const uint32_t* base = lane_dw + ((k & 1) ? kPRows * kPRowDw : 0);   // rows of chunk k
-        uint4* rows_next = pair_buf((k + 1) & 1);
-        if (have_next) pair_build_store(wave, lane, B, rows_next);
-        have_next = k + 2 < nchunks && wave < nrows_n;
-        pair_build_load<NI, T>(p, t0, rn, lane, B);
-        if (active) {
-            if (E.n == kPairs) {   // the usual full chunk: straight-line code, four rows per batch of reads
-                pair_x4<R> X;
-                pair_read4<R>(base, E.e[0], E.e[1], X);
-                pair_acc4<R>(X, acc16);
-                pair_read4<R>(base, E.e[2], E.e[3], X);
-                pair_acc4<R>(X, acc16);
-            } else {
-                pair_sum<R>(recs[k], wave, E.n, lane, pair_buf(k & 1), acc16);
-            }
-        }
-        if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) pair_flush<R>(acc16, acc32);
-        kr = min(k + 3, nchunks - 1);
-        rn = recs[kr].row[wave];
-        nrows_n = recs[kr].nrows;
-        E = pair_ents_of(recs[min(k + 1, nchunks - 1)], wave);
-        __syncthreads();
-    }
-
-    if (active) {
-        const int d = dt * kDms + wave;
-        float* orow = p.out + (size_t)d * p.out_pitch;
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int j = 0; j < R / 2; ++j) {
-                const long tt = t0 + h * pair_geom<T>::SPLIT + lane + 64 * j;
-                if (tt < p.t_processed)
-                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * (R / 2) + j], p.fnchans), p.fbin);
-            }
-    }
-}
-
-template <int NI, int T>
-__global__ __launch_bounds__(kThreads) void dedisperse_pair_kernel(lds_params p)
-{
-    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
-    const int per_xcd = (ntiles + 7) / 8;
-    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (w >= ntiles) return;  // whole work-group leaves before any barrier
-    process_pair_tile<NI, T>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * T);
-}
-
 // ---------------------------------------------------------------------------
-// Pair pre-summation with 16-byte LDS reads (chunk shape 4).  Rows, plan
-// records and the chunk pipeline are those above; what changes is how a DM
-// reads a row.  ds_read_b128 moves about twice the bytes per clock of the
-// dword forms on this part, but only from 16-byte-aligned addresses, and a
-// DM's row offset e is arbitrary.  So lane L reads the aligned quads at
-// A + 252 j + 4 L (A = e rounded down to a multiple of 4 dwords, r = e - A,
-// j < 4) and needs the dwords e + 252 j + 4 L + q (q < 4): the last 4 - r of
-// its own quad and the first r of lane L+1's, which a DPP wave shift (lane L
-// reads lane L+1) brings over inside the add.  Lane 63 reads the quad after
-// lane 62's (the first of block j+1) and owns no samples, so a tile is
-// 2 x 4 x 252 = 2016 samples: dword k = 252 j + 4 L + q of lane L < 63 holds
-// samples t0 + k (low half) and t0 + 1008 + k (high half).  r is the same for
-// the whole wave (one DM per wave): each pair of entries branches, on the
-// scalar unit, to the add code of its two r values.
+// Pair pre-summation with 16-byte LDS reads (chunk shapes 4 and 2).  Rows
+// and plan records are those above; this is how a DM reads a row.
+// ds_read_b128 moves about twice the bytes per clock of the dword forms on
+// this part, but only from 16-byte-aligned addresses, and a DM's row offset
+// e is arbitrary.  So lane L reads the aligned quads at A + 252 j + 4 L
+// (A = e rounded down to a multiple of 4 dwords, r = e - A, j < J) and needs
+// the dwords e + 252 j + 4 L + q (q < 4): the last 4 - r of its own quad and
+// the first r of lane L+1's, which a DPP wave shift (lane L reads lane L+1)
+// brings over inside the add.  Lane 63 reads the quad after lane 62's (the
+// first of block j+1) and owns no samples, so a tile is 2 x J x 252 samples
+// (J = 4: 2016, shape 4; J = 2: 1008, shape 2): dword k = 252 j + 4 L + q of
+// lane L < 63 holds samples t0 + k (low half) and t0 + 252 J + k (high
+// half).  r is the same for the whole wave (one DM per wave): each entry
+// branches, on the scalar unit, to the add code of its r.
 // ---------------------------------------------------------------------------
 
 // Lane L's x of lane L + 1 (lane 63 gets 0): DPP wave_shl:1.
@@ -716,11 +595,12 @@ __device__ __forceinline__ uint32_t wide_next(uint32_t x)
 
 // The lane's four quads of the row entry at dword offset e (from the lane's
 // base: row buffer + 4 * lane): D[4j + m] = dword (e & ~3) + 252 j + 4 L + m.
-__device__ __forceinline__ void wide_read(const uint32_t* lane_q, uint32_t e, uint32_t (&D)[kWAcc])
+template <int J>
+__device__ __forceinline__ void wide_read(const uint32_t* lane_q, uint32_t e, uint32_t (&D)[4 * J])
 {
     const uint32_t* p = lane_q + (e & ~3u);
 #pragma unroll
-    for (int j = 0; j < kWJ; ++j) {
+    for (int j = 0; j < J; ++j) {
         const u32x4 v = *reinterpret_cast<const u32x4*>(p + kWBlk * j);
         D[4 * j] = v.x; D[4 * j + 1] = v.y; D[4 * j + 2] = v.z; D[4 * j + 3] = v.w;
     }
@@ -737,12 +617,12 @@ __device__ __forceinline__ void wide_add_next(uint32_t& acc, uint32_t x)
 
 // acc[4j + q] += dword e + 252 j + 4 L + q of two entries (RA, RB: their e & 3):
 // one v_add3 where both dwords are the lane's own, else one add each.
-template <int RA, int RB>
-__device__ __forceinline__ void wide_acc2(const uint32_t (&Da)[kWAcc], const uint32_t (&Db)[kWAcc],
-                                          uint32_t (&acc)[kWAcc])
+template <int RA, int RB, int J>
+__device__ __forceinline__ void wide_acc2(const uint32_t (&Da)[4 * J], const uint32_t (&Db)[4 * J],
+                                          uint32_t (&acc)[4 * J])
 {
 #pragma unroll
-    for (int j = 0; j < kWJ; ++j)
+    for (int j = 0; j < J; ++j)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int x = 4 * j + q;
@@ -763,83 +643,68 @@ __device__ __forceinline__ void wide_acc2(const uint32_t (&Da)[kWAcc], const uin
 
 // The lane's own dwords first, then the shifted ones: a DPP add reading an
 // accumulator written by the instruction just before it needs wait states.
-template <int RA>
-__device__ __forceinline__ void wide_acc1(const uint32_t (&Da)[kWAcc], uint32_t (&acc)[kWAcc])
+template <int RA, int J>
+__device__ __forceinline__ void wide_acc1(const uint32_t (&Da)[4 * J], uint32_t (&acc)[4 * J])
 {
 #pragma unroll
-    for (int j = 0; j < kWJ; ++j)
+    for (int j = 0; j < J; ++j)
 #pragma unroll
         for (int q = 0; q < 4 - RA; ++q) acc[4 * j + q] += Da[4 * j + q + RA];
 #pragma unroll
-    for (int j = 0; j < kWJ; ++j)
+    for (int j = 0; j < J; ++j)
 #pragma unroll
         for (int q = 4 - RA; q < 4; ++q) wide_add_next(acc[4 * j + q], Da[4 * j + q + RA - 4]);
 }
 
 // Two entries packed in one dword (low, high 16 bits).
-__device__ __forceinline__ void wide_add2(const uint32_t* lane_q, uint32_t e2, uint32_t (&acc)[kWAcc])
+template <int J>
+__device__ __forceinline__ void wide_add2(const uint32_t* lane_q, uint32_t e2, uint32_t (&acc)[4 * J])
 {
-    uint32_t Da[kWAcc], Db[kWAcc];
-    wide_read(lane_q, e2 & 0xffffu, Da);
-    wide_read(lane_q, e2 >> 16, Db);
+    uint32_t Da[4 * J], Db[4 * J];
+    wide_read<J>(lane_q, e2 & 0xffffu, Da);
+    wide_read<J>(lane_q, e2 >> 16, Db);
     switch (((e2 & 3u) << 2) | ((e2 >> 16) & 3u)) {
-    case 0: wide_acc2<0, 0>(Da, Db, acc); break;
-    case 1: wide_acc2<0, 1>(Da, Db, acc); break;
-    case 2: wide_acc2<0, 2>(Da, Db, acc); break;
-    case 3: wide_acc2<0, 3>(Da, Db, acc); break;
-    case 4: wide_acc2<1, 0>(Da, Db, acc); break;
-    case 5: wide_acc2<1, 1>(Da, Db, acc); break;
-    case 6: wide_acc2<1, 2>(Da, Db, acc); break;
-    case 7: wide_acc2<1, 3>(Da, Db, acc); break;
-    case 8: wide_acc2<2, 0>(Da, Db, acc); break;
-    case 9: wide_acc2<2, 1>(Da, Db, acc); break;
-    case 10: wide_acc2<2, 2>(Da, Db, acc); break;
-    case 11: wide_acc2<2, 3>(Da, Db, acc); break;
-    case 12: wide_acc2<3, 0>(Da, Db, acc); break;
-    case 13: wide_acc2<3, 1>(Da, Db, acc); break;
-    case 14: wide_acc2<3, 2>(Da, Db, acc); break;
-    default: wide_acc2<3, 3>(Da, Db, acc); break;
+    case 0: wide_acc2<0, 0, J>(Da, Db, acc); break;
+    case 1: wide_acc2<0, 1, J>(Da, Db, acc); break;
+    case 2: wide_acc2<0, 2, J>(Da, Db, acc); break;
+    case 3: wide_acc2<0, 3, J>(Da, Db, acc); break;
+    case 4: wide_acc2<1, 0, J>(Da, Db, acc); break;
+    case 5: wide_acc2<1, 1, J>(Da, Db, acc); break;
+    case 6: wide_acc2<1, 2, J>(Da, Db, acc); break;
+    case 7: wide_acc2<1, 3, J>(Da, Db, acc); break;
+    case 8: wide_acc2<2, 0, J>(Da, Db, acc); break;
+    case 9: wide_acc2<2, 1, J>(Da, Db, acc); break;
+    case 10: wide_acc2<2, 2, J>(Da, Db, acc); break;
+    case 11: wide_acc2<2, 3, J>(Da, Db, acc); break;
+    case 12: wide_acc2<3, 0, J>(Da, Db, acc); break;
+    case 13: wide_acc2<3, 1, J>(Da, Db, acc); break;
+    case 14: wide_acc2<3, 2, J>(Da, Db, acc); break;
+    default: wide_acc2<3, 3, J>(Da, Db, acc); break;
     }
 }
 
-__device__ __forceinline__ void wide_add1(const uint32_t* lane_q, uint32_t e, uint32_t (&acc)[kWAcc])
+template <int J>
+__device__ __forceinline__ void wide_add1(const uint32_t* lane_q, uint32_t e, uint32_t (&acc)[4 * J])
 {
-    uint32_t Da[kWAcc];
-    wide_read(lane_q, e, Da);
+    uint32_t Da[4 * J];
+    wide_read<J>(lane_q, e, Da);
     switch (e & 3u) {
-    case 0: wide_acc1<0>(Da, acc); break;
-    case 1: wide_acc1<1>(Da, acc); break;
-    case 2: wide_acc1<2>(Da, acc); break;
-    default: wide_acc1<3>(Da, acc); break;
+    case 0: wide_acc1<0, J>(Da, acc); break;
+    case 1: wide_acc1<1, J>(Da, acc); break;
+    case 2: wide_acc1<2, J>(Da, acc); break;
+    default: wide_acc1<3, J>(Da, acc); break;
     }
 }
 
 // acc += the entry whose quads are in D (r: its e & 3).
-__device__ __forceinline__ void wide_acc_r(const uint32_t (&D)[kWAcc], uint32_t r, uint32_t (&acc)[kWAcc])
+template <int J>
+__device__ __forceinline__ void wide_acc_r(const uint32_t (&D)[4 * J], uint32_t r, uint32_t (&acc)[4 * J])
 {
     switch (r) {
-    case 0: wide_acc1<0>(D, acc); break;
-    case 1: wide_acc1<1>(D, acc); break;
-    case 2: wide_acc1<2>(D, acc); break;
-    default: wide_acc1<3>(D, acc); break;
-    }
-}
-
-// The 8 entries of a full chunk (two per dword of e), one at a time, each
-// entry's reads issued before the previous entry's adds (D0: the first
-// entry's quads, read by the caller).  (A ring of five
-// half-entry slots, reading each entry up to two entries ahead, measured
-// 1 % slower.)
-__device__ __forceinline__ void wide_sum8(const uint32_t* lane_q, const uint32_t (&e)[kPMaxEnt / 2],
-                                          uint32_t (&D0)[kWAcc], uint32_t (&acc)[kWAcc])
-{
-    uint32_t D1[kWAcc];
-#pragma unroll
-    for (int i = 0; i < kPairs / 2; ++i) {
-        wide_read(lane_q, e[i] >> 16, D1);
-        wide_acc_r(D0, e[i] & 3u, acc);
-        if (i + 1 < kPairs / 2) wide_read(lane_q, e[i + 1] & 0xffffu, D0);
-        wide_acc_r(D1, (e[i] >> 16) & 3u, acc);
+    case 0: wide_acc1<0, J>(D, acc); break;
+    case 1: wide_acc1<1, J>(D, acc); break;
+    case 2: wide_acc1<2, J>(D, acc); break;
+    default: wide_acc1<3, J>(D, acc); break;
     }
 }
 
@@ -851,46 +716,51 @@ __device__ __forceinline__ void wide_sum8(const uint32_t* lane_q, const uint32_t
 // chunk to arrive.  Items go after entries 1, 3, 5, 6, 7 (measured against
 // other placements: entries 0-4 4 % slower, items ahead of the adds 7 %).
 constexpr int kWItemAfter[kPairs] = {-1, 0, -1, 1, -1, 2, 3, 4};
-template <int NI>
+template <int NI, int J>
 __device__ __forceinline__ void wide_sum8_build(const uint32_t* lane_q, const uint32_t (&e)[kPMaxEnt / 2],
-                                                uint32_t (&D0)[kWAcc], uint32_t (&acc)[kWAcc], pair_build<NI>& B,
+                                                uint32_t (&D0)[4 * J], uint32_t (&acc)[4 * J], pair_build<NI>& B,
                                                 bool store, uint4* r, const uint8_t* ra, const uint8_t* rb,
                                                 uint32_t mb_next, int lane)
 {
     const pair_sel S = pair_sel_of(B.mb);
-    uint32_t D1[kWAcc];
+    uint32_t D1[4 * J];
     auto item = [&](int j) {
         const int it = kWItemAfter[j];
         if (it >= 0 && it < NI) {
             if (store) pair_store_item<NI>(it, B, S, r);
-            pair_load_item<NI, kWT>(it, ra, rb, lane, B);
+            pair_load_item<NI, 2 * kWBlk * J>(it, ra, rb, lane, B);
         }
     };
 #pragma unroll
     for (int i = 0; i < kPairs / 2; ++i) {
-        wide_read(lane_q, e[i] >> 16, D1);
-        wide_acc_r(D0, e[i] & 3u, acc);
+        wide_read<J>(lane_q, e[i] >> 16, D1);
+        wide_acc_r<J>(D0, e[i] & 3u, acc);
         item(2 * i);
-        if (i + 1 < kPairs / 2) wide_read(lane_q, e[i + 1] & 0xffffu, D0);
-        wide_acc_r(D1, (e[i] >> 16) & 3u, acc);
+        if (i + 1 < kPairs / 2) wide_read<J>(lane_q, e[i + 1] & 0xffffu, D0);
+        wide_acc_r<J>(D1, (e[i] >> 16) & 3u, acc);
         item(2 * i + 1);
     }
     B.mb = mb_next;
 }
 
 // All n entries of DM slot `wave` in chunk record rc, read from the record
-// (see pair_sum).
+// itself (a register array indexed at run time would live in scratch memory).
+template <int J>
 __device__ __forceinline__ void wide_sum(const pair_rec& rc, int wave, int n, const uint32_t* lane_q,
-                                         uint32_t (&acc)[kWAcc])
+                                         uint32_t (&acc)[4 * J])
 {
     const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
     int i = 0;
-    for (; i + 1 < n; i += 2) wide_add2(lane_q, ew[i >> 1], acc);
-    if (i < n) wide_add1(lane_q, ew[i >> 1] & 0xffffu, acc);
+    for (; i + 1 < n; i += 2) wide_add2<J>(lane_q, ew[i >> 1], acc);
+    if (i < n) wide_add1<J>(lane_q, ew[i >> 1] & 0xffffu, acc);
 }
 
-// The chunk pipeline of process_pair_tile, with the 16-byte-read sums.
-template <int NI>
+// Software pipeline over the chunks: step k sums chunk k, stores chunk k+1's
+// row from the registers loaded during step k-1, issues the loads of chunk
+// k+2, and ends with the chunk's one barrier; a row's loads thus have about
+// a whole step to arrive.  NI: quads per lane of a row (the range's row
+// width rounded up to 64 quads); J: 252-dword blocks per lane read.
+template <int NI, int J>
 __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, long t0)
 {
     const int tid = threadIdx.x, lane = tid & 63;
@@ -898,19 +768,20 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
     const bool active = dt * kDms + wave < p.ndms;
     const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
     const pair_rec* recs = reinterpret_cast<const pair_rec*>(p.pair_recs) + (size_t)dt * nchunks;
-    uint32_t acc16[kWAcc];
-    uint32_t acc32[2 * kWAcc];
+    constexpr int A = 4 * J, SPLIT = kWBlk * J, T = 2 * SPLIT;
+    uint32_t acc16[A];
+    uint32_t acc32[2 * A];
 #pragma unroll
-    for (int j = 0; j < kWAcc; ++j) acc16[j] = 0;
+    for (int j = 0; j < A; ++j) acc16[j] = 0;
 #pragma unroll
-    for (int j = 0; j < 2 * kWAcc; ++j) acc32[j] = 0;
+    for (int j = 0; j < 2 * A; ++j) acc32[j] = 0;
 
     pair_build<NI> B;
-    pair_build_load<NI, kWT>(p, t0, recs[0].row[wave], lane, B);
+    pair_build_load<NI, T>(p, t0, recs[0].row[wave], lane, B);
     if (wave < recs[0].nrows) pair_build_store(wave, lane, B, pair_buf(0));
     const int k1 = min(1, nchunks - 1);
     bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
-    pair_build_load<NI, kWT>(p, t0, recs[k1].row[wave], lane, B);
+    pair_build_load<NI, T>(p, t0, recs[k1].row[wave], lane, B);
     int kr = min(2, nchunks - 1);
     pair_row rn = recs[kr].row[wave];                     // row record of chunk k+2
     int nrows_n = recs[kr].nrows;
@@ -921,23 +792,23 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
         const uint32_t* base = lane_q + ((k & 1) ? kPRows * kPRowDw : 0);   // rows of chunk k
         uint4* rows_next = pair_buf((k + 1) & 1);
         // the first entry's reads go ahead of the row stores in the LDS queue
-        uint32_t D0[kWAcc];
-        wide_read(base, E.e[0] & 0xffffu, D0);
+        uint32_t D0[A];
+        wide_read<J>(base, E.e[0] & 0xffffu, D0);
         const bool store = have_next;
         have_next = k + 2 < nchunks && wave < nrows_n;
         if (active && E.n == kPairs) {   // the usual full chunk: straight-line code
-            wide_sum8_build<NI>(base, E.e, D0, acc16, B, store, rows_next + wave * kPRQ + lane,
+            wide_sum8_build<NI, J>(base, E.e, D0, acc16, B, store, rows_next + wave * kPRQ + lane,
                                 p.L + t0 + rn.offa, p.L + t0 + rn.offb + rn.sh, rn.mb, lane);
         } else {
             if (store) pair_build_store(wave, lane, B, rows_next);
-            pair_build_load<NI, kWT>(p, t0, rn, lane, B);
-            if (active) wide_sum(recs[k], wave, E.n, base, acc16);
+            pair_build_load<NI, T>(p, t0, rn, lane, B);
+            if (active) wide_sum<J>(recs[k], wave, E.n, base, acc16);
         }
         if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) {
 #pragma unroll
-            for (int j = 0; j < kWAcc; ++j) {
+            for (int j = 0; j < A; ++j) {
                 acc32[j] += acc16[j] & 0xFFFFu;
-                acc32[kWAcc + j] += acc16[j] >> 16;
+                acc32[A + j] += acc16[j] >> 16;
                 acc16[j] = 0;
             }
         }
@@ -954,22 +825,22 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int x = 0; x < kWAcc; ++x) {
-                const long tt = t0 + h * kWSplit + kWBlk * (x >> 2) + 4 * lane + (x & 3);
+            for (int x = 0; x < A; ++x) {
+                const long tt = t0 + h * SPLIT + kWBlk * (x >> 2) + 4 * lane + (x & 3);
                 if (tt < p.t_processed)
-                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * kWAcc + x], p.fnchans), p.fbin);
+                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * A + x], p.fnchans), p.fbin);
             }
     }
 }
 
-template <int NI>
+template <int NI, int J>
 __global__ __launch_bounds__(kThreads) void dedisperse_wide_kernel(lds_params p)
 {
     const int ntiles = p.n_dm_tiles * p.n_t_tiles;
     const int per_xcd = (ntiles + 7) / 8;
     const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (w >= ntiles) return;  // whole work-group leaves before any barrier
-    process_wide_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kWT);
+    process_wide_tile<NI, J>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * (2 * kWBlk * J));
 }
 
 // The XCD this work-group runs on (hardware register XCC_ID, bits 0-3), so
@@ -1290,8 +1161,8 @@ static int quad_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
 int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
 {
     if (variant < 0 || variant >= kLdsVariants) return -1;
-    if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, 1024, nullptr);
-    if (variant == 4) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT, nullptr);
+    if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT2, nullptr);
+    if (variant == 4) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT4, nullptr);
     if (variant == 3) return quad_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
@@ -1573,7 +1444,7 @@ int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, 
     }
     if (variant == 2 || variant == 4) {
         std::vector<pair_rec> recs;
-        const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, variant == 4 ? kWT : 1024,
+        const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, variant == 4 ? kWT4 : kWT2,
                                 bytes ? &recs : nullptr);
         if (q > 0 && bytes) {
             bytes->resize(recs.size() * sizeof(pair_rec));
@@ -1588,7 +1459,7 @@ int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartd
                     std::vector<uint8_t>* bytes)
 {
     std::vector<pair_rec> recs;
-    const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, 1024, bytes ? &recs : nullptr);
+    const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, kWT2, bytes ? &recs : nullptr);
     if (q > 0 && bytes) {
         bytes->resize(recs.size() * sizeof(pair_rec));
         std::memcpy(bytes->data(), recs.data(), bytes->size());
@@ -1596,7 +1467,10 @@ int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartd
     return q;
 }
 
-long lds_tile_samples(int variant) { return variant == 2 || variant == 3 ? 1024 : variant == 4 ? kWT : kT; }
+long lds_tile_samples(int variant)
+{
+    return variant == 2 ? kWT2 : variant == 3 ? kQT : variant == 4 ? kWT4 : kT;
+}
 int lds_tile_dms() { return kDms; }
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s)
@@ -1630,16 +1504,16 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
         const int ni = (p.row_quads + 63) / 64;
         const dim3 g((unsigned)grid), b(kThreads);
         if (variant == 4) {   // 2016-sample tiles: rows of at least 252 quads
-            if (ni <= 4) hipLaunchKernelGGL(dedisperse_wide_kernel<4>, g, b, 0, s, q);
-            else hipLaunchKernelGGL(dedisperse_wide_kernel<5>, g, b, 0, s, q);
-        } else if (ni <= 2) {
-            hipLaunchKernelGGL((dedisperse_pair_kernel<2, 1024>), g, b, 0, s, q);
+            if (ni <= 4) hipLaunchKernelGGL((dedisperse_wide_kernel<4, 4>), g, b, 0, s, q);
+            else hipLaunchKernelGGL((dedisperse_wide_kernel<5, 4>), g, b, 0, s, q);
+        } else if (ni <= 2) {   // 1008-sample tiles: rows of at least 126 quads
+            hipLaunchKernelGGL((dedisperse_wide_kernel<2, 2>), g, b, 0, s, q);
         } else if (ni == 3) {
-            hipLaunchKernelGGL((dedisperse_pair_kernel<3, 1024>), g, b, 0, s, q);
+            hipLaunchKernelGGL((dedisperse_wide_kernel<3, 2>), g, b, 0, s, q);
         } else if (ni == 4) {
-            hipLaunchKernelGGL((dedisperse_pair_kernel<4, 1024>), g, b, 0, s, q);
+            hipLaunchKernelGGL((dedisperse_wide_kernel<4, 2>), g, b, 0, s, q);
         } else {
-            hipLaunchKernelGGL((dedisperse_pair_kernel<5, 1024>), g, b, 0, s, q);
+            hipLaunchKernelGGL((dedisperse_wide_kernel<5, 2>), g, b, 0, s, q);
         }
     }
     return hipGetLastError();

@@ -572,7 +572,7 @@ class DdtrPlan:
 
     def set_range_variant(self, r: int, variant: int):
         """Preferred LDS chunk shape of range r (0: 32 channels, 1: 16 channels, 2: 16 channels in
-        pre-summed pairs on 1024-sample tiles, 3: 16 channels in pre-summed groups of four, 4: shape 2
+        pre-summed pairs on 1008-sample tiles, 3: 16 channels in pre-summed groups of four, 4: shape 2
         on 2016-sample tiles with 16-byte LDS reads, -1: first that fits)."""
         check(_lib_or_raise().aa_hip_ddtr_set_range_variant(self.handle, int(r), int(variant)), "set_range_variant")
 

@@ -50,7 +50,7 @@ typedef enum {
     AA_HIP_KERNEL_LDS_C16 = 4,  /* LDS kernel, 16-channel chunks (rows up to 4864 bytes) */
     AA_HIP_KERNEL_LDS_PAIR = 5, /* LDS kernel, 16-channel chunks pre-summed in channel pairs
                                    (2016-sample tiles read with 16-byte LDS reads
-                                   where they fit, else 1024-sample tiles) */
+                                   where they fit, else 1008-sample tiles) */
     AA_HIP_KERNEL_LDS_QUAD = 6  /* LDS kernel, 16-channel chunks pre-summed in groups of four */
 } aa_hip_kernel;
 
@@ -243,7 +243,7 @@ int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_o
  * five channel-chunk shapes: 0 = 32 channels x 2432-byte rows, 1 = 16
  * channels x 4864-byte rows, 2 = 16 channels pre-added in channel pairs
  * (rows of 16-bit pair sums, one per distinct relative delay of the pair in
- * a 16-DM tile) on 1024-sample tiles, 3 = 16 channels pre-added in groups of
+ * a 16-DM tile) on 1008-sample tiles, 3 = 16 channels pre-added in groups of
  * four, 4 = shape 2 on 2016-sample tiles read with 16-byte LDS reads (window
  * offsets up to 268 samples).
  * set_range_variant records a preference for range r (-1 = first that

@@ -76,6 +76,9 @@ GEOMS = [
     (96, 900.0, -2.0, 1e-4, 300.0, 2.0, 70, 1001, 4),
     (77, 1200.0, -3.0, 64e-6, 0.0, 1.0, 48, 2500, 1),      # odd channel count, partial DM tile
     (33, 1400.0, -8.0, 64e-6, 20.0, 0.5, 16, 7000, 2),     # 33 channels: one full chunk + 1
+    (64, 1400.0, -4.0, 64e-6, 0.0, 1.0, 64, 4033, 1),      # 2 x 2016 + 1 samples: one-sample last tile
+    (128, 1400.0, -2.0, 64e-6, 0.0, 2.0, 100, 3500, 1),    # window offsets ~490: pair shape on 1008-sample tiles
+    (96, 1200.0, -3.0, 64e-6, 0.0, 1.5, 70, 2017, 1),      # offsets ~730, near that shape's 772 limit
 ]
 
 
