@@ -91,6 +91,14 @@ class aa_py_pipeline_component_options(ctypes.Structure):
         "fdas_norm")]
 
 
+class filterbank_header(ctypes.Structure):
+    """aa_hip_filterbank_header: the SIGPROC header fields the reader keeps."""
+    _fields_ = [(n, ctypes.c_int) for n in ("telescope_id", "machine_id", "data_type", "barycentric",
+                                            "pulsarcentric", "nbits", "nsamples", "nchans", "nifs")] + \
+               [(n, ctypes.c_double) for n in ("az_start", "za_start", "src_raj", "src_dej", "tstart", "tsamp",
+                                               "fch1", "foff", "fchannel", "refdm", "period")]
+
+
 def _declare(lib: ctypes.CDLL) -> None:
     vp, i, f, d, sz, b = (ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double,
                           ctypes.c_size_t, ctypes.c_bool)
@@ -169,6 +177,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_frac_grouping": ([vp, pi, pi], i),
         "aa_hip_frac_forward": ([vp, vp, i, vp, vp, vp], i),
         "aa_hip_frac_adjoint": ([vp, vp, vp, i, vp, vp, vp, vp], i),
+        "aa_hip_sigproc_open": ([ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(filterbank_header)], i),
+        "aa_hip_sigproc_read_u8": ([vp, i, ctypes.POINTER(ctypes.POINTER(ctypes.c_ubyte)), ctypes.POINTER(sz)], i),
+        "aa_hip_sigproc_read_u16": ([vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_ushort)), ctypes.POINTER(sz)], i),
+        "aa_hip_sigproc_close": ([vp], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -474,6 +486,70 @@ class aa_py_ddtr_strategy:
         low, _, step, ib, _ = self.dm(r)
         tsamp_r = np.float32(np.float32(self.tsamp) * np.float32(ib))
         return np.float32(np.float32(low) / tsamp_r), np.float32(np.float32(step) / tsamp_r)
+
+
+class aa_sigproc_input:
+    """SIGPROC filterbank reader, the reference's aa_sigproc_input
+    (include/aa_sigproc_input.hpp:17-79) over aa_hip_sigproc_*.  read_signal()
+    widens to u16 as the reference does; read_signal_u8() keeps 1/2/4/8-bit
+    data as u8 ([t][c], the dedispersion input), in pinned host memory when
+    asked.  Buffers are numpy views valid while this object lives."""
+
+    def __init__(self, path: str):
+        self.path = path
+        self._h = ctypes.c_void_p()
+        self._hdr = filterbank_header()
+        self._data = None
+
+    def open(self) -> bool:
+        if self._h:
+            return True
+        st = _lib_or_raise().aa_hip_sigproc_open(self.path.encode(), ctypes.byref(self._h), ctypes.byref(self._hdr))
+        if st != 0:
+            self._h = ctypes.c_void_p()
+            self._error = AAHipError(f"aa_hip_sigproc_open: {_lib_or_raise().aa_hip_last_error().decode()}", st)
+            return False
+        return True
+
+    def close(self) -> bool:
+        if self._h:
+            _lib_or_raise().aa_hip_sigproc_close(self._h)
+            self._h = ctypes.c_void_p()
+            self._data = None
+        return True
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def read_metadata(self) -> dict:
+        """The header fields (a dict); raises AAHipError when the file cannot be read."""
+        if not self.open():
+            raise self._error
+        return {n: getattr(self._hdr, n) for n, _ in filterbank_header._fields_}
+
+    def metadata(self):
+        """filterbank_metadata_struct of the header (geometry fields), as the pipeline takes it."""
+        h = self.read_metadata()
+        return make_metadata(h["nchans"], h["nsamples"], h["fch1"], h["foff"], h["tsamp"], nbits=h["nbits"],
+                             tstart=h["tstart"])
+
+    def read_signal(self) -> np.ndarray:
+        """[nsamples, nchans] u16, as the reference's input_buffer()."""
+        ptr, n = ctypes.POINTER(ctypes.c_ushort)(), ctypes.c_size_t()
+        check(_lib_or_raise().aa_hip_sigproc_read_u16(self._h, ctypes.byref(ptr), ctypes.byref(n)), "read_u16")
+        self._data = np.ctypeslib.as_array(ptr, shape=(n.value,)).reshape(self._hdr.nsamples, self._hdr.nchans)
+        return self._data
+
+    def read_signal_u8(self, pinned: bool = False) -> np.ndarray:
+        """[nsamples, nchans] u8 (nbits <= 8); pinned host memory needs a GPU."""
+        ptr, n = ctypes.POINTER(ctypes.c_ubyte)(), ctypes.c_size_t()
+        check(_lib_or_raise().aa_hip_sigproc_read_u8(self._h, int(bool(pinned)), ctypes.byref(ptr), ctypes.byref(n)),
+              "read_u8")
+        self._data = np.ctypeslib.as_array(ptr, shape=(n.value,)).reshape(self._hdr.nsamples, self._hdr.nchans)
+        return self._data
 
 
 def make_plan(ranges: Sequence[Sequence[float]]) -> aa_py_ddtr_plan:

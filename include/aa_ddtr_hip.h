@@ -384,6 +384,29 @@ int aa_hip_refine_run(aa_hip_refine *h, const float *d_x, const float *d_delay_p
                       float scan_lo, float scan_step, int nsteps, float step_max, float step_min,
                       void *stream);
 
+/* SIGPROC filterbank input.  Replaces aa_sigproc_input::read_metadata /
+ * read_signal (include/aa_sigproc_input.hpp:31-47, src/aa_sigproc_input.cpp:52-80,
+ * 85-541) for callers over the C ABI; the C++ class of the same name is in
+ * include/aa_sigproc_input.hpp.  The header fields are those the reference
+ * reads; nsamples comes from the data size.  aa_hip_sigproc_read_u8 keeps
+ * 1/2/4/8-bit data as one byte per sample ([t][c], the dedispersion path's
+ * input) instead of widening to u16, in pinned host memory when `pinned`
+ * (needs a GPU; host-to-device copies from it run asynchronously);
+ * aa_hip_sigproc_read_u16 widens as the reference does (any nbits; 32-bit
+ * floats scaled to 0..65535).  A file is read once, through one of the two;
+ * the buffer lives until aa_hip_sigproc_close. */
+typedef struct aa_hip_sigproc aa_hip_sigproc;
+typedef struct aa_hip_filterbank_header {
+    int telescope_id, machine_id, data_type, barycentric, pulsarcentric;
+    int nbits, nsamples, nchans, nifs;
+    double az_start, za_start, src_raj, src_dej, tstart, tsamp;
+    double fch1, foff, fchannel, refdm, period;
+} aa_hip_filterbank_header;
+int aa_hip_sigproc_open(const char *path, aa_hip_sigproc **out, aa_hip_filterbank_header *hdr);
+int aa_hip_sigproc_read_u8(aa_hip_sigproc *h, int pinned, const uint8_t **data, size_t *bytes);
+int aa_hip_sigproc_read_u16(aa_hip_sigproc *h, const uint16_t **data, size_t *count);
+void aa_hip_sigproc_close(aa_hip_sigproc *h);
+
 #ifdef __cplusplus
 }
 #endif
