@@ -33,8 +33,8 @@ struct py_pipeline {
     aa_device_info device;
     aa_ddtr_pipeline pipe;
     py_pipeline(const aa_pipeline::pipeline &c, const aa_pipeline::pipeline_option &o,
-                const aa_filterbank_metadata &m, const void *input, int bytes, int card)
-        : device(card), pipe(c, o, m, input, bytes, device) {}
+                const aa_filterbank_metadata &m, const void *input, int bytes, int card, size_t cap = 0)
+        : device(card, cap), pipe(c, o, m, input, bytes, device) {}
 };
 
 aa_pipeline::pipeline components_of(const pipeline_components_struct &p) {
@@ -140,6 +140,12 @@ void *aa_py_pipeline_api_u8(pipeline_components_struct pipeline, pipeline_compon
                             int card_number) {
     return new (std::nothrow) py_pipeline(components_of(pipeline), options_of(options), metadata_of(metadata),
                                           input_data, 1, card_number);
+}
+void *aa_hip_py_pipeline_api_u8_capped(pipeline_components_struct pipeline, pipeline_component_option_struct options,
+                                       aa_py_filterbank_metadata_struct metadata, unsigned char const *input_data,
+                                       int card_number, size_t max_device_bytes) {
+    return new (std::nothrow) py_pipeline(components_of(pipeline), options_of(options), metadata_of(metadata),
+                                          input_data, 1, card_number, max_device_bytes);
 }
 void aa_py_pipeline_api_delete(void const *obj) { delete static_cast<py_pipeline const *>(obj); }
 bool aa_py_pipeline_api_bind_ddtr_plan(void *obj, void const *plan) {

@@ -211,6 +211,9 @@ private:
         }
         if (he == hipSuccess) he = hipMalloc(&m_d_output, std::max<size_t>(max_out, 1) * sizeof(float));
         if (he == hipSuccess) he = hipStreamCreate(&m_stream);
+        if (he == hipSuccess) he = hipStreamCreate(&m_copy_stream);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&m_ev_ingested, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&m_ev_copied, hipEventDisableTiming);
         if (he != hipSuccess) {
             LOG(log_level::error, std::string("pipeline setup: ") + hipGetErrorString(he));
             m_finished_setup_failed = true;
@@ -218,14 +221,24 @@ private:
             return false;
         }
         if (!m_host_out &&
-            m_options.find(aa_pipeline::component_option::copy_ddtr_data_to_host) != m_options.end())
-            allocate_host_output();
+            m_options.find(aa_pipeline::component_option::copy_ddtr_data_to_host) != m_options.end() &&
+            !allocate_host_output()) {
+            LOG(log_level::error, "pipeline setup: pinned host allocation of the DM-time output failed");
+            aa_hip::set_last_error("pipeline setup: pinned host allocation of the DM-time output failed");
+            free_host_output();
+            m_finished_setup_failed = true;
+            release_device();
+            return false;
+        }
         return true;
     }
 
     // Host output as float***[range][dm] row pointers into one block per
-    // range, so a whole range is one strided device-to-host copy.
-    void allocate_host_output() {
+    // range, so a whole range is one strided device-to-host copy.  The
+    // blocks are pinned host memory (the copies then run at link speed);
+    // every column of a block is written by the chunk that computes it, as
+    // the reference's malloc'd rows are.
+    bool allocate_host_output() {
         free_host_output();
         const int nranges = (int)m_strategy.get_nRanges();
         m_host_out = (float ***)std::calloc((size_t)nranges, sizeof(float **));
@@ -234,18 +247,24 @@ private:
             size_t total = 0;
             for (int v : m_strategy.t_processed()[(size_t)r]) total += (size_t)v;
             const int nd = m_strategy.ndms((size_t)r);
-            m_host_blocks[(size_t)r] = (float *)std::calloc(std::max<size_t>(total * (size_t)nd, 1), sizeof(float));
+            void *blk = nullptr;
+            if (hipHostMalloc(&blk, std::max<size_t>(total * (size_t)nd, 1) * sizeof(float), hipHostMallocDefault) !=
+                hipSuccess)
+                blk = nullptr;
+            m_host_blocks[(size_t)r] = (float *)blk;
             m_host_out[r] = (float **)std::calloc((size_t)std::max(nd, 1), sizeof(float *));
-            for (int d = 0; d < nd; ++d) m_host_out[r][d] = m_host_blocks[(size_t)r] + (size_t)d * total;
             m_host_total.push_back(total);
+            if (!blk || !m_host_out[r]) return false;
+            for (int d = 0; d < nd; ++d) m_host_out[r][d] = m_host_blocks[(size_t)r] + (size_t)d * total;
         }
+        return true;
     }
 
     void free_host_output() {
         if (m_host_out) {
             for (size_t r = 0; r < m_host_blocks.size(); ++r) {
                 std::free(m_host_out[r]);
-                std::free(m_host_blocks[r]);
+                if (m_host_blocks[r]) (void)hipHostFree(m_host_blocks[r]);
             }
             std::free(m_host_out);
         }
@@ -256,16 +275,33 @@ private:
 
     void release_device() {
         if (m_engine) (void)hipSetDevice(m_engine->device());
+        if (m_copy_stream) {
+            (void)hipStreamSynchronize(m_copy_stream);
+            (void)hipStreamDestroy(m_copy_stream);
+        }
         if (m_stream) {
             (void)hipStreamSynchronize(m_stream);
             (void)hipStreamDestroy(m_stream);
         }
+        if (m_ev_ingested) (void)hipEventDestroy(m_ev_ingested);
+        if (m_ev_copied) (void)hipEventDestroy(m_ev_copied);
         if (m_d_input) (void)hipFree(m_d_input);
         if (m_d_output) (void)hipFree(m_d_output);
+        m_copy_stream = nullptr;
+        m_ev_ingested = nullptr;
+        m_ev_copied = nullptr;
+        m_prefetched_chunk = -1;
         m_stream = nullptr;
         m_d_input = nullptr;
         m_d_output = nullptr;
         m_engine.reset();
+    }
+
+    // Input of time chunk c: t_processed[0][c] + maxshift samples of every
+    // channel ([t][c]).
+    size_t chunk_input_bytes(int c) const {
+        const size_t rows = (size_t)m_strategy.t_processed()[0][(size_t)c] + (size_t)m_strategy.maxshift();
+        return rows * (size_t)m_metadata.nchans() * (size_t)m_bytes;
     }
 
     bool step(int chunk, int range) {
@@ -274,10 +310,14 @@ private:
         (void)hipSetDevice(m_engine->device());
         if (range == 0) {
             const int tp0 = m_strategy.t_processed()[0][(size_t)chunk];
-            const size_t rows = (size_t)tp0 + (size_t)m_strategy.maxshift();
-            const size_t bytes = rows * (size_t)nchans * (size_t)m_bytes;
-            const char *src = (const char *)m_input + (size_t)m_inc * (size_t)nchans * (size_t)m_bytes;
-            hipError_t he = hipMemcpyAsync(m_d_input, src, bytes, hipMemcpyHostToDevice, m_stream);
+            hipError_t he = hipSuccess;
+            if (m_prefetched_chunk == chunk) {   // copied during the previous chunk's ranges
+                he = hipStreamWaitEvent(m_stream, m_ev_copied, 0);
+            } else {
+                const size_t bytes = chunk_input_bytes(chunk);
+                const char *src = (const char *)m_input + (size_t)m_inc * (size_t)nchans * (size_t)m_bytes;
+                he = hipMemcpyAsync(m_d_input, src, bytes, hipMemcpyHostToDevice, m_stream);
+            }
             if (he != hipSuccess) {
                 LOG(log_level::error, std::string("H2D copy: ") + hipGetErrorString(he));
                 return false;
@@ -286,6 +326,24 @@ private:
             if (st != AA_HIP_SUCCESS) {
                 LOG(log_level::error, std::string("corner turn: ") + aa_hip::last_error());
                 return false;
+            }
+            // The corner turn has taken the chunk out of m_d_input: the next
+            // chunk's copy starts behind it on the copy stream and overlaps
+            // this chunk's ranges (and their output copies).
+            if (chunk + 1 < m_strategy.num_tchunks()) {
+                const long inc_next = m_inc + tp0;
+                const char *src = (const char *)m_input + (size_t)inc_next * (size_t)nchans * (size_t)m_bytes;
+                he = hipEventRecord(m_ev_ingested, m_stream);
+                if (he == hipSuccess) he = hipStreamWaitEvent(m_copy_stream, m_ev_ingested, 0);
+                if (he == hipSuccess)
+                    he = hipMemcpyAsync(m_d_input, src, chunk_input_bytes(chunk + 1), hipMemcpyHostToDevice,
+                                        m_copy_stream);
+                if (he == hipSuccess) he = hipEventRecord(m_ev_copied, m_copy_stream);
+                if (he != hipSuccess) {
+                    LOG(log_level::error, std::string("H2D prefetch: ") + hipGetErrorString(he));
+                    return false;
+                }
+                m_prefetched_chunk = chunk + 1;
             }
             if (m_bytes == 2) {
                 st = m_engine->take_input_overflow(m_stream);
@@ -346,6 +404,10 @@ private:
     void *m_d_input = nullptr;
     float *m_d_output = nullptr;
     hipStream_t m_stream = nullptr;
+    hipStream_t m_copy_stream = nullptr;   // next chunk's input copy
+    hipEvent_t m_ev_ingested = nullptr;    // corner turn of the current chunk done
+    hipEvent_t m_ev_copied = nullptr;      // next chunk's input on the device
+    int m_prefetched_chunk = -1;
     float ***m_host_out = nullptr;
     std::vector<float *> m_host_blocks;
     std::vector<size_t> m_host_total;

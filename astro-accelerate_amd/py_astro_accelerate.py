@@ -123,6 +123,9 @@ def _declare(lib: ctypes.CDLL) -> None:
                                 filterbank_metadata_struct, ctypes.POINTER(ctypes.c_ushort), i], vp),
         "aa_py_pipeline_api_u8": ([aa_py_pipeline_components, aa_py_pipeline_component_options,
                                    filterbank_metadata_struct, ctypes.POINTER(ctypes.c_ubyte), i], vp),
+        "aa_hip_py_pipeline_api_u8_capped": ([aa_py_pipeline_components, aa_py_pipeline_component_options,
+                                              filterbank_metadata_struct, ctypes.POINTER(ctypes.c_ubyte), i, sz],
+                                             vp),
         "aa_py_pipeline_api_delete": ([vp], None),
         "aa_py_pipeline_api_bind_ddtr_plan": ([vp, vp], b),
         "aa_py_pipeline_api_ddtr_strategy": ([vp], vp),
@@ -327,18 +330,28 @@ class aa_py_pipeline:
 
     ``input_data`` is a ctypes pointer (or numpy array) of u16 samples in
     [t][c] order, as the reference's SIGPROC reader produces; a numpy uint8
-    array selects the u8 entry point.
+    array selects the u8 entry point (aa_sigproc_input.read_signal_u8's
+    buffer, pinned or not, is one).  ``max_device_bytes`` (u8 only, no
+    reference counterpart) caps the device memory the strategy plans for,
+    which sets the time-chunk size.
     """
 
     def __init__(self, pipeline: aa_py_pipeline_components, pipeline_options: aa_py_pipeline_component_options,
-                 metadata: filterbank_metadata_struct, input_data, card_number: int):
+                 metadata: filterbank_metadata_struct, input_data, card_number: int, max_device_bytes: int = 0):
         lib = _lib_or_raise()
         self._keepalive = input_data
         if isinstance(input_data, np.ndarray):
             if input_data.dtype == np.uint8:
                 ptr = input_data.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
-                self.m_obj = lib.aa_py_pipeline_api_u8(pipeline, pipeline_options, metadata, ptr, int(card_number))
+                if max_device_bytes:
+                    self.m_obj = lib.aa_hip_py_pipeline_api_u8_capped(pipeline, pipeline_options, metadata, ptr,
+                                                                      int(card_number), int(max_device_bytes))
+                else:
+                    self.m_obj = lib.aa_py_pipeline_api_u8(pipeline, pipeline_options, metadata, ptr,
+                                                           int(card_number))
                 return
+            if max_device_bytes:
+                raise TypeError("max_device_bytes needs uint8 input")
             if input_data.dtype != np.uint16:
                 raise TypeError("input_data must be uint16 or uint8")
             input_data = input_data.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort))

@@ -146,6 +146,15 @@ void *aa_py_pipeline_api_u8(pipeline_components_struct pipeline,
                             pipeline_component_option_struct options,
                             aa_py_filterbank_metadata_struct metadata,
                             unsigned char const *input_data, int card_number);
+/* No reference counterpart over the FFI (the C++ API takes the cap through
+ * aa_device_info's constructor, include/aa_device_info.hpp:61-130): the u8
+ * pipeline with the device memory it may plan for capped at
+ * max_device_bytes (0: no cap), which sets the time-chunk size. */
+void *aa_hip_py_pipeline_api_u8_capped(pipeline_components_struct pipeline,
+                                       pipeline_component_option_struct options,
+                                       aa_py_filterbank_metadata_struct metadata,
+                                       unsigned char const *input_data, int card_number,
+                                       size_t max_device_bytes);
 /* Replaces src/aa_py_pipeline_api.cpp:46-48. */
 void aa_py_pipeline_api_delete(void const *obj);
 /* Replaces src/aa_py_pipeline_api.cpp:50-52. */
