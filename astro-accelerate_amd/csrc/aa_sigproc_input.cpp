@@ -4,10 +4,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <unistd.h>
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "aa_ddtr_hip.h"
 #include "ddtr_engine.hpp"
@@ -103,6 +106,32 @@ bool aa_sigproc_input::read_header(aa_filterbank_metadata &metadata)
     return true;
 }
 
+// `bytes` bytes from the data start into dst, in up to 8 slices read by
+// their own threads (pread on the file's descriptor): one thread moves page
+// cache to memory at a few GB/s.
+bool aa_sigproc_input::read_parallel(uint8_t *dst, size_t bytes)
+{
+    const int fd = fileno(m_fp);
+    const size_t kMinSlice = (size_t)64 << 20;
+    const int nthreads = (int)std::max<size_t>(1, std::min<size_t>(8, bytes / kMinSlice));
+    std::vector<std::thread> pool;
+    std::vector<char> ok((size_t)nthreads, 0);
+    for (int i = 0; i < nthreads; ++i) {
+        const size_t a = bytes * (size_t)i / (size_t)nthreads, b = bytes * (size_t)(i + 1) / (size_t)nthreads;
+        pool.emplace_back([=, &ok]() {
+            size_t done = a;
+            while (done < b) {
+                const ssize_t n = pread(fd, dst + done, b - done, (off_t)(m_data_start + (long)done));
+                if (n <= 0) return;
+                done += (size_t)n;
+            }
+            ok[(size_t)i] = 1;
+        });
+    }
+    for (auto &t : pool) t.join();
+    return std::all_of(ok.begin(), ok.end(), [](char c) { return c != 0; });
+}
+
 // Whole records from the data start into u8 (nbits <= 8) or u16 (any nbits),
 // a block of records at a time (src/aa_sigproc_input.cpp:341-541).
 bool aa_sigproc_input::read_records(uint8_t *u8, unsigned short *u16)
@@ -112,7 +141,7 @@ bool aa_sigproc_input::read_records(uint8_t *u8, unsigned short *u16)
     const size_t nsamp = (size_t)m_meta.nsamples();
     const size_t record = nchans * (size_t)nbits / 8;
     if (std::fseek(m_fp, m_data_start, SEEK_SET) != 0) return false;
-    if (nbits == 8 && u8) return std::fread(u8, record, nsamp, m_fp) == nsamp;
+    if (nbits == 8 && u8) return read_parallel(u8, record * nsamp);
     float fmin = 0.0f, bin = 1.0f;
     if (nbits == 32) {   // value range first (src/aa_sigproc_input.cpp:362-378)
         float mx = -10000000000.0f, mn = 10000000000.0f;

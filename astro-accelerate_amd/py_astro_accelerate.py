@@ -565,6 +565,42 @@ class aa_sigproc_input:
         return self._data
 
 
+def write_sigproc(path: str, nbits: int, nchans: int, payload: bytes, nifs: int = 1, strings=None,
+                  **fields) -> None:
+    """Writes a SIGPROC filterbank file: HEADER_START, optional (keyword,
+    string) records, the int and double keywords aa_sigproc_input reads (from
+    ``fields``, with defaults), HEADER_END, then ``payload`` (records of
+    nchans samples at nbits each, [t][c])."""
+    import struct
+
+    def rec(text: str) -> bytes:
+        b = text.encode()
+        return struct.pack("<i", len(b)) + b
+
+    ints = dict(telescope_id=4, machine_id=10, data_type=1, nbits=nbits, nchans=nchans, nifs=nifs,
+                nsamples=(len(payload) * 8) // max(1, nchans * nbits))
+    dbls = dict(tstart=60000.5, tsamp=64e-6, fch1=1550.0, foff=-0.5859, src_raj=0.0, src_dej=0.0,
+                az_start=0.0, za_start=0.0, refdm=0.0, period=0.0)
+    for k, v in fields.items():
+        if k in ints:
+            ints[k] = v
+        elif k in dbls:
+            dbls[k] = v
+        else:
+            raise KeyError(k)
+    h = rec("HEADER_START")
+    for k, v in (strings or {}).items():
+        h += rec(k) + rec(v)
+    for k, v in ints.items():
+        h += rec(k) + struct.pack("<i", int(v))
+    for k, v in dbls.items():
+        h += rec(k) + struct.pack("<d", float(v))
+    h += rec("HEADER_END")
+    with open(path, "wb") as fh:
+        fh.write(h)
+        fh.write(payload)
+
+
 def make_plan(ranges: Sequence[Sequence[float]]) -> aa_py_ddtr_plan:
     """Plan from (low, high, step, inBin[, outBin]) tuples."""
     return aa_py_ddtr_plan([aa_py_dm(r[0], r[1], r[2], int(r[3]), int(r[4]) if len(r) > 4 else int(r[3]))

@@ -75,6 +75,11 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=16, help="DM rows per range in the CPU sample")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy (reference kernel for HBM byte counters)")
+    ap.add_argument("--e2e", action="store_true",
+                    help="end-to-end run instead: SIGPROC file -> reader -> FFI pipeline -> host DM-time "
+                         "planes; prints the real-time factor (not the contract line)")
+    ap.add_argument("--e2e-chunks", type=int, default=1,
+                    help="--e2e: cap device memory so the observation runs in about this many time chunks")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -84,6 +89,8 @@ def main():
     import torch
     import py_astro_accelerate as aa
 
+    if args.e2e:
+        return bench_e2e(args, CONFIGS[args.config if args.config in CONFIGS else 2])
     if args.config == 3:
         return bench_frac(args)
     if args.config == 4:
@@ -223,6 +230,76 @@ def main():
 
 
 FP32_PEAK_TFLOPS = 157.3   # MI355X dense FP32 (packed FMA); non-packed issue is half of it
+
+
+def bench_e2e(args, cfg):
+    """SIGPROC 8-bit file -> aa_sigproc_input.read_signal_u8 (pinned) -> FFI
+    pipeline (H2D of each time chunk, corner turn, binning, dedispersion,
+    D2H into pinned host planes).  The file is written first (untimed) from
+    the same synthetic noise as the device benchmark; the timed part is the
+    read and the pipeline, from bind to the last range.  Real-time factor =
+    observation length / that time."""
+    import tempfile
+    import time
+    import torch
+    import py_astro_accelerate as aa
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nchans, nsamp = cfg["nchans"], cfg["nsamp"]
+    d_in = torch.empty((nsamp, nchans), dtype=torch.uint8, device=dev)
+    aa.synth_noise_u8(SEED, d_in, 128.0, 16.0)
+    torch.cuda.synchronize()
+    host = d_in.cpu().numpy()
+    del d_in
+    tmpdir = os.environ.get("TMPDIR", tempfile.gettempdir())
+    path = os.path.join(tmpdir, f"aa_e2e_{os.getpid()}.fil")
+    aa.write_sigproc(path, 8, nchans, host.tobytes(), fch1=cfg["fch1"], foff=cfg["foff"], tsamp=cfg["tsamp"])
+    del host
+    log(f"[e2e] wrote {path}: {nchans} ch x {nsamp} samples u8 ({os.path.getsize(path) / 1e9:.2f} GB)")
+    plan = aa.make_plan(cfg["ranges"])
+    cap = 0
+    if args.e2e_chunks > 1:   # largest cap (1 GB steps down) that gives that many chunks
+        md0 = aa.make_metadata(nchans, nsamp, cfg["fch1"], cfg["foff"], cfg["tsamp"])
+        for gb in range(200, 1, -1):
+            st0 = aa.aa_py_ddtr_strategy(plan, md0, gb << 30)
+            if st0.ready and st0.num_tchunks >= args.e2e_chunks:
+                cap = gb << 30
+                break
+    try:
+        t0 = time.perf_counter()
+        reader = aa.aa_sigproc_input(path)
+        md = reader.metadata()
+        buf = reader.read_signal_u8(pinned=True)
+        t1 = time.perf_counter()
+        comps = aa.aa_py_pipeline_components()
+        comps.dedispersion = True
+        opts = aa.aa_py_pipeline_component_options()
+        opts.copy_ddtr_data_to_host = True
+        pipe = aa.aa_py_pipeline(comps, opts, md, buf, 0, max_device_bytes=cap)
+        assert pipe.bind_ddtr_plan(plan)
+        steps = 0
+        while pipe.run():
+            steps += 1
+        ok = pipe.status_code() == 0
+        t2 = time.perf_counter()
+        strat = aa.aa_py_ddtr_strategy(plan, md, cap if cap else torch.cuda.mem_get_info()[0])
+        nchunks = strat.num_tchunks
+        pipe.cleanUp()
+        reader.close()
+    finally:
+        os.remove(path)
+    obs_s = nsamp * cfg["tsamp"]
+    total = t2 - t0
+    log(f"[e2e] read {t1 - t0:.3f} s, pipeline {t2 - t1:.3f} s ({nchunks} chunks, {steps} run() calls), "
+        f"observation {obs_s:.2f} s -> real-time factor {obs_s / total:.1f}")
+    print(json.dumps({
+        "metric": "end-to-end real-time factor (SIGPROC file -> host DM-time planes)",
+        "value": obs_s / total, "unit": "x real time", "higher_is_better": True, "ok": ok,
+        "read_s": t1 - t0, "pipeline_s": t2 - t1, "observation_s": obs_s, "time_chunks": nchunks,
+        "config": {"workload": cfg["name"], "nchans": nchans, "nsamp": nsamp, "input": "SIGPROC 8-bit file",
+                   "host_buffers": "pinned", "device_memory_cap_bytes": cap},
+    }), flush=True)
 
 
 def bench_frac(args):

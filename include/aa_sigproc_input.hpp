@@ -68,6 +68,7 @@ public:
 private:
     bool read_header(aa_filterbank_metadata &metadata);
     bool read_records(uint8_t *u8, unsigned short *u16);
+    bool read_parallel(uint8_t *dst, size_t bytes);
     void free_u8();
 
     std::string m_path;

@@ -16,28 +16,13 @@ import pytest
 import sigproc_ref
 
 
-def _str(s: str) -> bytes:
-    b = s.encode()
-    return struct.pack("<i", len(b)) + b
-
-
 def write_fil(path, nbits, nchans, payload: bytes, nifs=1, extra_strings=True, **fields):
-    """A SIGPROC file: header records then the payload."""
-    h = _str("HEADER_START")
-    if extra_strings:
-        h += _str("source_name") + _str("FAKE_PSR") + _str("rawdatafile") + _str("x.raw")
-    ints = dict(telescope_id=4, machine_id=10, data_type=1, nbits=nbits, nchans=nchans, nifs=nifs, nsamples=7)
-    dbls = dict(tstart=60000.5, tsamp=64e-6, fch1=1550.0, foff=-0.5859, src_raj=123456.7, src_dej=-654321.0,
-                az_start=12.5, za_start=33.25, refdm=0.0, period=0.0)
-    dbls.update({k: v for k, v in fields.items() if k in dbls})
-    ints.update({k: v for k, v in fields.items() if k in ints})
-    for k, v in ints.items():
-        h += _str(k) + struct.pack("<i", v)
-    for k, v in dbls.items():
-        h += _str(k) + struct.pack("<d", v)
-    h += _str("HEADER_END")
-    with open(path, "wb") as fh:
-        fh.write(h + payload)
+    """A SIGPROC file through the package's writer (header records, then the payload)."""
+    import py_astro_accelerate as aa
+    strings = {"source_name": "FAKE_PSR", "rawdatafile": "x.raw"} if extra_strings else None
+    base = dict(nsamples=7, src_raj=123456.7, src_dej=-654321.0, az_start=12.5, za_start=33.25)
+    base.update(fields)
+    aa.write_sigproc(path, nbits, nchans, payload, nifs=nifs, strings=strings, **base)
 
 
 # ------------------------------------------------------------ oracle pinning
