@@ -361,6 +361,23 @@ int aa_hip_bin2_u8(const uint8_t *d_in, size_t in_pitch, int nchans, int n_out, 
                     "binning");
 }
 
+int aa_hip_zero_dm_u8(uint8_t *d_data_tc, int nsamp, int nchans, const float *d_bandpass, int nbits, void *stream) {
+    if (nsamp < 0 || nchans <= 0 || nbits < 1 || nbits > 8) return bad("invalid geometry or nbits");
+    if (nsamp == 0) return AA_HIP_SUCCESS;
+    if (!d_data_tc || !d_bandpass) return bad("NULL buffer");
+    return from_hip(aa_hip::launch_zero_dm_u8(d_data_tc, nsamp, nchans, d_bandpass, nbits, (hipStream_t)stream),
+                    "zero-DM");
+}
+
+int aa_hip_zero_dm_u16(uint16_t *d_data_tc, int nsamp, int nchans, const float *d_bandpass, int nbits,
+                       void *stream) {
+    if (nsamp < 0 || nchans <= 0 || nbits < 1 || nbits > 16) return bad("invalid geometry or nbits");
+    if (nsamp == 0) return AA_HIP_SUCCESS;
+    if (!d_data_tc || !d_bandpass) return bad("NULL buffer");
+    return from_hip(aa_hip::launch_zero_dm_u16(d_data_tc, nsamp, nchans, d_bandpass, nbits, (hipStream_t)stream),
+                    "zero-DM");
+}
+
 int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nchans, const float *h_dmshifts,
                          const float *d_dmshifts, int ndms, int t_processed, float mstartdm, float mdmstep,
                          int inBin, float *d_out, size_t out_pitch, int kernel, void *stream) {

@@ -18,6 +18,7 @@
 #include "aa_log.hpp"
 #include "aa_pipeline_api.hpp"
 #include "ddtr_engine.hpp"
+#include "ddtr_kernels.h"
 
 namespace astroaccelerate {
 
@@ -72,8 +73,7 @@ public:
         for (auto o : m_options) {
             // Options that change the dedispersed values are refused; options
             // that only concern output files or other components are ignored.
-            const bool changes_values = o == opt::zero_dm || o == opt::zero_dm_with_outliers ||
-                                        o == opt::input_DDTR_normalization ||
+            const bool changes_values = o == opt::zero_dm_with_outliers || o == opt::input_DDTR_normalization ||
                                         o == opt::output_DDTR_normalization || o == opt::old_rfi;
             if (changes_values) {
                 LOG(log_level::error, "pipeline option " + std::to_string((int)o) +
@@ -81,12 +81,13 @@ public:
                 aa_hip::set_last_error("unsupported pipeline option");
                 return false;
             }
-            if (o != opt::copy_ddtr_data_to_host && o != opt::empty)
+            if (o != opt::copy_ddtr_data_to_host && o != opt::empty && o != opt::zero_dm)
                 LOG(log_level::notice, "pipeline option " + std::to_string((int)o) + " has no effect here (ignored).");
         }
-        if (m_metadata.nbits() != 8) {
-            LOG(log_level::error, "only 8-bit filterbank data is supported (nbits=" +
-                                      std::to_string(m_metadata.nbits()) + ").");
+        const int nb = m_metadata.nbits();
+        if (nb != 1 && nb != 2 && nb != 4 && nb != 8) {
+            LOG(log_level::error, "only 1, 2, 4 or 8-bit filterbank data (one byte per sample in memory) is "
+                                  "supported (nbits=" + std::to_string(nb) + ").");
             aa_hip::set_last_error("unsupported nbits");
             return false;
         }
@@ -210,6 +211,20 @@ private:
             max_out = std::max(max_out, (size_t)tp * (size_t)ndms[(size_t)r]);
         }
         if (he == hipSuccess) he = hipMalloc(&m_d_output, std::max<size_t>(max_out, 1) * sizeof(float));
+        m_zero_dm = m_options.find(aa_pipeline::component_option::zero_dm) != m_options.end();
+        if (he == hipSuccess && m_zero_dm) {   // the strategy's bandpass normalisation, one float per channel
+            const std::vector<float> bp = m_strategy.bandpass_normalization();
+            if (bp.size() != (size_t)nchans) {
+                LOG(log_level::error, "zero-DM: the bandpass normalisation does not have nchans values");
+                aa_hip::set_last_error("zero-DM: bandpass normalisation size");
+                m_finished_setup_failed = true;
+                release_device();
+                return false;
+            }
+            he = hipMalloc(&m_d_bandpass, bp.size() * sizeof(float));
+            if (he == hipSuccess)
+                he = hipMemcpy(m_d_bandpass, bp.data(), bp.size() * sizeof(float), hipMemcpyHostToDevice);
+        }
         if (he == hipSuccess) he = hipStreamCreate(&m_stream);
         if (he == hipSuccess) he = hipStreamCreate(&m_copy_stream);
         if (he == hipSuccess) he = hipEventCreateWithFlags(&m_ev_ingested, hipEventDisableTiming);
@@ -287,6 +302,8 @@ private:
         if (m_ev_copied) (void)hipEventDestroy(m_ev_copied);
         if (m_d_input) (void)hipFree(m_d_input);
         if (m_d_output) (void)hipFree(m_d_output);
+        if (m_d_bandpass) (void)hipFree(m_d_bandpass);
+        m_d_bandpass = nullptr;
         m_copy_stream = nullptr;
         m_ev_ingested = nullptr;
         m_ev_copied = nullptr;
@@ -321,6 +338,18 @@ private:
             if (he != hipSuccess) {
                 LOG(log_level::error, std::string("H2D copy: ") + hipGetErrorString(he));
                 return false;
+            }
+            if (m_zero_dm) {   // the reference filters the [t][c] chunk before its corner turn
+                const int rows = tp0 + m_strategy.maxshift();
+                he = m_bytes == 1
+                         ? aa_hip::launch_zero_dm_u8((uint8_t *)m_d_input, rows, nchans, m_d_bandpass,
+                                                     m_metadata.nbits(), m_stream)
+                         : aa_hip::launch_zero_dm_u16((uint16_t *)m_d_input, rows, nchans, m_d_bandpass,
+                                                      m_metadata.nbits(), m_stream);
+                if (he != hipSuccess) {
+                    LOG(log_level::error, std::string("zero-DM: ") + hipGetErrorString(he));
+                    return false;
+                }
             }
             int st = m_engine->ingest(m_d_input, m_bytes == 1 ? 8 : 16, tp0, m_stream);
             if (st != AA_HIP_SUCCESS) {
@@ -408,6 +437,8 @@ private:
     hipEvent_t m_ev_ingested = nullptr;    // corner turn of the current chunk done
     hipEvent_t m_ev_copied = nullptr;      // next chunk's input on the device
     int m_prefetched_chunk = -1;
+    bool m_zero_dm = false;                // zero-DM filter before each chunk's corner turn
+    float *m_d_bandpass = nullptr;         // its per-channel normalisation
     float ***m_host_out = nullptr;
     std::vector<float *> m_host_blocks;
     std::vector<size_t> m_host_total;

@@ -10,8 +10,11 @@
 #include <mutex>
 #include <vector>
 
+#include "ddtr_kernels.h"
+
 #include "aa_bin_gpu.hpp"
 #include "aa_corner_turn.hpp"
+#include "aa_zero_dm.hpp"
 #include "aa_dedisperse.hpp"
 #include "aa_device_load_data.hpp"
 #include "aa_log.hpp"
@@ -125,6 +128,31 @@ void bin_gpu(unsigned short *const d_input, float *const d_output, const int nch
     hipError_t e = aa_hip::launch_bin2_u16_packed(d_input, nchans, nsamp, d_output, nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) LOG(log_level::error, std::string("bin_gpu: ") + hipGetErrorString(e));
+}
+
+void zero_dm(unsigned short *const d_input, const int nchans, const int nsamp, const int nbits,
+             float *normalization_factor)
+{
+    const hipError_t e = aa_hip::launch_zero_dm_u16(d_input, nsamp, nchans, normalization_factor, nbits, 0);
+    if (e != hipSuccess) LOG(log_level::error, std::string("zero_dm: ") + hipGetErrorString(e));
+    (void)hipDeviceSynchronize();
+}
+
+void zero_dm(unsigned short *const d_input, const int nchans, const int nsamp, const int nbits)
+{
+    if (nchans <= 0 || nbits < 1 || nbits > 16) {
+        LOG(log_level::error, "zero_dm: invalid nchans or nbits");
+        return;
+    }
+    const std::vector<float> bp((size_t)nchans, (float)(((1u << nbits) - 1u) / 2.0));
+    float *d_bp = nullptr;
+    if (hipMalloc(&d_bp, bp.size() * sizeof(float)) != hipSuccess) {
+        LOG(log_level::error, "zero_dm: could not allocate the normalisation factors");
+        return;
+    }
+    if (hipMemcpy(d_bp, bp.data(), bp.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess)
+        zero_dm(d_input, nchans, nsamp, nbits, d_bp);
+    (void)hipFree(d_bp);
 }
 
 }  // namespace astroaccelerate

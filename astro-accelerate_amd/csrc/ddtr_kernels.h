@@ -36,6 +36,10 @@ hipError_t launch_transpose_u16(const uint16_t* in, int rows, int cols,
                                 uint16_t* out, hipStream_t s);
 
 // out[c][t] = (in[c][2t] + in[c][2t+1]) >> 1 for t < n_out.
+// Zero-DM filter in place on [t][c] samples (bp: nchans device floats;
+// values clamped to 2^nbits - 1).
+hipError_t launch_zero_dm_u8(uint8_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s);
+hipError_t launch_zero_dm_u16(uint16_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s);
 hipError_t launch_bin2_u8(const uint8_t* in, size_t in_pitch, int nchans, int n_out,
                           uint8_t* out, size_t out_pitch, hipStream_t s);
 // Reference bin_gpu semantics on u16 rows of length nsamp packed into rows

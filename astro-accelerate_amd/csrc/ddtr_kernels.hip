@@ -219,6 +219,66 @@ __global__ __launch_bounds__(256) void inject_pulse_u8_kernel(uint8_t* __restric
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// ---------------------------------------------------------------------------
+// Zero-DM filter on [t][c] samples, in place (src/aa_device_zero_dm_kernel.cu:
+// 37-88): per time sample, x' = (x - mean over channels) + bandpass[c], in
+// float as written there, clamped to [0, maxval] and truncated.  One wave per
+// time sample; the channel sum is an exact integer, which is what the
+// reference's float accumulator holds while sums stay below 2^24.  HBM-bound:
+// each sample is read twice (sum, then filter; the second read mostly hits
+// L2) and written once.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t zero_dm_wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t zero_dm_one(uint32_t x, float mean, float bp, float maxval)
+{
+    float r = __fadd_rn(__fsub_rn((float)x, mean), bp);
+    r = r < 0.0f ? 0.0f : (r > maxval ? maxval : r);
+    return (uint32_t)r;
+}
+
+// u8 rows of a multiple of 4 channels: one dword (4 channels) per lane step.
+__global__ __launch_bounds__(256) void zero_dm_u8x4_kernel(uint8_t* data, int nsamp, int nchans,
+                                                           const float* __restrict__ bp, float maxval)
+{
+    const int lane = threadIdx.x & 63;
+    const long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= nsamp) return;   // the whole wave leaves; no barrier here
+    uint32_t* row = reinterpret_cast<uint32_t*>(data + t * (long)nchans);
+    const int nq = nchans >> 2;
+    uint32_t sum = 0;
+    for (int q = lane; q < nq; q += 64) sum = __builtin_amdgcn_sad_u8(row[q], 0u, sum);
+    sum = zero_dm_wave_sum(sum);
+    const float mean = __fdiv_rn((float)sum, (float)nchans);
+    for (int q = lane; q < nq; q += 64) {
+        const uint32_t x = row[q];
+        const float4 b = reinterpret_cast<const float4*>(bp)[q];
+        row[q] = zero_dm_one(x & 0xffu, mean, b.x, maxval) | zero_dm_one((x >> 8) & 0xffu, mean, b.y, maxval) << 8 |
+                 zero_dm_one((x >> 16) & 0xffu, mean, b.z, maxval) << 16 |
+                 zero_dm_one(x >> 24, mean, b.w, maxval) << 24;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void zero_dm_kernel(T* data, int nsamp, int nchans, const float* __restrict__ bp,
+                                                      float maxval)
+{
+    const int lane = threadIdx.x & 63;
+    const long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= nsamp) return;
+    T* row = data + t * (long)nchans;
+    uint32_t sum = 0;
+    for (int c = lane; c < nchans; c += 64) sum += row[c];
+    sum = zero_dm_wave_sum(sum);
+    const float mean = __fdiv_rn((float)sum, (float)nchans);
+    for (int c = lane; c < nchans; c += 64) row[c] = (T)zero_dm_one(row[c], mean, bp[c], maxval);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
@@ -250,6 +310,31 @@ hipError_t launch_transpose_u16(const uint16_t* in, int rows, int cols, uint16_t
     if (rows == 0) return hipSuccess;
     dim3 grid(ceil_div(rows, kTile), ceil_div(cols, kTile));
     hipLaunchKernelGGL(transpose_u16_kernel, grid, dim3(256), 0, s, in, rows, cols, out);
+    return hipGetLastError();
+}
+
+static float zero_dm_max(int nbits) { return (float)((1u << (nbits > 16 ? 16 : nbits)) - 1u); }
+
+hipError_t launch_zero_dm_u8(uint8_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s)
+{
+    if (nsamp < 0 || nchans <= 0 || !data_tc || !bp || nbits < 1 || nbits > 16) return hipErrorInvalidValue;
+    if (nsamp == 0) return hipSuccess;
+    const dim3 grid((unsigned)ceil_div((long)nsamp, 4L));
+    if (nchans % 4 == 0 && (uintptr_t)data_tc % 4 == 0 && (uintptr_t)bp % 16 == 0)
+        hipLaunchKernelGGL(zero_dm_u8x4_kernel, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp, zero_dm_max(nbits));
+    else
+        hipLaunchKernelGGL(zero_dm_kernel<uint8_t>, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp,
+                           zero_dm_max(nbits));
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_dm_u16(uint16_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s)
+{
+    if (nsamp < 0 || nchans <= 0 || !data_tc || !bp || nbits < 1 || nbits > 16) return hipErrorInvalidValue;
+    if (nsamp == 0) return hipSuccess;
+    const dim3 grid((unsigned)ceil_div((long)nsamp, 4L));
+    hipLaunchKernelGGL(zero_dm_kernel<uint16_t>, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp,
+                       zero_dm_max(nbits));
     return hipGetLastError();
 }
 

@@ -180,6 +180,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_frac_grouping": ([vp, pi, pi], i),
         "aa_hip_frac_forward": ([vp, vp, i, vp, vp, vp], i),
         "aa_hip_frac_adjoint": ([vp, vp, vp, i, vp, vp, vp, vp], i),
+        "aa_hip_zero_dm_u8": ([vp, i, i, vp, i, vp], i),
+        "aa_hip_zero_dm_u16": ([vp, i, i, vp, i, vp], i),
         "aa_hip_sigproc_open": ([ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(filterbank_header)], i),
         "aa_hip_sigproc_read_u8": ([vp, i, ctypes.POINTER(ctypes.POINTER(ctypes.c_ubyte)), ctypes.POINTER(sz)], i),
         "aa_hip_sigproc_read_u16": ([vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_ushort)), ctypes.POINTER(sz)], i),
@@ -782,6 +784,14 @@ def dedisperse_u8(d_in, pitch: int, row_len: int, dmshifts: np.ndarray, d_dmshif
                                                int(t_processed), float(mstartdm), float(mdmstep), int(inBin),
                                                _ptr(d_out), int(out_pitch), int(kernel), _stream_ptr(stream)),
           "dedisperse_u8")
+
+
+def zero_dm(d_data_tc, d_bandpass, nbits: int = 8, stream=None):
+    """Zero-DM filter in place on a [nsamp, nchans] uint8 or uint16 device tensor
+    (d_bandpass: nchans float32 device values)."""
+    nsamp, nchans = d_data_tc.shape
+    fn = _lib_or_raise().aa_hip_zero_dm_u8 if d_data_tc.element_size() == 1 else _lib_or_raise().aa_hip_zero_dm_u16
+    check(fn(_ptr(d_data_tc), int(nsamp), int(nchans), _ptr(d_bandpass), int(nbits), _stream_ptr(stream)), "zero_dm")
 
 
 def synth_noise_u8(seed: int, d_out, mean: float = 128.0, sigma: float = 16.0, stream=None):

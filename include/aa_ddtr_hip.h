@@ -393,6 +393,19 @@ int aa_hip_refine_run(aa_hip_refine *h, const float *d_x, const float *d_delay_p
                       float scan_lo, float scan_step, int nsteps, float step_max, float step_min,
                       void *stream);
 
+/* Zero-DM filter, in place on [t][c] samples before the corner turn.
+ * Replaces zero_dm(d_input, nchans, nsamp, nbits, d_normalization_factor)
+ * (include/aa_zero_dm.hpp:22, src/aa_zero_dm.cpp:28-37,
+ * src/aa_device_zero_dm_kernel.cu:37-88): per time sample,
+ * x' = (x - mean over channels) + d_bandpass[c] in float, clamped to
+ * [0, 2^nbits - 1] and truncated.  d_bandpass: nchans device floats (the
+ * strategy's bandpass normalisation, default (2^nbits - 1) / 2).  u8: nbits
+ * 1-8; u16: 1-16. */
+int aa_hip_zero_dm_u8(uint8_t *d_data_tc, int nsamp, int nchans, const float *d_bandpass, int nbits,
+                      void *stream);
+int aa_hip_zero_dm_u16(uint16_t *d_data_tc, int nsamp, int nchans, const float *d_bandpass, int nbits,
+                       void *stream);
+
 /* SIGPROC filterbank input.  Replaces aa_sigproc_input::read_metadata /
  * read_signal (include/aa_sigproc_input.hpp:31-47, src/aa_sigproc_input.cpp:52-80,
  * 85-541) for callers over the C ABI; the C++ class of the same name is in

@@ -273,6 +273,50 @@ void oracle_corner_turn_u16(const uint16_t *in_tc, int nsamp, int nchans,
 /* src/aa_device_binning_kernel.cu:43 computes (a+b)/2.0f in float and
  * src/aa_device_corner_turn_kernel.cu:96 truncates it back to an integer, so
  * one level is floor((a+b)/2). */
+/* src/aa_device_zero_dm_kernel.cu:37-88.  The reference sums a time
+ * sample's channels in a float accumulator; the values are integers and the
+ * sum stays below 2^24 (nchans * 65535 < 2^24 for nchans <= 256 at 16 bits,
+ * any nchans up to 65793 at 8 bits), so an int64 sum converted once is the
+ * same float.  Then sum / nchans, then (x - mean) + bandpass[c], each in
+ * float as written; results below 0 give 0, above the nbits maximum give
+ * the maximum, else truncation (the kernel's conversion to unsigned short).
+ * The reference clamps for nbits 4, 8 and 16 and leaves other widths
+ * undefined; here every width clamps to 2^nbits - 1. */
+static float zero_dm_value(float x, float mean, float bp, int nbits)
+{
+    const float r = (x - mean) + bp;
+    const float mx = (float)((1u << (nbits > 16 ? 16 : nbits)) - 1u);
+    if (r < 0.0f) return 0.0f;
+    if (r > mx) return mx;
+    return (float)(unsigned)r;
+}
+
+void oracle_zero_dm_u8(uint8_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits)
+{
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nsamp; t++) {
+        uint8_t *row = data_tc + (size_t)t * (size_t)nchans;
+        int64_t sum = 0;
+        for (int c = 0; c < nchans; c++) sum += row[c];
+        const float mean = (float)sum / (float)nchans;
+        for (int c = 0; c < nchans; c++)
+            row[c] = (uint8_t)(unsigned)zero_dm_value((float)row[c], mean, bandpass[c], nbits);
+    }
+}
+
+void oracle_zero_dm_u16(uint16_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits)
+{
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nsamp; t++) {
+        uint16_t *row = data_tc + (size_t)t * (size_t)nchans;
+        int64_t sum = 0;
+        for (int c = 0; c < nchans; c++) sum += row[c];
+        const float mean = (float)sum / (float)nchans;
+        for (int c = 0; c < nchans; c++)
+            row[c] = (uint16_t)(unsigned)zero_dm_value((float)row[c], mean, bandpass[c], nbits);
+    }
+}
+
 void oracle_bin_u8(const uint8_t *in, size_t in_pitch, int nchans, int n_out,
                    uint8_t *out, size_t out_pitch)
 {

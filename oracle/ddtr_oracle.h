@@ -91,6 +91,12 @@ void oracle_corner_turn_u16(const uint16_t *in_tc, int nsamp, int nchans,
                             uint8_t *out_ct, size_t out_pitch);
 
 /* One x2 level: out[c][t] = floor((in[c][2t] + in[c][2t+1]) / 2), t < n_out. */
+/* Zero-DM filter on [t][c] samples in place (src/aa_device_zero_dm_kernel.cu:
+ * 37-88, src/aa_zero_dm.cpp:28-37): per time sample, mean = (channel sum) /
+ * nchans in float; x' = (x - mean) + bandpass[c] in float, clamped to
+ * [0, 2^nbits - 1] and truncated. */
+void oracle_zero_dm_u8(uint8_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits);
+void oracle_zero_dm_u16(uint16_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits);
 void oracle_bin_u8(const uint8_t *in, size_t in_pitch, int nchans, int n_out,
                    uint8_t *out, size_t out_pitch);
 

@@ -54,6 +54,8 @@ def lib() -> ctypes.CDLL:
             "oracle_corner_turn_u8": ([vp, i, i, vp, sz], None),
             "oracle_corner_turn_u16": ([vp, i, i, vp, sz], None),
             "oracle_bin_u8": ([vp, sz, i, i, vp, sz], None),
+            "oracle_zero_dm_u8": ([vp, i, i, vp, i], None),
+            "oracle_zero_dm_u16": ([vp, i, i, vp, i], None),
             "oracle_dedisperse_range": ([vp, sz, i, i, pf, i, i, f, f, i, pf, sz], None),
             "oracle_dedisperse_rows": ([vp, sz, i, pf, pi, i, i, f, f, i, pf, sz], None),
             "oracle_ddtr_run_u8": ([ctypes.POINTER(_Strategy), vp, ctypes.POINTER(pf)], i),
@@ -151,6 +153,18 @@ def corner_turn(input_tc: np.ndarray, nsamp: int, nchans: int) -> np.ndarray:
         lib().oracle_corner_turn_u8(_vp(input_tc), nsamp, nchans, _vp(out), nsamp)
     else:
         lib().oracle_corner_turn_u16(_vp(np.ascontiguousarray(input_tc, np.uint16)), nsamp, nchans, _vp(out), nsamp)
+    return out
+
+
+def zero_dm(data_tc: np.ndarray, bandpass=None, nbits: int = 8) -> np.ndarray:
+    """Zero-DM filtered copy of [nsamp, nchans] u8 or u16 samples (default
+    bandpass (2^nbits - 1) / 2 per channel, as the strategy sets it)."""
+    out = np.ascontiguousarray(data_tc).copy()
+    nsamp, nchans = out.shape
+    bp = np.full(nchans, ((1 << nbits) - 1.0) / 2.0, dtype=np.float32) if bandpass is None else \
+        np.ascontiguousarray(bandpass, dtype=np.float32)
+    fn = lib().oracle_zero_dm_u8 if out.dtype == np.uint8 else lib().oracle_zero_dm_u16
+    fn(_vp(out), nsamp, nchans, _vp(bp), nbits)
     return out
 
 
