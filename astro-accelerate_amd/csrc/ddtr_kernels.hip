@@ -242,6 +242,49 @@ __device__ __forceinline__ uint32_t zero_dm_one(uint32_t x, float mean, float bp
     return (uint32_t)r;
 }
 
+// u8 rows of a multiple of 16 channels, at most 1024 NQ: the row is read
+// once into registers (NQ 16-byte quads per lane), summed, filtered and
+// written back.
+template <int NQ>
+__global__ __launch_bounds__(256) void zero_dm_u8_reg_kernel(uint8_t* data, int nsamp, int nchans,
+                                                             const float* __restrict__ bp, float maxval)
+{
+    const int lane = threadIdx.x & 63;
+    const long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= nsamp) return;   // the whole wave leaves; no barrier here
+    uint4* row = reinterpret_cast<uint4*>(data + t * (long)nchans);
+    const int nv = nchans >> 4;
+    uint4 v[NQ];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int q = lane + 64 * i;
+        v[i] = q < nv ? row[q] : make_uint4(0, 0, 0, 0);
+        sum = __builtin_amdgcn_sad_u8(v[i].x, 0u, sum);
+        sum = __builtin_amdgcn_sad_u8(v[i].y, 0u, sum);
+        sum = __builtin_amdgcn_sad_u8(v[i].z, 0u, sum);
+        sum = __builtin_amdgcn_sad_u8(v[i].w, 0u, sum);
+    }
+    sum = zero_dm_wave_sum(sum);
+    const float mean = __fdiv_rn((float)sum, (float)nchans);
+    const float4* b4 = reinterpret_cast<const float4*>(bp);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int q = lane + 64 * i;
+        if (q >= nv) continue;
+        uint32_t w[4] = {v[i].x, v[i].y, v[i].z, v[i].w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float4 b = b4[4 * q + k];
+            const uint32_t x = w[k];
+            w[k] = zero_dm_one(x & 0xffu, mean, b.x, maxval) | zero_dm_one((x >> 8) & 0xffu, mean, b.y, maxval) << 8 |
+                   zero_dm_one((x >> 16) & 0xffu, mean, b.z, maxval) << 16 |
+                   zero_dm_one(x >> 24, mean, b.w, maxval) << 24;
+        }
+        row[q] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
 // u8 rows of a multiple of 4 channels: one dword (4 channels) per lane step.
 __global__ __launch_bounds__(256) void zero_dm_u8x4_kernel(uint8_t* data, int nsamp, int nchans,
                                                            const float* __restrict__ bp, float maxval)
@@ -320,8 +363,14 @@ hipError_t launch_zero_dm_u8(uint8_t* data_tc, int nsamp, int nchans, const floa
     if (nsamp < 0 || nchans <= 0 || !data_tc || !bp || nbits < 1 || nbits > 16) return hipErrorInvalidValue;
     if (nsamp == 0) return hipSuccess;
     const dim3 grid((unsigned)ceil_div((long)nsamp, 4L));
-    if (nchans % 4 == 0 && (uintptr_t)data_tc % 4 == 0 && (uintptr_t)bp % 16 == 0)
-        hipLaunchKernelGGL(zero_dm_u8x4_kernel, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp, zero_dm_max(nbits));
+    const float mx = zero_dm_max(nbits);
+    if (nchans % 16 == 0 && nchans <= 8192 && (uintptr_t)data_tc % 16 == 0 && (uintptr_t)bp % 16 == 0) {
+        if (nchans <= 1024) hipLaunchKernelGGL(zero_dm_u8_reg_kernel<1>, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp, mx);
+        else if (nchans <= 2048) hipLaunchKernelGGL(zero_dm_u8_reg_kernel<2>, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp, mx);
+        else if (nchans <= 4096) hipLaunchKernelGGL(zero_dm_u8_reg_kernel<4>, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp, mx);
+        else hipLaunchKernelGGL(zero_dm_u8_reg_kernel<8>, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp, mx);
+    } else if (nchans % 4 == 0 && (uintptr_t)data_tc % 4 == 0 && (uintptr_t)bp % 16 == 0)
+        hipLaunchKernelGGL(zero_dm_u8x4_kernel, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp, mx);
     else
         hipLaunchKernelGGL(zero_dm_kernel<uint8_t>, grid, dim3(256), 0, s, data_tc, nsamp, nchans, bp,
                            zero_dm_max(nbits));

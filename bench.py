@@ -80,6 +80,9 @@ def main():
                          "planes; prints the real-time factor (not the contract line)")
     ap.add_argument("--e2e-chunks", type=int, default=1,
                     help="--e2e: cap device memory so the observation runs in about this many time chunks")
+    ap.add_argument("--zero-dm", action="store_true",
+                    help="--e2e: with the pipeline's zero_dm option; alone: time the zero-DM kernel on the "
+                         "config's [t][c] input (HBM roofline line, not the contract line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -91,6 +94,8 @@ def main():
 
     if args.e2e:
         return bench_e2e(args, CONFIGS[args.config if args.config in CONFIGS else 2])
+    if args.zero_dm:
+        return bench_zero_dm(args, CONFIGS[args.config if args.config in CONFIGS else 2])
     if args.config == 3:
         return bench_frac(args)
     if args.config == 4:
@@ -276,6 +281,7 @@ def bench_e2e(args, cfg):
         comps.dedispersion = True
         opts = aa.aa_py_pipeline_component_options()
         opts.copy_ddtr_data_to_host = True
+        opts.zero_dm = bool(args.zero_dm)
         pipe = aa.aa_py_pipeline(comps, opts, md, buf, 0, max_device_bytes=cap)
         assert pipe.bind_ddtr_plan(plan)
         steps = 0
@@ -298,8 +304,41 @@ def bench_e2e(args, cfg):
         "value": obs_s / total, "unit": "x real time", "higher_is_better": True, "ok": ok,
         "read_s": t1 - t0, "pipeline_s": t2 - t1, "observation_s": obs_s, "time_chunks": nchunks,
         "config": {"workload": cfg["name"], "nchans": nchans, "nsamp": nsamp, "input": "SIGPROC 8-bit file",
-                   "host_buffers": "pinned", "device_memory_cap_bytes": cap},
+                   "host_buffers": "pinned", "device_memory_cap_bytes": cap, "zero_dm": bool(args.zero_dm)},
     }), flush=True)
+
+
+def bench_zero_dm(args, cfg):
+    """The zero-DM kernel on the config's [t][c] u8 input (4096 x 1,048,576
+    for config 2), in place, timed with HIP events on its stream.  HBM-bound:
+    algorithmic bytes = the input read once and written once."""
+    import torch
+    import py_astro_accelerate as aa
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    nchans, nsamp = cfg["nchans"], cfg["nsamp"]
+    d = torch.empty((nsamp, nchans), dtype=torch.uint8, device=dev)
+    aa.synth_noise_u8(SEED, d, 128.0, 16.0)
+    bp = torch.full((nchans,), 127.5, dtype=torch.float32, device=dev)
+    stream = torch.cuda.Stream()
+    for _ in range(args.warmup):
+        aa.zero_dm(d, bp, nbits=8, stream=stream)
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        aa.zero_dm(d, bp, nbits=8, stream=stream)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    nbytes = 2.0 * nchans * nsamp
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"[zero-dm] {nchans} ch x {nsamp} samples u8: {ms:.3f} ms per pass, {gbs:.0f} GB/s "
+        f"({gbs / HBM_PEAK_GBS * 100:.1f}% of {HBM_PEAK_GBS:.0f})")
+    print(json.dumps({"metric": "zero-DM filter throughput", "value": gbs, "unit": "GB/s (input read + written)",
+                      "ms_per_pass": ms, "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                                      "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS},
+                      "config": {"workload": cfg["name"], "nchans": nchans, "nsamp": nsamp}}), flush=True)
 
 
 def bench_frac(args):
