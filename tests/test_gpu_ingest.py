@@ -61,3 +61,38 @@ def test_sigproc_to_pipeline_multi_chunk(aa, oracle, cuda, tmp_path, pinned):
         np.testing.assert_array_equal(planes[r], want[r])
     pipe.cleanUp()
     reader.close()
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4])
+def test_sub_byte_sigproc_to_pipeline(aa, oracle, cuda, tmp_path, nbits):
+    """1/2/4-bit files: the reader unpacks them to one byte per sample and the
+    same kernels dedisperse them (exact sums)."""
+    say(f"{nbits}-bit SIGPROC -> pipeline")
+    nchans, nsamp, tsamp, fch1, foff = 64, 60000, 64e-6, 1400.0, -4.0
+    ranges = [(0, 60, 0.5, 1, 1), (60, 180, 1.0, 2, 2)]
+    rng = np.random.default_rng(nbits)
+    packed = rng.integers(0, 256, size=nsamp * nchans * nbits // 8, dtype=np.uint8).tobytes()
+    path = str(tmp_path / f"b{nbits}.fil")
+    write_fil(path, nbits, nchans, packed, fch1=fch1, foff=foff, tsamp=tsamp)
+    reader = aa.aa_sigproc_input(path)
+    meta = reader.metadata()
+    data = reader.read_signal_u8(pinned=True)
+    assert data.max() <= (1 << nbits) - 1
+    s = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp)
+    want = oracle.run(s, np.ascontiguousarray(data))
+    comps = aa.aa_py_pipeline_components()
+    comps.dedispersion = True
+    opts = aa.aa_py_pipeline_component_options()
+    opts.copy_ddtr_data_to_host = True
+    plan = aa.make_plan(ranges)
+    pipe = aa.aa_py_pipeline(comps, opts, meta, data, 0)
+    assert pipe.bind_ddtr_plan(plan)
+    while pipe.run():
+        pass
+    assert pipe.status_code() == 0
+    strat = aa.aa_py_ddtr_strategy(plan, meta, 280 * 2**30)
+    planes = pipe.output_planes(strat)
+    for r in range(s.nranges):
+        np.testing.assert_array_equal(planes[r], want[r])
+    pipe.cleanUp()
+    reader.close()

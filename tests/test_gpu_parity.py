@@ -79,6 +79,7 @@ GEOMS = [
     (64, 1400.0, -4.0, 64e-6, 0.0, 1.0, 64, 4033, 1),      # 2 x 2016 + 1 samples: one-sample last tile
     (128, 1400.0, -2.0, 64e-6, 0.0, 2.0, 100, 3500, 1),    # window offsets ~490: pair shape on 1008-sample tiles
     (96, 1200.0, -3.0, 64e-6, 0.0, 1.5, 70, 2017, 1),      # offsets ~730, near that shape's 772 limit
+    (16384, 1500.0, -0.0244140625, 64e-6, 0.0, 0.2, 20, 2500, 1),  # above the reference's 8192-channel cap
 ]
 
 
