@@ -22,6 +22,7 @@
 // as in the reference's failsafe kernel (src/aa_device_dedispersion_kernel.cu:448),
 // with delays trunc(dmshift[c] * fma(d, mdmstep, mstartdm)) (:430,440).
 #include "ddtr_lds.h"
+#include "ddtr_pair_common.h"
 
 #include <cstring>
 
@@ -34,8 +35,6 @@ namespace {
 
 constexpr int kR = 32;              // samples per lane
 constexpr int kT = 64 * kR;         // samples per tile
-constexpr int kDms = 16;            // DM trials per work-group (= waves)
-constexpr int kThreads = 64 * kDms; // 1024
 
 // Channel-chunk shape: KC channels per LDS chunk, staged rows of up to RQ
 // uint4.  A staged row holds the even quads of the row's window in its first
@@ -107,8 +106,6 @@ __device__ __forceinline__ void accumulate_one(uint32_t (&acc)[kR / 2], const ui
 // Reads the 3 quads of a lane run from a split row: `row` is the row base,
 // E the run's first quad for lane 0 (uniform).  Quad 2k is at row[k], quad
 // 2k+1 at row[kHalf + k].
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
 template <class S>
 __device__ __forceinline__ void load_run(const uint4* row, int E, int lane, uint32_t (&w)[12])
 {
@@ -356,491 +353,6 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
             if (tt < p.t_processed) orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[j], p.fnchans), p.fbin);
         }
     }
-}
-
-// ---------------------------------------------------------------------------
-// Pair pre-summation (chunk shapes 2 and 4).  For a 16-DM tile the relative
-// delay v = delay(c+1, d) - delay(c, d) of two adjacent channels takes only a
-// few values (about two: the truncations of nearly equal delays differ by k
-// or k+1).  Per chunk of 16 channels (8 pairs) the work-group builds, for each
-// pair and each of its values v, one row of 16-bit sums
-//     S_v[i] = L[c][base + i] + L[c+1][base + v + i],
-// base = the pair's delay at the tile's first DM rounded down to 4 samples.
-// A DM trial then adds the row of its own v at offset e = delay(c, d) - base:
-// two channels per 16-bit field, exactly the sum of the single-channel
-// kernel.
-//
-// Tiles are 16 DMs x T samples; each wave sums one DM.  Transposed rows: a
-// row stores dword k = S[k] | S[k + T/2] << 16, so a DM adding its row at
-// offset e reads dwords e + k whatever e is, and each 16-bit half of an
-// accumulator holds one output sample (t0 + k low, t0 + T/2 + k high).  How
-// a lane reads those dwords (16-byte reads, realigned across lanes) is
-// described with dedisperse_wide_kernel below.
-//
-// Which rows a chunk needs and where each DM reads them depend only on the
-// plan, not on the data: the host computes them once per range (pair_plan:
-// one pair_rec per DM tile and chunk, read by every time tile through L2).
-// Rows are built from global memory through registers; one barrier per
-// chunk.
-// ---------------------------------------------------------------------------
-// Tile geometry for T samples: R = T / 64 samples per lane, halves of a
-// dword T / 2 samples apart (the quad shape's 1024-sample tiles; the pair
-// shapes use SPLIT only).
-template <int T>
-struct pair_geom {
-    static constexpr int R = T / 64;
-    static constexpr int SPLIT = T / 2;
-};
-constexpr int kPairs = 8;                      // pairs per chunk
-constexpr int kPairChunk = 2 * kPairs;         // 16 channels
-constexpr int kPIt = 5;                        // quads per lane per row
-constexpr int kPRQ = 64 * kPIt;                // 320 quads per row
-constexpr int kPRowDw = 4 * kPRQ;              // 1280 dwords per row
-constexpr int kPRows = 16;                     // rows per buffer (one per wave)
-constexpr int kPMaxEnt = 2 * kPairs;           // row entries per DM per chunk
-constexpr int kPFlushChunks = 256 / kPairChunk;   // 128 pair sums (<= 510 each) per 16-bit field
-// Tiles of the 16-byte-read kernel (below): 2 x J blocks of 252 dwords,
-// J = 4 (shape 4, 2016 samples) or 2 (shape 2, 1008 samples).
-constexpr int kWBlk = 252;                     // dwords per block: 63 lanes x 4
-constexpr int kWT4 = 2 * 4 * kWBlk;            // 2016 samples per shape-4 tile
-constexpr int kWT2 = 2 * 2 * kWBlk;            // 1008 samples per shape-2 tile
-// Largest window offset e of a tile of T samples: reads of a DM end at dword
-// e + T/2 - 1, or for the 16-byte-read tiles at e + T/2 + 3 (lane 63's last
-// quad), of an 1280-dword row.
-__host__ __device__ constexpr int pair_max_offset(int T)
-{
-    return T % (2 * kWBlk) == 0 ? kPRowDw - (T / 2 + 4) : kPRowDw - T / 2;
-}
-static_assert(kPRows == kDms, "one row per wave is built");
-
-// Plan record of one (DM tile, chunk).  A pair whose relative delay takes
-// more than two values in the tile is split into two single-channel rows
-// (each DM then adds both), so a chunk never needs more than 16 rows or a DM
-// more than 16 entries; entries 0 .. npairs-1 belong to the pairs in order,
-// the second rows of split pairs follow.
-struct pair_row {
-    long long offa, offb;   // bytes from the level start to the two channels' window starts (offb: dword floor)
-    uint32_t sh, mb;        // byte shift of b's window in its dwords; ~0 pair row, 0 single row (b masked off)
-    int pad0, pad1;
-};
-struct pair_rec {
-    uint16_t ent[kDms][kPMaxEnt];  // per DM slot: dword offset (row * kPRowDw + e) of each entry
-    pair_row row[kPRows];
-    int nrows, nent, pad0, pad1;
-};
-static_assert(sizeof(pair_row) == 32 && sizeof(pair_rec) % 16 == 0, "records are 16-byte aligned");
-static_assert(kPRows * kPRowDw <= 65536, "entry offsets fit 16 bits");
-
-// Two row buffers of 80 KiB (the whole LDS) in one array: buffer b starts
-// at quad b * kPRows * kPRQ.
-__shared__ uint4 g_prow[2 * kPRows * kPRQ];
-__device__ __forceinline__ uint4* pair_buf(int b) { return g_prow + b * (kPRows * kPRQ); }
-
-// Registers of one row in flight: quads q = lane + 64 i, i < NI.
-template <int NI>
-struct pair_build {
-    uint32_t al[NI], ah[NI], bl[NI], bh[NI];
-    uint32_t mb;   // ~0: pair row; 0: single-channel row (b's bytes select zero)
-};
-
-// Loads of one row (R: its record), issued whatever the row's use (no
-// register merges on a branch): a wave without a row in the chunk reads the
-// zeroed record's window at the level start.  Channel b's window starts at
-// any byte: its dwords are read unaligned (one global_load_dword each; the
-// memory path handles the misalignment).  A lane over-reads at most 63
-// quads (252 bytes) past the window's needed width, inside the 4096-byte
-// slack every row has past its longest window.
-template <int NI, int T>
-__device__ __forceinline__ void pair_load_item(int i, const uint8_t* ra, const uint8_t* rb, int lane,
-                                               pair_build<NI>& B)
-{
-    const int q = lane + 64 * i;
-    const uint32_t* pa = reinterpret_cast<const uint32_t*>(ra + 4 * q);
-    B.al[i] = pa[0];
-    B.ah[i] = pa[pair_geom<T>::SPLIT / 4];
-    __builtin_memcpy(&B.bl[i], rb + 4 * q, 4);
-    __builtin_memcpy(&B.bh[i], rb + 4 * q + pair_geom<T>::SPLIT, 4);
-}
-
-template <int NI, int T>
-__device__ __forceinline__ void pair_build_load(const lds_params& p, long t0, const pair_row& R, int lane,
-                                                pair_build<NI>& B)
-{
-    const uint8_t* ra = p.L + t0 + R.offa;
-    const uint8_t* rb = p.L + t0 + R.offb + R.sh;
-    B.mb = R.mb;
-#pragma unroll
-    for (int i = 0; i < NI; ++i) pair_load_item<NI, T>(i, ra, rb, lane, B);
-}
-
-// Pair sums of row `wave` into `rows`: dword m of quad q is
-// (a[4q+m] + b[4q+m]) | (a[T/2+4q+m] + b[T/2+4q+m]) << 16.  Whole items are
-// stored (rows are 320 quads wide).  Byte m of the low word goes to bits
-// 0..7, byte m of the high word to bits 16..23; channel b's selectors pick
-// zero bytes (0x0c) in a single-channel row.
-struct pair_sel {
-    uint32_t s[4];
-};
-
-__device__ __forceinline__ pair_sel pair_sel_of(uint32_t mb)
-{
-    const uint32_t z = 0x0c0c0c0cu;
-    return pair_sel{{mb ? 0x0c040c00u : z, mb ? 0x0c050c01u : z, mb ? 0x0c060c02u : z, mb ? 0x0c070c03u : z}};
-}
-
-template <int NI>
-__device__ __forceinline__ void pair_store_item(int i, const pair_build<NI>& B, const pair_sel& S, uint4* r)
-{
-    uint4 o;
-    o.x = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c040c00u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], S.s[0]);
-    o.y = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c050c01u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], S.s[1]);
-    o.z = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c060c02u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], S.s[2]);
-    o.w = __builtin_amdgcn_perm(B.ah[i], B.al[i], 0x0c070c03u) + __builtin_amdgcn_perm(B.bh[i], B.bl[i], S.s[3]);
-    r[64 * i] = o;
-}
-
-template <int NI>
-__device__ __forceinline__ void pair_build_store(int wave, int lane, const pair_build<NI>& B, uint4* rows)
-{
-    uint4* r = rows + wave * kPRQ + lane;
-    const pair_sel S = pair_sel_of(B.mb);
-#pragma unroll
-    for (int i = 0; i < NI; ++i) pair_store_item<NI>(i, B, S, r);
-}
-
-// acc16[j] += row dword k = e + L + 64 j of the entry at dword offset o0.
-template <int R>
-__device__ __forceinline__ void pair_add1(const uint32_t* base, int o0, uint32_t (&acc)[R / 2])
-{
-    const uint32_t* r0 = base + o0;
-#pragma unroll
-    for (int j = 0; j < R / 2; ++j) acc[j] += r0[64 * j];
-}
-
-// Four rows' dwords of a lane (entries e01, e23), read ahead of their adds.
-template <int R>
-struct pair_x4 {
-    uint32_t x[4][R / 2];
-};
-
-template <int R>
-__device__ __forceinline__ void pair_read4(const uint32_t* base, uint32_t e01, uint32_t e23, pair_x4<R>& X)
-{
-    const uint32_t* r[4] = {base + (e01 & 0xffffu), base + (e01 >> 16), base + (e23 & 0xffffu), base + (e23 >> 16)};
-#pragma unroll
-    for (int j = 0; j < R / 2; ++j)
-#pragma unroll
-        for (int m = 0; m < 4; ++m) X.x[m][j] = r[m][64 * j];
-}
-
-template <int R>
-__device__ __forceinline__ void pair_acc4(const pair_x4<R>& X, uint32_t (&acc)[R / 2])
-{
-#pragma unroll
-    for (int j = 0; j < R / 2; ++j) acc[j] += X.x[0][j] + X.x[1][j];
-#pragma unroll
-    for (int j = 0; j < R / 2; ++j) acc[j] += X.x[2][j] + X.x[3][j];
-}
-
-// Entries of one DM in one chunk (16-bit dword offsets, two per dword),
-// held in scalar registers.
-struct pair_ents {
-    uint32_t e[kPMaxEnt / 2];
-    int n;
-};
-
-__device__ __forceinline__ pair_ents pair_ents_of(const pair_rec& rc, int wave)
-{
-    pair_ents E;
-    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
-#pragma unroll
-    for (int i = 0; i < kPMaxEnt / 2; ++i) E.e[i] = ew[i];
-    E.n = rc.nent;
-    return E;
-}
-
-template <int R>
-__device__ __forceinline__ void pair_flush(uint32_t (&acc16)[R / 2], uint32_t (&acc32)[R])
-{
-#pragma unroll
-    for (int j = 0; j < R / 2; ++j) {
-        acc32[j] += acc16[j] & 0xFFFFu;
-        acc32[R / 2 + j] += acc16[j] >> 16;
-        acc16[j] = 0;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Pair pre-summation with 16-byte LDS reads (chunk shapes 4 and 2).  Rows
-// and plan records are those above; this is how a DM reads a row.
-// ds_read_b128 moves about twice the bytes per clock of the dword forms on
-// this part, but only from 16-byte-aligned addresses, and a DM's row offset
-// e is arbitrary.  So lane L reads the aligned quads at A + 252 j + 4 L
-// (A = e rounded down to a multiple of 4 dwords, r = e - A, j < J) and needs
-// the dwords e + 252 j + 4 L + q (q < 4): the last 4 - r of its own quad and
-// the first r of lane L+1's, which a DPP wave shift (lane L reads lane L+1)
-// brings over inside the add.  Lane 63 reads the quad after lane 62's (the
-// first of block j+1) and owns no samples, so a tile is 2 x J x 252 samples
-// (J = 4: 2016, shape 4; J = 2: 1008, shape 2): dword k = 252 j + 4 L + q of
-// lane L < 63 holds samples t0 + k (low half) and t0 + 252 J + k (high
-// half).  r is the same for the whole wave (one DM per wave): each entry
-// branches, on the scalar unit, to the add code of its r.
-// ---------------------------------------------------------------------------
-
-// Lane L's x of lane L + 1 (lane 63 gets 0): DPP wave_shl:1.
-__device__ __forceinline__ uint32_t wide_next(uint32_t x)
-{
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x130, 0xf, 0xf, true);
-}
-
-// The lane's four quads of the row entry at dword offset e (from the lane's
-// base: row buffer + 4 * lane): D[4j + m] = dword (e & ~3) + 252 j + 4 L + m.
-template <int J>
-__device__ __forceinline__ void wide_read(const uint32_t* lane_q, uint32_t e, uint32_t (&D)[4 * J])
-{
-    const uint32_t* p = lane_q + (e & ~3u);
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-        const u32x4 v = *reinterpret_cast<const u32x4*>(p + kWBlk * j);
-        D[4 * j] = v.x; D[4 * j + 1] = v.y; D[4 * j + 2] = v.z; D[4 * j + 3] = v.w;
-    }
-}
-
-// acc += lane L+1's x, as one v_add_u32 with the DPP shift on its source:
-// the empty asm keeps the compiler from merging this add with the previous
-// one into a v_add3 (which cannot take a DPP operand).
-__device__ __forceinline__ void wide_add_next(uint32_t& acc, uint32_t x)
-{
-    asm("" : "+v"(acc));
-    acc += wide_next(x);
-}
-
-// acc[4j + q] += dword e + 252 j + 4 L + q of two entries (RA, RB: their e & 3):
-// one v_add3 where both dwords are the lane's own, else one add each.
-template <int RA, int RB, int J>
-__device__ __forceinline__ void wide_acc2(const uint32_t (&Da)[4 * J], const uint32_t (&Db)[4 * J],
-                                          uint32_t (&acc)[4 * J])
-{
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int x = 4 * j + q;
-            if (q + RA < 4 && q + RB < 4) {
-                acc[x] += Da[x + RA] + Db[x + RB];
-            } else if (q + RA < 4) {
-                acc[x] += Da[x + RA];
-                wide_add_next(acc[x], Db[x + RB - 4]);
-            } else if (q + RB < 4) {
-                acc[x] += Db[x + RB];
-                wide_add_next(acc[x], Da[x + RA - 4]);
-            } else {
-                wide_add_next(acc[x], Da[x + RA - 4]);
-                wide_add_next(acc[x], Db[x + RB - 4]);
-            }
-        }
-}
-
-// The lane's own dwords first, then the shifted ones: a DPP add reading an
-// accumulator written by the instruction just before it needs wait states.
-template <int RA, int J>
-__device__ __forceinline__ void wide_acc1(const uint32_t (&Da)[4 * J], uint32_t (&acc)[4 * J])
-{
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-        for (int q = 0; q < 4 - RA; ++q) acc[4 * j + q] += Da[4 * j + q + RA];
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-#pragma unroll
-        for (int q = 4 - RA; q < 4; ++q) wide_add_next(acc[4 * j + q], Da[4 * j + q + RA - 4]);
-}
-
-// Two entries packed in one dword (low, high 16 bits).
-template <int J>
-__device__ __forceinline__ void wide_add2(const uint32_t* lane_q, uint32_t e2, uint32_t (&acc)[4 * J])
-{
-    uint32_t Da[4 * J], Db[4 * J];
-    wide_read<J>(lane_q, e2 & 0xffffu, Da);
-    wide_read<J>(lane_q, e2 >> 16, Db);
-    switch (((e2 & 3u) << 2) | ((e2 >> 16) & 3u)) {
-    case 0: wide_acc2<0, 0, J>(Da, Db, acc); break;
-    case 1: wide_acc2<0, 1, J>(Da, Db, acc); break;
-    case 2: wide_acc2<0, 2, J>(Da, Db, acc); break;
-    case 3: wide_acc2<0, 3, J>(Da, Db, acc); break;
-    case 4: wide_acc2<1, 0, J>(Da, Db, acc); break;
-    case 5: wide_acc2<1, 1, J>(Da, Db, acc); break;
-    case 6: wide_acc2<1, 2, J>(Da, Db, acc); break;
-    case 7: wide_acc2<1, 3, J>(Da, Db, acc); break;
-    case 8: wide_acc2<2, 0, J>(Da, Db, acc); break;
-    case 9: wide_acc2<2, 1, J>(Da, Db, acc); break;
-    case 10: wide_acc2<2, 2, J>(Da, Db, acc); break;
-    case 11: wide_acc2<2, 3, J>(Da, Db, acc); break;
-    case 12: wide_acc2<3, 0, J>(Da, Db, acc); break;
-    case 13: wide_acc2<3, 1, J>(Da, Db, acc); break;
-    case 14: wide_acc2<3, 2, J>(Da, Db, acc); break;
-    default: wide_acc2<3, 3, J>(Da, Db, acc); break;
-    }
-}
-
-template <int J>
-__device__ __forceinline__ void wide_add1(const uint32_t* lane_q, uint32_t e, uint32_t (&acc)[4 * J])
-{
-    uint32_t Da[4 * J];
-    wide_read<J>(lane_q, e, Da);
-    switch (e & 3u) {
-    case 0: wide_acc1<0, J>(Da, acc); break;
-    case 1: wide_acc1<1, J>(Da, acc); break;
-    case 2: wide_acc1<2, J>(Da, acc); break;
-    default: wide_acc1<3, J>(Da, acc); break;
-    }
-}
-
-// acc += the entry whose quads are in D (r: its e & 3).
-template <int J>
-__device__ __forceinline__ void wide_acc_r(const uint32_t (&D)[4 * J], uint32_t r, uint32_t (&acc)[4 * J])
-{
-    switch (r) {
-    case 0: wide_acc1<0, J>(D, acc); break;
-    case 1: wide_acc1<1, J>(D, acc); break;
-    case 2: wide_acc1<2, J>(D, acc); break;
-    default: wide_acc1<3, J>(D, acc); break;
-    }
-}
-
-// wide_sum8 with the row build spread over the entries: after entry j's
-// adds, item kWItemAfter[j] (if any) of this wave's row of the next chunk is
-// stored (if `store`) and the same item of the row after that loaded into
-// the freed registers.  The build's VALU work and LDS writes thus fill the
-// waits on the entries' reads, and the global loads still have about a
-// chunk to arrive.  Items go after entries 1, 3, 5, 6, 7 (measured against
-// other placements: entries 0-4 4 % slower, items ahead of the adds 7 %).
-constexpr int kWItemAfter[kPairs] = {-1, 0, -1, 1, -1, 2, 3, 4};
-template <int NI, int J>
-__device__ __forceinline__ void wide_sum8_build(const uint32_t* lane_q, const uint32_t (&e)[kPMaxEnt / 2],
-                                                uint32_t (&D0)[4 * J], uint32_t (&acc)[4 * J], pair_build<NI>& B,
-                                                bool store, uint4* r, const uint8_t* ra, const uint8_t* rb,
-                                                uint32_t mb_next, int lane)
-{
-    const pair_sel S = pair_sel_of(B.mb);
-    uint32_t D1[4 * J];
-    auto item = [&](int j) {
-        const int it = kWItemAfter[j];
-        if (it >= 0 && it < NI) {
-            if (store) pair_store_item<NI>(it, B, S, r);
-            pair_load_item<NI, 2 * kWBlk * J>(it, ra, rb, lane, B);
-        }
-    };
-#pragma unroll
-    for (int i = 0; i < kPairs / 2; ++i) {
-        wide_read<J>(lane_q, e[i] >> 16, D1);
-        wide_acc_r<J>(D0, e[i] & 3u, acc);
-        item(2 * i);
-        if (i + 1 < kPairs / 2) wide_read<J>(lane_q, e[i + 1] & 0xffffu, D0);
-        wide_acc_r<J>(D1, (e[i] >> 16) & 3u, acc);
-        item(2 * i + 1);
-    }
-    B.mb = mb_next;
-}
-
-// All n entries of DM slot `wave` in chunk record rc, read from the record
-// itself (a register array indexed at run time would live in scratch memory).
-template <int J>
-__device__ __forceinline__ void wide_sum(const pair_rec& rc, int wave, int n, const uint32_t* lane_q,
-                                         uint32_t (&acc)[4 * J])
-{
-    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
-    int i = 0;
-    for (; i + 1 < n; i += 2) wide_add2<J>(lane_q, ew[i >> 1], acc);
-    if (i < n) wide_add1<J>(lane_q, ew[i >> 1] & 0xffffu, acc);
-}
-
-// Software pipeline over the chunks: step k sums chunk k, stores chunk k+1's
-// row from the registers loaded during step k-1, issues the loads of chunk
-// k+2, and ends with the chunk's one barrier; a row's loads thus have about
-// a whole step to arrive.  NI: quads per lane of a row (the range's row
-// width rounded up to 64 quads); J: 252-dword blocks per lane read.
-template <int NI, int J>
-__device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, long t0)
-{
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const bool active = dt * kDms + wave < p.ndms;
-    const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
-    const pair_rec* recs = reinterpret_cast<const pair_rec*>(p.pair_recs) + (size_t)dt * nchunks;
-    constexpr int A = 4 * J, SPLIT = kWBlk * J, T = 2 * SPLIT;
-    uint32_t acc16[A];
-    uint32_t acc32[2 * A];
-#pragma unroll
-    for (int j = 0; j < A; ++j) acc16[j] = 0;
-#pragma unroll
-    for (int j = 0; j < 2 * A; ++j) acc32[j] = 0;
-
-    pair_build<NI> B;
-    pair_build_load<NI, T>(p, t0, recs[0].row[wave], lane, B);
-    if (wave < recs[0].nrows) pair_build_store(wave, lane, B, pair_buf(0));
-    const int k1 = min(1, nchunks - 1);
-    bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
-    pair_build_load<NI, T>(p, t0, recs[k1].row[wave], lane, B);
-    int kr = min(2, nchunks - 1);
-    pair_row rn = recs[kr].row[wave];                     // row record of chunk k+2
-    int nrows_n = recs[kr].nrows;
-    pair_ents E = pair_ents_of(recs[0], wave);
-    __syncthreads();
-    const uint32_t* lane_q = reinterpret_cast<const uint32_t*>(pair_buf(0)) + 4 * lane;
-    for (int k = 0; k < nchunks; ++k) {
-        const uint32_t* base = lane_q + ((k & 1) ? kPRows * kPRowDw : 0);   // rows of chunk k
-        uint4* rows_next = pair_buf((k + 1) & 1);
-        // the first entry's reads go ahead of the row stores in the LDS queue
-        uint32_t D0[A];
-        wide_read<J>(base, E.e[0] & 0xffffu, D0);
-        const bool store = have_next;
-        have_next = k + 2 < nchunks && wave < nrows_n;
-        if (active && E.n == kPairs) {   // the usual full chunk: straight-line code
-            wide_sum8_build<NI, J>(base, E.e, D0, acc16, B, store, rows_next + wave * kPRQ + lane,
-                                p.L + t0 + rn.offa, p.L + t0 + rn.offb + rn.sh, rn.mb, lane);
-        } else {
-            if (store) pair_build_store(wave, lane, B, rows_next);
-            pair_build_load<NI, T>(p, t0, rn, lane, B);
-            if (active) wide_sum<J>(recs[k], wave, E.n, base, acc16);
-        }
-        if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) {
-#pragma unroll
-            for (int j = 0; j < A; ++j) {
-                acc32[j] += acc16[j] & 0xFFFFu;
-                acc32[A + j] += acc16[j] >> 16;
-                acc16[j] = 0;
-            }
-        }
-        kr = min(k + 3, nchunks - 1);
-        rn = recs[kr].row[wave];
-        nrows_n = recs[kr].nrows;
-        E = pair_ents_of(recs[min(k + 1, nchunks - 1)], wave);
-        __syncthreads();
-    }
-
-    if (active && lane < 63) {
-        const int d = dt * kDms + wave;
-        float* orow = p.out + (size_t)d * p.out_pitch;
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int x = 0; x < A; ++x) {
-                const long tt = t0 + h * SPLIT + kWBlk * (x >> 2) + 4 * lane + (x & 3);
-                if (tt < p.t_processed)
-                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * A + x], p.fnchans), p.fbin);
-            }
-    }
-}
-
-template <int NI, int J>
-__global__ __launch_bounds__(kThreads) void dedisperse_wide_kernel(lds_params p)
-{
-    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
-    const int per_xcd = (ntiles + 7) / 8;
-    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (w >= ntiles) return;  // whole work-group leaves before any barrier
-    process_wide_tile<NI, J>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * (2 * kWBlk * J));
 }
 
 // The XCD this work-group runs on (hardware register XCC_ID, bits 0-3), so
@@ -1502,19 +1014,7 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
         // quads per lane: the row width rounded up to whole 64-quad items
         // (a row is at least half a tile wide)
         const int ni = (p.row_quads + 63) / 64;
-        const dim3 g((unsigned)grid), b(kThreads);
-        if (variant == 4) {   // 2016-sample tiles: rows of at least 252 quads
-            if (ni <= 4) hipLaunchKernelGGL((dedisperse_wide_kernel<4, 4>), g, b, 0, s, q);
-            else hipLaunchKernelGGL((dedisperse_wide_kernel<5, 4>), g, b, 0, s, q);
-        } else if (ni <= 2) {   // 1008-sample tiles: rows of at least 126 quads
-            hipLaunchKernelGGL((dedisperse_wide_kernel<2, 2>), g, b, 0, s, q);
-        } else if (ni == 3) {
-            hipLaunchKernelGGL((dedisperse_wide_kernel<3, 2>), g, b, 0, s, q);
-        } else if (ni == 4) {
-            hipLaunchKernelGGL((dedisperse_wide_kernel<4, 2>), g, b, 0, s, q);
-        } else {
-            hipLaunchKernelGGL((dedisperse_wide_kernel<5, 2>), g, b, 0, s, q);
-        }
+        return launch_dedisperse_wide(q, variant == 4 ? 4 : 2, ni, (unsigned)grid, s);
     }
     return hipGetLastError();
 }
