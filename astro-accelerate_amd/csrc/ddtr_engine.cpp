@@ -73,7 +73,7 @@ int check_delays(const float *dmshifts, int nchans, int ndms, float mstartdm, fl
 
 lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant) {
     lds_choice c{-1, -1};
-    static const int order[kLdsVariants] = {4, 2, 3, 0, 1};
+    static const int order[kLdsVariants] = {4, 5, 2, 3, 0, 1};
     for (int v : order) {
         if (variant >= 0 && v != variant) continue;
         const int q = lds_row_quads(v, h_dmshifts, nchans, ndms, mstartdm, mdmstep);
@@ -104,12 +104,13 @@ int resolve_launch(int kernel, const uint8_t *L, size_t pitch, int nchans, const
     rl->variant = -1;
     const int forced = kernel == AA_HIP_KERNEL_LDS_C32    ? 0
                        : kernel == AA_HIP_KERNEL_LDS_C16  ? 1
-                       : kernel == AA_HIP_KERNEL_LDS_PAIR ? 4   // the wider pair tile, else shape 2
+                       : kernel == AA_HIP_KERNEL_LDS_PAIR ? 4   // the widest pair tile that fits: 4, 5, 2
                        : kernel == AA_HIP_KERNEL_LDS_QUAD ? 3
                                                           : -1;
     if (kernel != AA_HIP_KERNEL_DIRECT && (pitch % 16) == 0 && ((uintptr_t)L % 16) == 0) {
         if (forced >= 0) {
             choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, forced);
+            if (choice.variant < 0 && forced == 4) choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 5);
             if (choice.variant < 0 && forced == 4) choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 2);
         }
         else if (choice.variant == -2) choice = choose_lds(h_dmshifts, nchans, ndms, mstartdm, mdmstep, -1);

@@ -14,10 +14,11 @@ namespace aa_hip {
 constexpr int kLdsRowQuadsMax = 152;
 // Chunk shapes: 0 = 32 channels x 152 quads, 1 = 16 channels x 304 quads,
 // 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums, 1008-sample
-// tiles), 3 = 16 channels pre-summed in groups of four, 4 = shape 2 with
-// 2016-sample tiles read with 16-byte LDS reads (window offsets up to 268
-// samples).
-constexpr int kLdsVariants = 5;
+// tiles, window offsets up to 772 samples), 3 = 16 channels pre-summed in
+// groups of four, 4 = shape 2 on 2016-sample tiles (offsets up to 268),
+// 5 = shape 2 on 1512-sample tiles (offsets up to 520).  Shapes 2, 4 and 5
+// read rows with 16-byte LDS reads.
+constexpr int kLdsVariants = 6;
 
 struct lds_params {
     const uint8_t* L;        // nchans rows of u8 samples

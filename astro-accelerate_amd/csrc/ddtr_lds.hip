@@ -675,6 +675,7 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
     if (variant < 0 || variant >= kLdsVariants) return -1;
     if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT2, nullptr);
     if (variant == 4) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT4, nullptr);
+    if (variant == 5) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT3, nullptr);
     if (variant == 3) return quad_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
@@ -954,9 +955,10 @@ int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, 
         }
         return q;
     }
-    if (variant == 2 || variant == 4) {
+    if (variant == 2 || variant == 4 || variant == 5) {
         std::vector<pair_rec> recs;
-        const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, variant == 4 ? kWT4 : kWT2,
+        const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch,
+                                variant == 4 ? kWT4 : variant == 5 ? kWT3 : kWT2,
                                 bytes ? &recs : nullptr);
         if (q > 0 && bytes) {
             bytes->resize(recs.size() * sizeof(pair_rec));
@@ -981,7 +983,7 @@ int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartd
 
 long lds_tile_samples(int variant)
 {
-    return variant == 2 ? kWT2 : variant == 3 ? kQT : variant == 4 ? kWT4 : kT;
+    return variant == 2 ? kWT2 : variant == 3 ? kQT : variant == 4 ? kWT4 : variant == 5 ? kWT3 : kT;
 }
 int lds_tile_dms() { return kDms; }
 
@@ -1014,7 +1016,7 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
         // quads per lane: the row width rounded up to whole 64-quad items
         // (a row is at least half a tile wide)
         const int ni = (p.row_quads + 63) / 64;
-        return launch_dedisperse_wide(q, variant == 4 ? 4 : 2, ni, (unsigned)grid, s);
+        return launch_dedisperse_wide(q, variant == 4 ? 4 : variant == 5 ? 3 : 2, ni, (unsigned)grid, s);
     }
     return hipGetLastError();
 }

@@ -55,11 +55,12 @@ constexpr int kPRowDw = 4 * kPRQ;              // 1280 dwords per row
 constexpr int kPRows = 16;                     // rows per buffer (one per wave)
 constexpr int kPMaxEnt = 2 * kPairs;           // row entries per DM per chunk
 constexpr int kPFlushChunks = 256 / kPairChunk;   // 128 pair sums (<= 510 each) per 16-bit field
-// Tiles of the 16-byte-read kernel (below): 2 x J blocks of 252 dwords,
-// J = 4 (shape 4, 2016 samples) or 2 (shape 2, 1008 samples).
+// Tiles of the 16-byte-read kernel: 2 x J blocks of 252 dwords, J = 4
+// (shape 4, 2016 samples), 3 (shape 5, 1512) or 2 (shape 2, 1008).
 constexpr int kWBlk = 252;                     // dwords per block: 63 lanes x 4
 constexpr int kWT4 = 2 * 4 * kWBlk;            // 2016 samples per shape-4 tile
 constexpr int kWT2 = 2 * 2 * kWBlk;            // 1008 samples per shape-2 tile
+constexpr int kWT3 = 2 * 3 * kWBlk;            // 1512 samples per shape-5 tile
 // Largest window offset e of a tile of T samples: reads of a DM end at dword
 // e + T/2 - 1, or for the 16-byte-read tiles at e + T/2 + 3 (lane 63's last
 // quad), of an 1280-dword row.
@@ -229,7 +230,8 @@ __device__ __forceinline__ void pair_flush(uint32_t (&acc16)[R / 2], uint32_t (&
 }  // namespace
 
 // Launches the 16-byte-read pair kernel with J blocks per lane (4: shape 4,
-// 2: shape 2) and NI = ni quads per lane of a row, on `grid` work-groups.
+// 3: shape 5, 2: shape 2) and NI = ni quads per lane of a row, on `grid`
+// work-groups.
 hipError_t launch_dedisperse_wide(const lds_params& q, int J, int ni, unsigned grid, hipStream_t s);
 
 }  // namespace aa_hip

@@ -18,7 +18,7 @@ namespace {
 // the first r of lane L+1's, which a DPP wave shift (lane L reads lane L+1)
 // brings over inside the add.  Lane 63 reads the quad after lane 62's (the
 // first of block j+1) and owns no samples, so a tile is 2 x J x 252 samples
-// (J = 4: 2016, shape 4; J = 2: 1008, shape 2): dword k = 252 j + 4 L + q of
+// (J = 4: 2016, shape 4; J = 3: 1512, shape 5; J = 2: 1008, shape 2): dword k = 252 j + 4 L + q of
 // lane L < 63 holds samples t0 + k (low half) and t0 + 252 J + k (high
 // half).  r is the same for the whole wave (one DM per wave): each entry
 // branches, on the scalar unit, to the add code of its r.
@@ -288,6 +288,10 @@ hipError_t launch_dedisperse_wide(const lds_params& q, int J, int ni, unsigned g
     if (J == 4) {   // 2016-sample tiles: rows of at least 252 quads
         if (ni <= 4) hipLaunchKernelGGL((dedisperse_wide_kernel<4, 4>), g, b, 0, s, q);
         else hipLaunchKernelGGL((dedisperse_wide_kernel<5, 4>), g, b, 0, s, q);
+    } else if (J == 3) {   // 1512-sample tiles: rows of at least 189 quads
+        if (ni <= 3) hipLaunchKernelGGL((dedisperse_wide_kernel<3, 3>), g, b, 0, s, q);
+        else if (ni == 4) hipLaunchKernelGGL((dedisperse_wide_kernel<4, 3>), g, b, 0, s, q);
+        else hipLaunchKernelGGL((dedisperse_wide_kernel<5, 3>), g, b, 0, s, q);
     } else if (ni <= 2) {   // 1008-sample tiles: rows of at least 126 quads
         hipLaunchKernelGGL((dedisperse_wide_kernel<2, 2>), g, b, 0, s, q);
     } else if (ni == 3) {

@@ -49,8 +49,8 @@ typedef enum {
     AA_HIP_KERNEL_LDS_C32 = 3,  /* LDS kernel, 32-channel chunks (rows up to 2432 bytes) */
     AA_HIP_KERNEL_LDS_C16 = 4,  /* LDS kernel, 16-channel chunks (rows up to 4864 bytes) */
     AA_HIP_KERNEL_LDS_PAIR = 5, /* LDS kernel, 16-channel chunks pre-summed in channel pairs
-                                   (2016-sample tiles read with 16-byte LDS reads
-                                   where they fit, else 1008-sample tiles) */
+                                   (the widest tile that fits: 2016, 1512 or 1008
+                                   samples, read with 16-byte LDS reads) */
     AA_HIP_KERNEL_LDS_QUAD = 6  /* LDS kernel, 16-channel chunks pre-summed in groups of four */
 } aa_hip_kernel;
 
@@ -249,18 +249,19 @@ int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_o
 
 /* Tile tuning (no reference counterpart: the reference picks its kernel by a
  * fixed LDS-size test, src/aa_dedisperse.cpp:90-131).  The LDS kernel has
- * five channel-chunk shapes: 0 = 32 channels x 2432-byte rows, 1 = 16
+ * six channel-chunk shapes: 0 = 32 channels x 2432-byte rows, 1 = 16
  * channels x 4864-byte rows, 2 = 16 channels pre-added in channel pairs
  * (rows of 16-bit pair sums, one per distinct relative delay of the pair in
- * a 16-DM tile) on 1008-sample tiles, 3 = 16 channels pre-added in groups of
- * four, 4 = shape 2 on 2016-sample tiles read with 16-byte LDS reads (window
- * offsets up to 268 samples).
+ * a 16-DM tile) on 1008-sample tiles (window offsets up to 772 samples),
+ * 3 = 16 channels pre-added in groups of four, 4 = shape 2 on 2016-sample
+ * tiles (offsets up to 268), 5 = shape 2 on 1512-sample tiles (offsets up
+ * to 520); shapes 2, 4 and 5 read rows with 16-byte LDS reads.
  * set_range_variant records a preference for range r (-1 = first that
- * fits, in the order 4, 2, 3, 0, 1), used whenever it fits.
+ * fits, in the order 4, 5, 2, 3, 0, 1), used whenever it fits.
  * autotune_range times every shape that fits range r on `stream` (reps
  * launches each, writing d_out), keeps the fastest as the preference and
- * reports it: *best_variant (-1 if none fits) and variant_ms[0..4] (-1 =
- * does not fit).
+ * reports it: *best_variant (-1 if none fits) and variant_ms[0..5] (-1 =
+ * does not fit; the array holds 6 floats).
  * The range's level must be resident (after ingest). */
 int aa_hip_ddtr_set_range_variant(aa_hip_ddtr *h, int range, int variant);
 int aa_hip_ddtr_autotune_range(aa_hip_ddtr *h, int range, int t_processed, float *d_out,
