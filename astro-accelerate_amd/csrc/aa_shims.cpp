@@ -7,12 +7,14 @@
 // plus a library-owned device copy of the dmshift table.
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <vector>
 
 #include "ddtr_kernels.h"
 
 #include "aa_bin_gpu.hpp"
+#include "aa_ddtr_hip.h"
 #include "aa_corner_turn.hpp"
 #include "aa_zero_dm.hpp"
 #include "aa_dedisperse.hpp"
@@ -23,6 +25,16 @@
 namespace astroaccelerate {
 
 namespace {
+// Plan tables of one range's pair shapes, kept across dedisperse() calls
+// while the range's geometry is unchanged.
+struct shim_tables {
+    int nchans = 0, ndms = 0, variant = -1;
+    float mstartdm = 0.0f, mdmstep = 0.0f;
+    size_t pitch = 0;
+    void *d = nullptr;
+    int row_quads = 0;
+};
+
 struct shim_constants {
     int nchans = 0;
     long i_nsamp = 0;
@@ -31,6 +43,10 @@ struct shim_constants {
     float *d_dmshifts = nullptr;
     int d_capacity = 0;
     int device = -1;
+    uint8_t *d_u8 = nullptr;      // the u16 input narrowed to u8 rows for the LDS kernels
+    size_t u8_capacity = 0;
+    int *d_err = nullptr;         // set when a u16 sample exceeds 255
+    std::map<int, shim_tables> tables;
 };
 std::mutex g_mutex;
 shim_constants g_const;
@@ -80,7 +96,11 @@ void set_dedispersion_constants(int t_processed, int maxshift) {
 
 // src/aa_dedisperse.cpp:5-163.  The failsafe kernel's semantics for every
 // geometry: its grid covers t < 8*(t_processed/8) and d < 64*(ndms/64), the
-// output row stride is the current i_t_processed_s.
+// output row stride is the current i_t_processed_s.  The u16 rows are
+// narrowed to a u8 copy (they hold 8-bit values) and dedispersed by the LDS
+// kernels, the first chunk shape that fits, with the range's plan tables
+// cached between calls; a sample above 255 sends the call to the direct
+// kernel on the u16 rows instead.
 int dedisperse(int i, int t_processed, int *inBin, float *dmshifts, unsigned short *d_input, float *d_output,
                float *d_dm_shifts, int nchans, float *tsamp, float *dm_low, float *dm_step, int const *const ndms,
                int nbits, int failsafe) {
@@ -101,14 +121,77 @@ int dedisperse(int i, int t_processed, int *inBin, float *dmshifts, unsigned sho
     const float mdmstep = (float)(dm_step[i] / (*tsamp));
     const int t_cov = (t_processed / 8) * 8;
     const int d_cov = (ndms[i] / 64) * 64;
-    if (aa_hip::check_delays(g_const.dmshifts.data(), g_const.nchans, d_cov, mstartdm, mdmstep, t_cov,
-                             g_const.i_nsamp) != 0) {
+    const int nch = g_const.nchans;
+    const long n = g_const.i_nsamp;
+    if (aa_hip::check_delays(g_const.dmshifts.data(), nch, d_cov, mstartdm, mdmstep, t_cov, n) != 0) {
         LOG(log_level::error, std::string("dedisperse: ") + aa_hip::last_error());
         return 1;
     }
-    hipError_t e = aa_hip::launch_dedisperse_direct_u16(d_input, (size_t)g_const.i_nsamp, g_const.nchans,
-                                                        g_const.d_dmshifts, d_cov, t_cov, mstartdm, mdmstep,
-                                                        inBin[i], d_output, (size_t)g_const.t_processed, nullptr);
+    if (d_cov == 0 || t_cov == 0) return 0;
+    // u8 rows, 16-byte aligned, with 4096 bytes of slack past each row
+    const size_t pitch = ((size_t)n + 4096 + 255) / 256 * 256;
+    hipError_t e = hipSuccess;
+    if (g_const.u8_capacity < pitch * (size_t)nch) {
+        if (g_const.d_u8) (void)hipFree(g_const.d_u8);
+        g_const.d_u8 = nullptr;
+        g_const.u8_capacity = 0;
+        e = hipMalloc(&g_const.d_u8, pitch * (size_t)nch);
+        if (e == hipSuccess) g_const.u8_capacity = pitch * (size_t)nch;
+    }
+    if (e == hipSuccess && !g_const.d_err) e = hipMalloc(&g_const.d_err, sizeof(int));
+    int err = 1;
+    if (e == hipSuccess) e = hipMemsetAsync(g_const.d_err, 0, sizeof(int), nullptr);
+    if (e == hipSuccess) e = aa_hip::launch_narrow_u16(d_input, n, nch, n, g_const.d_u8, pitch, g_const.d_err, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(&err, g_const.d_err, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        LOG(log_level::error, std::string("dedisperse: ") + hipGetErrorString(e));
+        return 1;
+    }
+    if (err) {   // u16 values above 8 bits: the direct kernel sums them as they are
+        e = aa_hip::launch_dedisperse_direct_u16(d_input, (size_t)n, nch, g_const.d_dmshifts, d_cov, t_cov, mstartdm,
+                                                 mdmstep, inBin[i], d_output, (size_t)g_const.t_processed, nullptr);
+        if (e == hipSuccess) e = hipGetLastError();
+        return e == hipSuccess ? 0 : 1;
+    }
+    aa_hip::range_launch rl;
+    if (aa_hip::resolve_launch(AA_HIP_KERNEL_AUTO, g_const.d_u8, pitch, nch, g_const.d_dmshifts,
+                               g_const.dmshifts.data(), d_cov, t_cov, mstartdm, mdmstep, inBin[i], d_output,
+                               (size_t)g_const.t_processed, aa_hip::lds_choice{-2, 0}, &rl) != AA_HIP_SUCCESS) {
+        LOG(log_level::error, std::string("dedisperse: ") + aa_hip::last_error());
+        return 1;
+    }
+    if (rl.variant >= 2) {
+        shim_tables &tb = g_const.tables[i];
+        if (!tb.d || tb.nchans != nch || tb.ndms != d_cov || tb.variant != rl.variant || tb.pitch != pitch ||
+            tb.mstartdm != mstartdm || tb.mdmstep != mdmstep) {
+            if (tb.d) (void)hipFree(tb.d);
+            tb = shim_tables{};
+            std::vector<uint8_t> bytes;
+            const int q = aa_hip::lds_plan_tables(rl.variant, g_const.dmshifts.data(), nch, d_cov, mstartdm, mdmstep,
+                                                  pitch, &bytes);
+            if (q != rl.row_quads) {
+                LOG(log_level::error, "dedisperse: plan tables disagree with the chosen row width");
+                return 1;
+            }
+            e = hipMalloc(&tb.d, bytes.size());
+            if (e == hipSuccess) e = hipMemcpy(tb.d, bytes.data(), bytes.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                if (tb.d) (void)hipFree(tb.d);
+                tb = shim_tables{};
+                LOG(log_level::error, std::string("dedisperse: plan tables: ") + hipGetErrorString(e));
+                return 1;
+            }
+            tb.nchans = nch;
+            tb.ndms = d_cov;
+            tb.variant = rl.variant;
+            tb.pitch = pitch;
+            tb.mstartdm = mstartdm;
+            tb.mdmstep = mdmstep;
+            tb.row_quads = q;
+        }
+        rl.pair_recs = tb.d;
+    }
+    e = aa_hip::launch_range(rl, nullptr);
     if (e == hipSuccess) e = hipGetLastError();
     return e == hipSuccess ? 0 : 1;
 }

@@ -36,6 +36,10 @@ hipError_t launch_transpose_u16(const uint16_t* in, int rows, int cols,
                                 uint16_t* out, hipStream_t s);
 
 // out[c][t] = (in[c][2t] + in[c][2t+1]) >> 1 for t < n_out.
+// u16 rows (in_pitch elements apart) of 8-bit values -> u8 rows (out_pitch
+// bytes apart), n samples each; a value above 255 sets *d_err (not cleared).
+hipError_t launch_narrow_u16(const uint16_t* in, long in_pitch, int rows, long n, uint8_t* out, size_t out_pitch,
+                             int* d_err, hipStream_t s);
 // Zero-DM filter in place on [t][c] samples (bp: nchans device floats;
 // values clamped to 2^nbits - 1).
 hipError_t launch_zero_dm_u8(uint8_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s);

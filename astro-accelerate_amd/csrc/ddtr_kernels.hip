@@ -228,6 +228,20 @@ inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 // each sample is read twice (sum, then filter; the second read mostly hits
 // L2) and written once.
 // ---------------------------------------------------------------------------
+// u16 rows holding 8-bit values -> u8 rows (the reference's dedisperse()
+// input layout to the LDS kernels'); a value above 255 sets *err.
+__global__ __launch_bounds__(256) void narrow_u16_rows_kernel(const uint16_t* __restrict__ in, long in_pitch,
+                                                              long n, uint8_t* __restrict__ out, size_t out_pitch,
+                                                              int* __restrict__ err)
+{
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    const long c = blockIdx.y;
+    if (t >= n) return;
+    const uint16_t v = in[c * in_pitch + t];
+    out[c * (long)out_pitch + t] = (uint8_t)v;
+    if (v > 255) atomicOr(err, 1);
+}
+
 __device__ __forceinline__ uint32_t zero_dm_wave_sum(uint32_t v)
 {
 #pragma unroll
@@ -353,6 +367,17 @@ hipError_t launch_transpose_u16(const uint16_t* in, int rows, int cols, uint16_t
     if (rows == 0) return hipSuccess;
     dim3 grid(ceil_div(rows, kTile), ceil_div(cols, kTile));
     hipLaunchKernelGGL(transpose_u16_kernel, grid, dim3(256), 0, s, in, rows, cols, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_narrow_u16(const uint16_t* in, long in_pitch, int rows, long n, uint8_t* out, size_t out_pitch,
+                             int* d_err, hipStream_t s)
+{
+    if (rows <= 0 || n < 0 || in_pitch < n || (long)out_pitch < n || !in || !out || !d_err)
+        return hipErrorInvalidValue;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(narrow_u16_rows_kernel, dim3(ceil_div(n, 256), rows), dim3(256), 0, s, in, in_pitch, n, out,
+                       out_pitch, d_err);
     return hipGetLastError();
 }
 
