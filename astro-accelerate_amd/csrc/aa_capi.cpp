@@ -1,6 +1,7 @@
 // extern "C" surface of libaa_ddtr_hip.so (declared in include/aa_ddtr_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -83,6 +84,25 @@ int from_hip(hipError_t e, const char *what) {
 }
 
 }  // namespace
+
+template <class T>
+static int input_norm(T *d, long nsamp, int nchans, int nbits, int max_bits, void *stream) {
+    if (nsamp < 0 || nchans <= 0 || nbits < 1 || nbits > max_bits) return bad("invalid geometry or nbits");
+    if (nsamp == 0) return AA_HIP_SUCCESS;
+    if (!d) return bad("NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    void *scratch = nullptr;
+    hipError_t e = hipMalloc(&scratch, aa_hip::normalize_columns_scratch_bytes(nchans));
+    if (e != hipSuccess) return from_hip(e, "input DDTR normalisation scratch");
+    const float factor = (float)((std::pow(2.0, nbits) - 1.0) / 2.0);
+    if constexpr (sizeof(T) == 1)
+        e = aa_hip::launch_normalize_columns_u8(d, nsamp, nchans, factor, nbits, scratch, s);
+    else
+        e = aa_hip::launch_normalize_columns_u16(d, nsamp, nchans, factor, nbits, scratch, s);
+    const hipError_t w = hipStreamSynchronize(s);   // the scratch is freed below
+    const hipError_t f = hipFree(scratch);
+    return from_hip(e != hipSuccess ? e : w != hipSuccess ? w : f, "input DDTR normalisation");
+}
 
 extern "C" {
 
@@ -376,6 +396,32 @@ int aa_hip_zero_dm_u16(uint16_t *d_data_tc, int nsamp, int nchans, const float *
     if (!d_data_tc || !d_bandpass) return bad("NULL buffer");
     return from_hip(aa_hip::launch_zero_dm_u16(d_data_tc, nsamp, nchans, d_bandpass, nbits, (hipStream_t)stream),
                     "zero-DM");
+}
+
+int aa_hip_input_ddtr_normalization_u8(uint8_t *d_data_tc, long nsamp, int nchans, int nbits, void *stream) {
+    return input_norm(d_data_tc, nsamp, nchans, nbits, 8, stream);
+}
+
+int aa_hip_input_ddtr_normalization_u16(uint16_t *d_data_tc, long nsamp, int nchans, int nbits, void *stream) {
+    return input_norm(d_data_tc, nsamp, nchans, nbits, 16, stream);
+}
+
+int aa_hip_output_ddtr_normalization(float *d_plane, int ndms, long nsamp, void *stream) {
+    if (ndms < 0 || nsamp < 0) return bad("invalid geometry");
+    if (ndms == 0 || nsamp == 0) return AA_HIP_SUCCESS;
+    if (!d_plane) return bad("NULL buffer");
+    return from_hip(aa_hip::launch_normalize_columns_f32(d_plane, ndms, nsamp, 127.5f, (hipStream_t)stream),
+                    "output DDTR normalisation");
+}
+
+int aa_hip_bandpass_average(const void *h_in_tc, int bytes_per_sample, size_t nchans, size_t nsamp,
+                            float *h_bandpass) {
+    if (bytes_per_sample != 1 && bytes_per_sample != 2) return bad("bytes_per_sample must be 1 or 2");
+    if (nchans == 0) return AA_HIP_SUCCESS;
+    if (nsamp == 0) return bad("nsamp must be positive");
+    if (!h_in_tc || !h_bandpass) return bad("NULL buffer");
+    aa_hip::bandpass_average(h_in_tc, bytes_per_sample, nchans, nsamp, h_bandpass);
+    return AA_HIP_SUCCESS;
 }
 
 int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nchans, const float *h_dmshifts,

@@ -10,15 +10,50 @@
 // that returns false with status finished.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <thread>
+#include <vector>
 
 #include "aa_ddtr_hip.h"
 #include "aa_log.hpp"
 #include "aa_pipeline_api.hpp"
 #include "ddtr_engine.hpp"
 #include "ddtr_kernels.h"
+
+namespace aa_hip {
+
+// set_bandpass_average (src/aa_host_cpu_msd.cpp:7-39, called at
+// include/aa_permitted_pipelines_generic.hpp:533-551 with the [t][c] host
+// input): out[i] = (float)(sum / nsamp) over input[i*nsamp .. i*nsamp +
+// nsamp) for i < nchans, a double sum of integers (exact, so integer sums
+// give the same float).  The runs are nsamp consecutive samples of the [t][c]
+// buffer, as the reference indexes it, not channel i.
+void bandpass_average(const void *in_tc, int bytes_per_sample, size_t nchans, size_t nsamp, float *out) {
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned k = 0; k < nt; ++k) {
+        pool.emplace_back([=] {
+            for (size_t i = k; i < nchans; i += nt) {
+                uint64_t sum = 0;
+                if (bytes_per_sample == 1) {
+                    const uint8_t *p = (const uint8_t *)in_tc + i * nsamp;
+                    for (size_t j = 0; j < nsamp; ++j) sum += p[j];
+                } else {
+                    const uint16_t *p = (const uint16_t *)in_tc + i * nsamp;
+                    for (size_t j = 0; j < nsamp; ++j) sum += p[j];
+                }
+                out[i] = (float)((double)sum / (double)nsamp);
+            }
+        });
+    }
+    for (auto &t : pool) t.join();
+}
+
+}  // namespace aa_hip
 
 namespace astroaccelerate {
 
@@ -56,6 +91,8 @@ public:
 
     aa_ddtr_strategy ddtr_strategy() { return m_bound ? m_strategy : aa_ddtr_strategy(); }
 
+    bool has(aa_pipeline::component_option o) const { return m_options.find(o) != m_options.end(); }
+
     bool ready() {
         if (m_ready) return true;
         for (auto c : m_components) {
@@ -73,15 +110,16 @@ public:
         for (auto o : m_options) {
             // Options that change the dedispersed values are refused; options
             // that only concern output files or other components are ignored.
-            const bool changes_values = o == opt::zero_dm_with_outliers || o == opt::input_DDTR_normalization ||
-                                        o == opt::output_DDTR_normalization || o == opt::old_rfi;
+            const bool changes_values = o == opt::zero_dm_with_outliers || o == opt::old_rfi;
             if (changes_values) {
                 LOG(log_level::error, "pipeline option " + std::to_string((int)o) +
-                                          " (zero-DM / normalisation / RFI) is not supported by this library.");
+                                          " (zero-DM with outliers / RFI) is not supported by this library.");
                 aa_hip::set_last_error("unsupported pipeline option");
                 return false;
             }
-            if (o != opt::copy_ddtr_data_to_host && o != opt::empty && o != opt::zero_dm)
+            if (o != opt::copy_ddtr_data_to_host && o != opt::empty && o != opt::zero_dm &&
+                o != opt::input_DDTR_normalization && o != opt::output_DDTR_normalization &&
+                o != opt::set_bandpass_average)
                 LOG(log_level::notice, "pipeline option " + std::to_string((int)o) + " has no effect here (ignored).");
         }
         const int nb = m_metadata.nbits();
@@ -213,7 +251,13 @@ private:
         if (he == hipSuccess) he = hipMalloc(&m_d_output, std::max<size_t>(max_out, 1) * sizeof(float));
         m_zero_dm = m_options.find(aa_pipeline::component_option::zero_dm) != m_options.end();
         if (he == hipSuccess && m_zero_dm) {   // the strategy's bandpass normalisation, one float per channel
-            const std::vector<float> bp = m_strategy.bandpass_normalization();
+            std::vector<float> bp = m_strategy.bandpass_normalization();
+            // set_bandpass_average replaces it before the first chunk
+            // (include/aa_permitted_pipelines_generic.hpp:533-551).
+            if (has(aa_pipeline::component_option::set_bandpass_average)) {
+                bp.assign((size_t)nchans, 0.0f);
+                aa_hip::bandpass_average(m_input, m_bytes, (size_t)nchans, (size_t)m_metadata.nsamples(), bp.data());
+            }
             if (bp.size() != (size_t)nchans) {
                 LOG(log_level::error, "zero-DM: the bandpass normalisation does not have nchans values");
                 aa_hip::set_last_error("zero-DM: bandpass normalisation size");
@@ -225,6 +269,10 @@ private:
             if (he == hipSuccess)
                 he = hipMemcpy(m_d_bandpass, bp.data(), bp.size() * sizeof(float), hipMemcpyHostToDevice);
         }
+        m_in_norm = has(aa_pipeline::component_option::input_DDTR_normalization);
+        m_out_norm = has(aa_pipeline::component_option::output_DDTR_normalization);
+        if (he == hipSuccess && m_in_norm)
+            he = hipMalloc(&m_d_norm_scratch, aa_hip::normalize_columns_scratch_bytes(nchans));
         if (he == hipSuccess) he = hipStreamCreate(&m_stream);
         if (he == hipSuccess) he = hipStreamCreate(&m_copy_stream);
         if (he == hipSuccess) he = hipEventCreateWithFlags(&m_ev_ingested, hipEventDisableTiming);
@@ -304,6 +352,8 @@ private:
         if (m_d_output) (void)hipFree(m_d_output);
         if (m_d_bandpass) (void)hipFree(m_d_bandpass);
         m_d_bandpass = nullptr;
+        if (m_d_norm_scratch) (void)hipFree(m_d_norm_scratch);
+        m_d_norm_scratch = nullptr;
         m_copy_stream = nullptr;
         m_ev_ingested = nullptr;
         m_ev_copied = nullptr;
@@ -351,6 +401,19 @@ private:
                     return false;
                 }
             }
+            if (m_in_norm) {   // after zero-DM, before the corner turn (same rows)
+                const long rows = (long)tp0 + m_strategy.maxshift();
+                const int nb = m_metadata.nbits();
+                const float factor = (float)((std::pow(2.0, nb) - 1.0) / 2.0);
+                he = m_bytes == 1 ? aa_hip::launch_normalize_columns_u8((uint8_t *)m_d_input, rows, nchans, factor,
+                                                                        nb, m_d_norm_scratch, m_stream)
+                                  : aa_hip::launch_normalize_columns_u16((uint16_t *)m_d_input, rows, nchans,
+                                                                         factor, nb, m_d_norm_scratch, m_stream);
+                if (he != hipSuccess) {
+                    LOG(log_level::error, std::string("input DDTR normalisation: ") + hipGetErrorString(he));
+                    return false;
+                }
+            }
             int st = m_engine->ingest(m_d_input, m_bytes == 1 ? 8 : 16, tp0, m_stream);
             if (st != AA_HIP_SUCCESS) {
                 LOG(log_level::error, std::string("corner turn: ") + aa_hip::last_error());
@@ -388,6 +451,14 @@ private:
         if (st != AA_HIP_SUCCESS) {
             LOG(log_level::error, std::string("dedispersion: ") + aa_hip::last_error());
             return false;
+        }
+        if (m_out_norm) {   // each time sample's mean over the range's DMs moved to 127.5
+            const hipError_t he = aa_hip::launch_normalize_columns_f32(
+                m_d_output, m_strategy.ndms((size_t)range), tp, 127.5f, m_stream);
+            if (he != hipSuccess) {
+                LOG(log_level::error, std::string("output DDTR normalisation: ") + hipGetErrorString(he));
+                return false;
+            }
         }
         if (m_host_out) {
             const int ib = m_strategy.dm((size_t)range).inBin;
@@ -439,6 +510,9 @@ private:
     int m_prefetched_chunk = -1;
     bool m_zero_dm = false;                // zero-DM filter before each chunk's corner turn
     float *m_d_bandpass = nullptr;         // its per-channel normalisation
+    bool m_in_norm = false;                // input DDTR normalisation of each chunk
+    bool m_out_norm = false;               // output DDTR normalisation of each plane
+    void *m_d_norm_scratch = nullptr;      // input normalisation's column sums
     float ***m_host_out = nullptr;
     std::vector<float *> m_host_blocks;
     std::vector<size_t> m_host_total;

@@ -7,6 +7,7 @@
 // plus a library-owned device copy of the dmshift table.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <vector>
@@ -236,6 +237,37 @@ void zero_dm(unsigned short *const d_input, const int nchans, const int nsamp, c
     if (hipMemcpy(d_bp, bp.data(), bp.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess)
         zero_dm(d_input, nchans, nsamp, nbits, d_bp);
     (void)hipFree(d_bp);
+}
+
+// zero_dm_normalization_dm (src/aa_zero_dm.cpp:49-60): normalises a u16
+// array of `second` rows of `first` values per column, factor
+// (2^nbits - 1)/2.  The reference's runner passes (nchans, nTimesamples) in
+// that order (include/aa_permitted_pipelines_generic.hpp:734), so each
+// channel of the [t][c] chunk is normalised over its time samples.
+void zero_dm_normalization_dm(unsigned short *d_input, size_t first, size_t second, int nbits)
+{
+    if (nbits < 1 || nbits > 16 || first == 0) {
+        LOG(log_level::error, "zero_dm_normalization_dm: invalid size or nbits");
+        return;
+    }
+    void *scratch = nullptr;
+    if (hipMalloc(&scratch, aa_hip::normalize_columns_scratch_bytes((long)first)) != hipSuccess) scratch = nullptr;
+    const float factor = (float)((std::pow(2.0, nbits) - 1.0) / 2.0);
+    const hipError_t e =
+        aa_hip::launch_normalize_columns_u16(d_input, (long)second, (long)first, factor, nbits, scratch, 0);
+    if (e != hipSuccess) LOG(log_level::error, std::string("zero_dm_normalization_dm: ") + hipGetErrorString(e));
+    (void)hipDeviceSynchronize();
+    if (scratch) (void)hipFree(scratch);
+}
+
+// post_DDTR_normalization (src/aa_zero_dm.cpp:62-74): each time sample of a
+// [nDMs][nTimesamples] float plane moved by (its mean over DMs - 127.5).
+void post_DDTR_normalization(float *d_input, size_t nTimesamples, size_t nDMs)
+{
+    if (nTimesamples == 0) return;
+    const hipError_t e = aa_hip::launch_normalize_columns_f32(d_input, (long)nDMs, (long)nTimesamples, 127.5f, 0);
+    if (e != hipSuccess) LOG(log_level::error, std::string("post_DDTR_normalization: ") + hipGetErrorString(e));
+    (void)hipDeviceSynchronize();
 }
 
 }  // namespace astroaccelerate

@@ -44,6 +44,22 @@ hipError_t launch_narrow_u16(const uint16_t* in, long in_pitch, int rows, long n
 // values clamped to 2^nbits - 1).
 hipError_t launch_zero_dm_u8(uint8_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s);
 hipError_t launch_zero_dm_u16(uint16_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s);
+// Column normalisation of a [rows][cols] array in place (the reference's
+// DDTR input/output normalisation kernel): per column s = (float sum of the
+// column in the reference's order) / rows - factor, x' = x - s (integers
+// clamped to [0, 2^nbits - 1], truncated).  `scratch` (may be null) holds
+// normalize_columns_scratch_bytes(cols) device bytes; with it, tall integer
+// arrays whose sums are exact take the row-segmented path.
+size_t normalize_columns_scratch_bytes(long cols);
+// Host: the set_bandpass_average bandpass of a [t][c] input of u8
+// (bytes_per_sample 1) or u16 samples: out[i] = mean of samples
+// i*nsamp .. i*nsamp + nsamp - 1, i < nchans (the reference's indexing).
+void bandpass_average(const void* in_tc, int bytes_per_sample, size_t nchans, size_t nsamp, float* out);
+hipError_t launch_normalize_columns_u8(uint8_t* data, long rows, long cols, float factor, int nbits, void* scratch,
+                                       hipStream_t s);
+hipError_t launch_normalize_columns_u16(uint16_t* data, long rows, long cols, float factor, int nbits,
+                                        void* scratch, hipStream_t s);
+hipError_t launch_normalize_columns_f32(float* data, long rows, long cols, float factor, hipStream_t s);
 hipError_t launch_bin2_u8(const uint8_t* in, size_t in_pitch, int nchans, int n_out,
                           uint8_t* out, size_t out_pitch, hipStream_t s);
 // Reference bin_gpu semantics on u16 rows of length nsamp packed into rows

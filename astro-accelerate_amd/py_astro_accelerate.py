@@ -182,6 +182,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_frac_adjoint": ([vp, vp, vp, i, vp, vp, vp, vp], i),
         "aa_hip_zero_dm_u8": ([vp, i, i, vp, i, vp], i),
         "aa_hip_zero_dm_u16": ([vp, i, i, vp, i, vp], i),
+        "aa_hip_input_ddtr_normalization_u8": ([vp, ctypes.c_long, i, i, vp], i),
+        "aa_hip_input_ddtr_normalization_u16": ([vp, ctypes.c_long, i, i, vp], i),
+        "aa_hip_output_ddtr_normalization": ([vp, i, ctypes.c_long, vp], i),
+        "aa_hip_bandpass_average": ([vp, i, ctypes.c_size_t, ctypes.c_size_t, vp], i),
         "aa_hip_sigproc_open": ([ctypes.c_char_p, ctypes.POINTER(vp), ctypes.POINTER(filterbank_header)], i),
         "aa_hip_sigproc_read_u8": ([vp, i, ctypes.POINTER(ctypes.POINTER(ctypes.c_ubyte)), ctypes.POINTER(sz)], i),
         "aa_hip_sigproc_read_u16": ([vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_ushort)), ctypes.POINTER(sz)], i),
@@ -792,6 +796,40 @@ def zero_dm(d_data_tc, d_bandpass, nbits: int = 8, stream=None):
     nsamp, nchans = d_data_tc.shape
     fn = _lib_or_raise().aa_hip_zero_dm_u8 if d_data_tc.element_size() == 1 else _lib_or_raise().aa_hip_zero_dm_u16
     check(fn(_ptr(d_data_tc), int(nsamp), int(nchans), _ptr(d_bandpass), int(nbits), _stream_ptr(stream)), "zero_dm")
+
+
+def input_ddtr_normalization(d_data_tc, nbits: int = 8, stream=None):
+    """Input DDTR normalisation in place on a [nsamp, nchans] uint8 or uint16
+    device tensor: each channel's mean over the nsamp samples moved to
+    (2^nbits - 1)/2 (the reference's zero_dm_normalization_dm)."""
+    nsamp, nchans = d_data_tc.shape
+    L = _lib_or_raise()
+    fn = L.aa_hip_input_ddtr_normalization_u8 if d_data_tc.element_size() == 1 else \
+        L.aa_hip_input_ddtr_normalization_u16
+    check(fn(_ptr(d_data_tc), int(nsamp), int(nchans), int(nbits), _stream_ptr(stream)), "input_ddtr_normalization")
+
+
+def output_ddtr_normalization(d_plane, stream=None):
+    """Output DDTR normalisation in place on a [ndms, nsamp] float32 device
+    tensor: each time sample's mean over DMs moved to 127.5 (the reference's
+    post_DDTR_normalization)."""
+    ndms, nsamp = d_plane.shape
+    check(_lib_or_raise().aa_hip_output_ddtr_normalization(_ptr(d_plane), int(ndms), int(nsamp),
+                                                           _stream_ptr(stream)), "output_ddtr_normalization")
+
+
+def bandpass_average(data_tc: np.ndarray) -> np.ndarray:
+    """The set_bandpass_average bandpass of a [nsamp, nchans] uint8/uint16
+    host array: value i is the mean of the i-th run of nsamp consecutive
+    samples of the [t][c] buffer, as the reference computes it."""
+    x = np.ascontiguousarray(data_tc)
+    if x.dtype not in (np.uint8, np.uint16):
+        raise TypeError("bandpass_average takes uint8 or uint16 samples")
+    nsamp, nchans = x.shape
+    out = np.zeros(nchans, dtype=np.float32)
+    check(_lib_or_raise().aa_hip_bandpass_average(x.ctypes.data, x.itemsize, nchans, nsamp, out.ctypes.data),
+          "bandpass_average")
+    return out
 
 
 def synth_noise_u8(seed: int, d_out, mean: float = 128.0, sigma: float = 16.0, stream=None):

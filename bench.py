@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--zero-dm", action="store_true",
                     help="--e2e: with the pipeline's zero_dm option; alone: time the zero-DM kernel on the "
                          "config's [t][c] input (HBM roofline line, not the contract line)")
+    ap.add_argument("--normalization", action="store_true",
+                    help="time the DDTR input normalisation on the config's [t][c] input and the output "
+                         "normalisation on its largest range's plane (HBM roofline lines, not the contract line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -96,6 +99,8 @@ def main():
         return bench_e2e(args, CONFIGS[args.config if args.config in CONFIGS else 2])
     if args.zero_dm:
         return bench_zero_dm(args, CONFIGS[args.config if args.config in CONFIGS else 2])
+    if args.normalization:
+        return bench_normalization(args, CONFIGS[args.config if args.config in CONFIGS else 2])
     if args.config == 3:
         return bench_frac(args)
     if args.config == 4:
@@ -339,6 +344,54 @@ def bench_zero_dm(args, cfg):
                       "ms_per_pass": ms, "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
                                                       "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS},
                       "config": {"workload": cfg["name"], "nchans": nchans, "nsamp": nsamp}}), flush=True)
+
+
+def bench_normalization(args, cfg):
+    """DDTR input normalisation on the config's [t][c] u8 input (per channel
+    over all samples) and output normalisation on a [1536][890624] float
+    plane (range 0 of the 8-range plan; per time sample over DMs), in place,
+    timed with HIP events on their stream.  HBM-bound: algorithmic bytes =
+    the data read once and written once (the kernels read twice: sums, then
+    the shift).  The input call includes its scratch allocation and its
+    return after the stream has finished."""
+    import torch
+    import py_astro_accelerate as aa
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.Stream()
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    nchans, nsamp = cfg["nchans"], cfg["nsamp"]
+    d = torch.empty((nsamp, nchans), dtype=torch.uint8, device=dev)
+    aa.synth_noise_u8(SEED, d, 128.0, 16.0)
+    ms_in = timed(lambda: aa.input_ddtr_normalization(d, nbits=8, stream=stream))
+    del d
+    ndms, tp = 1536, 890624
+    plane = torch.empty((ndms, tp), dtype=torch.float32, device=dev)
+    with torch.cuda.stream(stream):
+        plane.normal_(127.0, 10.0)
+    ms_out = timed(lambda: aa.output_ddtr_normalization(plane, stream=stream))
+    for name, ms, nbytes, shape in (("input", ms_in, 2.0 * nchans * nsamp, f"{nsamp} x {nchans} u8"),
+                                    ("output", ms_out, 2.0 * 4 * ndms * tp, f"{ndms} x {tp} f32")):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        log(f"[normalization] {name} {shape}: {ms:.3f} ms per pass, {gbs:.0f} GB/s "
+            f"({gbs / HBM_PEAK_GBS * 100:.1f}% of {HBM_PEAK_GBS:.0f})")
+        print(json.dumps({"metric": f"DDTR {name} normalisation throughput", "value": gbs,
+                          "unit": "GB/s (data read + written once)", "ms_per_pass": ms,
+                          "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": gbs / HBM_PEAK_GBS},
+                          "config": {"workload": cfg["name"], "shape": shape}}), flush=True)
 
 
 def bench_frac(args):

@@ -407,6 +407,34 @@ int aa_hip_zero_dm_u8(uint8_t *d_data_tc, int nsamp, int nchans, const float *d_
 int aa_hip_zero_dm_u16(uint16_t *d_data_tc, int nsamp, int nchans, const float *d_bandpass, int nbits,
                        void *stream);
 
+/* Input DDTR normalisation, in place on a [t][c] chunk before its corner
+ * turn.  Replaces zero_dm_normalization_dm(d_input, nchans, nsamp, nbits)
+ * as the reference's runner calls it (include/aa_zero_dm.hpp:26,
+ * src/aa_zero_dm.cpp:49-60, src/aa_device_zero_dm_kernel.cu:100-164,
+ * include/aa_permitted_pipelines_generic.hpp:731-735): per channel,
+ * s = (float sum over the chunk's nsamp samples, in the reference's order) /
+ * nsamp - (2^nbits - 1)/2; x' = x - s in float, clamped to [0, 2^nbits - 1],
+ * truncated.  u8: nbits 1-8; u16: 1-16.  Allocates 33 * nchans * 4 bytes
+ * of device scratch and returns after the stream has finished the work. */
+int aa_hip_input_ddtr_normalization_u8(uint8_t *d_data_tc, long nsamp, int nchans, int nbits, void *stream);
+int aa_hip_input_ddtr_normalization_u16(uint16_t *d_data_tc, long nsamp, int nchans, int nbits, void *stream);
+
+/* Output DDTR normalisation, in place on one range's dense [ndms][nsamp]
+ * float plane.  Replaces post_DDTR_normalization(d_output, t_processed,
+ * ndms) (include/aa_zero_dm.hpp:25, src/aa_zero_dm.cpp:62-74,
+ * include/aa_permitted_pipelines_generic.hpp:823-831): per time sample,
+ * s = (float sum over the DMs in the reference's order) / ndms - 127.5;
+ * x' = x - s. */
+int aa_hip_output_ddtr_normalization(float *d_plane, int ndms, long nsamp, void *stream);
+
+/* The set_bandpass_average option's bandpass, on the host
+ * (src/aa_host_cpu_msd.cpp:7-39, include/aa_permitted_pipelines_generic.hpp:
+ * 533-551): h_bandpass[i] = mean of h_in_tc[i*nsamp .. i*nsamp + nsamp - 1]
+ * for i < nchans, over the [t][c] input of nsamp x nchans samples of
+ * bytes_per_sample 1 (u8) or 2 (u16), indexed as the reference indexes it. */
+int aa_hip_bandpass_average(const void *h_in_tc, int bytes_per_sample, size_t nchans, size_t nsamp,
+                            float *h_bandpass);
+
 /* SIGPROC filterbank input.  Replaces aa_sigproc_input::read_metadata /
  * read_signal (include/aa_sigproc_input.hpp:31-47, src/aa_sigproc_input.cpp:52-80,
  * 85-541) for callers over the C ABI; the C++ class of the same name is in

@@ -317,6 +317,106 @@ void oracle_zero_dm_u16(uint16_t *data_tc, int nsamp, int nchans, const float *b
     }
 }
 
+/* Column normalisation of a [rows][cols] array in place, the reference's
+ * normalization_transposed_kernel (src/aa_device_zero_dm_kernel.cu:100-164):
+ * for each column, 32 float partial sums, partial y adding rows y, y+32,
+ * y+64, ... in row order from 0.0f; then p[y] += p[y+i] for i = 16, 8, 4,
+ * 2, 1 (y < i); s = p[0] / (float)rows - factor; every value becomes
+ * x - s in float.  Integer data (nbits 1..16) is clamped to [0, 2^nbits - 1]
+ * and truncated; float data (nbits 32) keeps x - s.  Input DDTR
+ * normalisation runs it on the [t][c] chunk with cols = nchans and factor
+ * (2^nbits - 1)/2 (src/aa_zero_dm.cpp:49-60, called at
+ * include/aa_permitted_pipelines_generic.hpp:731-735 with the size
+ * arguments in that order); output DDTR normalisation on each [dm][t]
+ * plane with cols = t_processed, factor 127.5 (src/aa_zero_dm.cpp:62-74,
+ * include/aa_permitted_pipelines_generic.hpp:823-831). */
+#define NORM_CB 64   /* columns per block: rows are walked in order, 64 columns at a time */
+
+static float load_kind(const void *data, int kind, size_t i)
+{
+    return kind == 0 ? (float)((const uint8_t *)data)[i]
+         : kind == 1 ? (float)((const uint16_t *)data)[i] : ((const float *)data)[i];
+}
+
+/* s[k] for columns c0 .. c0 + nc - 1.  Partial y of a column adds the rows
+ * r = y (mod 32) in increasing r, starting from 0.0f. */
+static void column_offsets(const void *data, int kind, size_t rows, size_t cols, size_t c0, size_t nc,
+                           float factor, float *s)
+{
+    float p[32][NORM_CB];
+    for (int y = 0; y < 32; y++)
+        for (size_t k = 0; k < nc; k++) p[y][k] = 0.0f;
+    for (size_t r = 0; r < rows; r++)
+        for (size_t k = 0; k < nc; k++) p[r % 32][k] = p[r % 32][k] + load_kind(data, kind, r * cols + c0 + k);
+    for (size_t k = 0; k < nc; k++) {
+        for (int i = 16; i > 0; i >>= 1)
+            for (int y = 0; y < i; y++) p[y][k] = p[y][k] + p[y + i][k];
+        s[k] = p[0][k] / (float)rows - factor;
+    }
+}
+
+static float norm_clamp(float r, int nbits)
+{
+    const float mx = (float)((1u << (nbits > 16 ? 16 : nbits)) - 1u);
+    if (r < 0.0f) return 0.0f;
+    if (r > mx) return mx;
+    return (float)(unsigned)r;
+}
+
+static void normalize_columns(void *data, int kind, size_t rows, size_t cols, float factor, int nbits)
+{
+    if (rows == 0) return;
+    const long nblk = (long)((cols + NORM_CB - 1) / NORM_CB);
+#pragma omp parallel for schedule(dynamic)
+    for (long b = 0; b < nblk; b++) {
+        const size_t c0 = (size_t)b * NORM_CB;
+        const size_t nc = cols - c0 < NORM_CB ? cols - c0 : NORM_CB;
+        float s[NORM_CB];
+        column_offsets(data, kind, rows, cols, c0, nc, factor, s);
+        for (size_t r = 0; r < rows; r++)
+            for (size_t k = 0; k < nc; k++) {
+                const size_t i = r * cols + c0 + k;
+                if (kind == 0)
+                    ((uint8_t *)data)[i] = (uint8_t)(unsigned)norm_clamp((float)((uint8_t *)data)[i] - s[k], nbits);
+                else if (kind == 1)
+                    ((uint16_t *)data)[i] =
+                        (uint16_t)(unsigned)norm_clamp((float)((uint16_t *)data)[i] - s[k], nbits);
+                else
+                    ((float *)data)[i] = ((float *)data)[i] - s[k];
+            }
+    }
+}
+
+void oracle_normalize_columns_u8(uint8_t *data, size_t rows, size_t cols, float factor, int nbits)
+{
+    normalize_columns(data, 0, rows, cols, factor, nbits);
+}
+
+void oracle_normalize_columns_u16(uint16_t *data, size_t rows, size_t cols, float factor, int nbits)
+{
+    normalize_columns(data, 1, rows, cols, factor, nbits);
+}
+
+void oracle_normalize_columns_f32(float *data, size_t rows, size_t cols, float factor)
+{
+    normalize_columns(data, 2, rows, cols, factor, 32);
+}
+
+/* The set_bandpass_average option (include/aa_permitted_pipelines_generic.hpp:
+ * 533-551 -> src/aa_host_cpu_msd.cpp:7-39): bandpass[i] = (float)(sum / n)
+ * with a double sum over input[i*nsamp .. i*nsamp + nsamp) for i < nchans,
+ * n = nsamp.  The input buffer is [t][c], so "channel" i averages the i-th
+ * run of nsamp consecutive samples of that buffer, not channel i; this is
+ * the reference's arithmetic as written. */
+void oracle_bandpass_average_u16(const uint16_t *in_tc, size_t nchans, size_t nsamp, float *bandpass)
+{
+    for (size_t i = 0; i < nchans; i++) {
+        double sum = 0.0;
+        for (size_t j = 0; j < nsamp; j++) sum += in_tc[i * nsamp + j];
+        bandpass[i] = (float)(sum / (double)nsamp);
+    }
+}
+
 void oracle_bin_u8(const uint8_t *in, size_t in_pitch, int nchans, int n_out,
                    uint8_t *out, size_t out_pitch)
 {

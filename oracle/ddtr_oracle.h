@@ -90,13 +90,23 @@ void oracle_corner_turn_u8(const uint8_t *in_tc, int nsamp, int nchans,
 void oracle_corner_turn_u16(const uint16_t *in_tc, int nsamp, int nchans,
                             uint8_t *out_ct, size_t out_pitch);
 
-/* One x2 level: out[c][t] = floor((in[c][2t] + in[c][2t+1]) / 2), t < n_out. */
 /* Zero-DM filter on [t][c] samples in place (src/aa_device_zero_dm_kernel.cu:
  * 37-88, src/aa_zero_dm.cpp:28-37): per time sample, mean = (channel sum) /
  * nchans in float; x' = (x - mean) + bandpass[c] in float, clamped to
  * [0, 2^nbits - 1] and truncated. */
 void oracle_zero_dm_u8(uint8_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits);
 void oracle_zero_dm_u16(uint16_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits);
+/* Column normalisation of a [rows][cols] array in place
+ * (src/aa_device_zero_dm_kernel.cu:100-164): per column, s = (float sum in
+ * the reference's order) / rows - factor; x' = x - s, clamped to
+ * [0, 2^nbits - 1] and truncated for integer data. */
+void oracle_normalize_columns_u8(uint8_t *data, size_t rows, size_t cols, float factor, int nbits);
+void oracle_normalize_columns_u16(uint16_t *data, size_t rows, size_t cols, float factor, int nbits);
+void oracle_normalize_columns_f32(float *data, size_t rows, size_t cols, float factor);
+/* set_bandpass_average (src/aa_host_cpu_msd.cpp:7-39): means of the first
+ * nchans runs of nsamp consecutive samples of the [t][c] input. */
+void oracle_bandpass_average_u16(const uint16_t *in_tc, size_t nchans, size_t nsamp, float *bandpass);
+/* One x2 level: out[c][t] = floor((in[c][2t] + in[c][2t+1]) / 2), t < n_out. */
 void oracle_bin_u8(const uint8_t *in, size_t in_pitch, int nchans, int n_out,
                    uint8_t *out, size_t out_pitch);
 

@@ -56,6 +56,10 @@ def lib() -> ctypes.CDLL:
             "oracle_bin_u8": ([vp, sz, i, i, vp, sz], None),
             "oracle_zero_dm_u8": ([vp, i, i, vp, i], None),
             "oracle_zero_dm_u16": ([vp, i, i, vp, i], None),
+            "oracle_normalize_columns_u8": ([vp, sz, sz, f, i], None),
+            "oracle_normalize_columns_u16": ([vp, sz, sz, f, i], None),
+            "oracle_normalize_columns_f32": ([vp, sz, sz, f], None),
+            "oracle_bandpass_average_u16": ([vp, sz, sz, vp], None),
             "oracle_dedisperse_range": ([vp, sz, i, i, pf, i, i, f, f, i, pf, sz], None),
             "oracle_dedisperse_rows": ([vp, sz, i, pf, pi, i, i, f, f, i, pf, sz], None),
             "oracle_ddtr_run_u8": ([ctypes.POINTER(_Strategy), vp, ctypes.POINTER(pf)], i),
@@ -166,6 +170,43 @@ def zero_dm(data_tc: np.ndarray, bandpass=None, nbits: int = 8) -> np.ndarray:
     fn = lib().oracle_zero_dm_u8 if out.dtype == np.uint8 else lib().oracle_zero_dm_u16
     fn(_vp(out), nsamp, nchans, _vp(bp), nbits)
     return out
+
+
+def normalize_columns(data: np.ndarray, factor: float, nbits: int = 32) -> np.ndarray:
+    """Column-normalised copy of a [rows, cols] u8, u16 or float32 array
+    (the reference's normalization_transposed_kernel)."""
+    out = np.ascontiguousarray(data).copy()
+    rows, cols = out.shape
+    if out.dtype == np.uint8:
+        lib().oracle_normalize_columns_u8(_vp(out), rows, cols, factor, nbits)
+    elif out.dtype == np.uint16:
+        lib().oracle_normalize_columns_u16(_vp(out), rows, cols, factor, nbits)
+    elif out.dtype == np.float32:
+        lib().oracle_normalize_columns_f32(_vp(out), rows, cols, factor)
+    else:
+        raise TypeError(out.dtype)
+    return out
+
+
+def input_normalization(data_tc: np.ndarray, nbits: int = 8) -> np.ndarray:
+    """Input DDTR normalisation of a [t][c] chunk: each channel's mean over
+    the chunk moved to (2^nbits - 1)/2."""
+    return normalize_columns(data_tc, float(np.float32(((1 << nbits) - 1) / 2.0)), nbits)
+
+
+def output_normalization(plane: np.ndarray) -> np.ndarray:
+    """Output DDTR normalisation of a [dm][t] float plane: each time sample's
+    mean over DMs moved to 127.5."""
+    return normalize_columns(np.asarray(plane, dtype=np.float32), 127.5, 32)
+
+
+def bandpass_average(data_tc: np.ndarray) -> np.ndarray:
+    """The set_bandpass_average bandpass of a [nsamp, nchans] input."""
+    x = np.ascontiguousarray(data_tc, dtype=np.uint16)
+    nsamp, nchans = x.shape
+    bp = np.zeros(nchans, dtype=np.float32)
+    lib().oracle_bandpass_average_u16(_vp(x), nchans, nsamp, _vp(bp))
+    return bp
 
 
 def bin2(level: np.ndarray, n_out: int) -> np.ndarray:
