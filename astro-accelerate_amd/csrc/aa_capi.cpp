@@ -398,6 +398,26 @@ int aa_hip_zero_dm_u16(uint16_t *d_data_tc, int nsamp, int nchans, const float *
                     "zero-DM");
 }
 
+int aa_hip_zero_dm_outliers_u8(uint8_t *d_data_tc, long nsamp, int nchans, const float *d_bandpass, int nbits,
+                               void *stream) {
+    if (nsamp < 0 || nchans <= 0 || nbits < 1 || nbits > 8) return bad("invalid geometry or nbits");
+    if (nsamp == 0) return AA_HIP_SUCCESS;
+    if (!d_data_tc || !d_bandpass) return bad("NULL buffer");
+    return from_hip(aa_hip::launch_zero_dm_outliers_u8(d_data_tc, nsamp, nchans, d_bandpass, nbits,
+                                                       (hipStream_t)stream),
+                    "zero-DM with outliers");
+}
+
+int aa_hip_zero_dm_outliers_u16(uint16_t *d_data_tc, long nsamp, int nchans, const float *d_bandpass, int nbits,
+                                void *stream) {
+    if (nsamp < 0 || nchans <= 0 || nbits < 1 || nbits > 16) return bad("invalid geometry or nbits");
+    if (nsamp == 0) return AA_HIP_SUCCESS;
+    if (!d_data_tc || !d_bandpass) return bad("NULL buffer");
+    return from_hip(aa_hip::launch_zero_dm_outliers_u16(d_data_tc, nsamp, nchans, d_bandpass, nbits,
+                                                        (hipStream_t)stream),
+                    "zero-DM with outliers");
+}
+
 int aa_hip_input_ddtr_normalization_u8(uint8_t *d_data_tc, long nsamp, int nchans, int nbits, void *stream) {
     return input_norm(d_data_tc, nsamp, nchans, nbits, 8, stream);
 }

@@ -110,14 +110,15 @@ public:
         for (auto o : m_options) {
             // Options that change the dedispersed values are refused; options
             // that only concern output files or other components are ignored.
-            const bool changes_values = o == opt::zero_dm_with_outliers || o == opt::old_rfi;
+            const bool changes_values = o == opt::old_rfi;
             if (changes_values) {
                 LOG(log_level::error, "pipeline option " + std::to_string((int)o) +
-                                          " (zero-DM with outliers / RFI) is not supported by this library.");
+                                          " (old RFI) is not supported by this library.");
                 aa_hip::set_last_error("unsupported pipeline option");
                 return false;
             }
             if (o != opt::copy_ddtr_data_to_host && o != opt::empty && o != opt::zero_dm &&
+                o != opt::zero_dm_with_outliers &&
                 o != opt::input_DDTR_normalization && o != opt::output_DDTR_normalization &&
                 o != opt::set_bandpass_average)
                 LOG(log_level::notice, "pipeline option " + std::to_string((int)o) + " has no effect here (ignored).");
@@ -249,8 +250,11 @@ private:
             max_out = std::max(max_out, (size_t)tp * (size_t)ndms[(size_t)r]);
         }
         if (he == hipSuccess) he = hipMalloc(&m_d_output, std::max<size_t>(max_out, 1) * sizeof(float));
-        m_zero_dm = m_options.find(aa_pipeline::component_option::zero_dm) != m_options.end();
-        if (he == hipSuccess && m_zero_dm) {   // the strategy's bandpass normalisation, one float per channel
+        m_zero_dm = has(aa_pipeline::component_option::zero_dm);
+        // zero_dm takes precedence when both are set, as in the reference's
+        // runner (include/aa_permitted_pipelines_generic.hpp:706-719).
+        m_zero_dm_outliers = !m_zero_dm && has(aa_pipeline::component_option::zero_dm_with_outliers);
+        if (he == hipSuccess && (m_zero_dm || m_zero_dm_outliers)) {   // the strategy's bandpass normalisation, one float per channel
             std::vector<float> bp = m_strategy.bandpass_normalization();
             // set_bandpass_average replaces it before the first chunk
             // (include/aa_permitted_pipelines_generic.hpp:533-551).
@@ -400,6 +404,17 @@ private:
                     LOG(log_level::error, std::string("zero-DM: ") + hipGetErrorString(he));
                     return false;
                 }
+            } else if (m_zero_dm_outliers) {
+                const long rows = (long)tp0 + m_strategy.maxshift();
+                he = m_bytes == 1 ? aa_hip::launch_zero_dm_outliers_u8((uint8_t *)m_d_input, rows, nchans,
+                                                                       m_d_bandpass, m_metadata.nbits(), m_stream)
+                                  : aa_hip::launch_zero_dm_outliers_u16((uint16_t *)m_d_input, rows, nchans,
+                                                                        m_d_bandpass, m_metadata.nbits(),
+                                                                        m_stream);
+                if (he != hipSuccess) {
+                    LOG(log_level::error, std::string("zero-DM with outliers: ") + hipGetErrorString(he));
+                    return false;
+                }
             }
             if (m_in_norm) {   // after zero-DM, before the corner turn (same rows)
                 const long rows = (long)tp0 + m_strategy.maxshift();
@@ -510,6 +525,7 @@ private:
     int m_prefetched_chunk = -1;
     bool m_zero_dm = false;                // zero-DM filter before each chunk's corner turn
     float *m_d_bandpass = nullptr;         // its per-channel normalisation
+    bool m_zero_dm_outliers = false;       // zero-DM with outlier rejection instead
     bool m_in_norm = false;                // input DDTR normalisation of each chunk
     bool m_out_norm = false;               // output DDTR normalisation of each plane
     void *m_d_norm_scratch = nullptr;      // input normalisation's column sums

@@ -18,6 +18,7 @@
 #include "aa_ddtr_hip.h"
 #include "aa_corner_turn.hpp"
 #include "aa_zero_dm.hpp"
+#include "aa_zero_dm_outliers.hpp"
 #include "aa_dedisperse.hpp"
 #include "aa_device_load_data.hpp"
 #include "aa_log.hpp"
@@ -236,6 +237,35 @@ void zero_dm(unsigned short *const d_input, const int nchans, const int nsamp, c
     }
     if (hipMemcpy(d_bp, bp.data(), bp.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess)
         zero_dm(d_input, nchans, nsamp, nbits, d_bp);
+    (void)hipFree(d_bp);
+}
+
+void zero_dm_outliers(unsigned short *const d_input, const int nchans, const int nsamp, const int nbits,
+                      float *normalization_factor)
+{
+    const hipError_t e =
+        aa_hip::launch_zero_dm_outliers_u16(d_input, nsamp, nchans, normalization_factor, nbits, 0);
+    if (e != hipSuccess) LOG(log_level::error, std::string("zero_dm_outliers: ") + hipGetErrorString(e));
+    (void)hipDeviceSynchronize();
+}
+
+// The reference's 4-argument form passes its pointer's value where the
+// allocation needs its address (src/aa_zero_dm_outliers.cpp:20); here the
+// default factors are allocated properly.
+void zero_dm_outliers(unsigned short *const d_input, const int nchans, const int nsamp, const int nbits)
+{
+    if (nchans <= 0 || nbits < 1 || nbits > 16) {
+        LOG(log_level::error, "zero_dm_outliers: invalid nchans or nbits");
+        return;
+    }
+    const std::vector<float> bp((size_t)nchans, (float)((std::pow(2.0, nbits) - 1.0) / 2.0));
+    float *d_bp = nullptr;
+    if (hipMalloc(&d_bp, bp.size() * sizeof(float)) != hipSuccess) {
+        LOG(log_level::error, "zero_dm_outliers: could not allocate the normalisation factors");
+        return;
+    }
+    if (hipMemcpy(d_bp, bp.data(), bp.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess)
+        zero_dm_outliers(d_input, nchans, nsamp, nbits, d_bp);
     (void)hipFree(d_bp);
 }
 

@@ -44,6 +44,12 @@ hipError_t launch_narrow_u16(const uint16_t* in, long in_pitch, int rows, long n
 // values clamped to 2^nbits - 1).
 hipError_t launch_zero_dm_u8(uint8_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s);
 hipError_t launch_zero_dm_u16(uint16_t* data_tc, int nsamp, int nchans, const float* bp, int nbits, hipStream_t s);
+// Zero-DM with outlier rejection in place on [t][c] samples (bp: nchans
+// device floats; values clamped to 2^nbits - 1).
+hipError_t launch_zero_dm_outliers_u8(uint8_t* data_tc, long nsamp, int nchans, const float* bp, int nbits,
+                                      hipStream_t s);
+hipError_t launch_zero_dm_outliers_u16(uint16_t* data_tc, long nsamp, int nchans, const float* bp, int nbits,
+                                       hipStream_t s);
 // Column normalisation of a [rows][cols] array in place (the reference's
 // DDTR input/output normalisation kernel): per column s = (float sum of the
 // column in the reference's order) / rows - factor, x' = x - s (integers

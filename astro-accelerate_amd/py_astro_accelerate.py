@@ -182,6 +182,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_frac_adjoint": ([vp, vp, vp, i, vp, vp, vp, vp], i),
         "aa_hip_zero_dm_u8": ([vp, i, i, vp, i, vp], i),
         "aa_hip_zero_dm_u16": ([vp, i, i, vp, i, vp], i),
+        "aa_hip_zero_dm_outliers_u8": ([vp, ctypes.c_long, i, vp, i, vp], i),
+        "aa_hip_zero_dm_outliers_u16": ([vp, ctypes.c_long, i, vp, i, vp], i),
         "aa_hip_input_ddtr_normalization_u8": ([vp, ctypes.c_long, i, i, vp], i),
         "aa_hip_input_ddtr_normalization_u16": ([vp, ctypes.c_long, i, i, vp], i),
         "aa_hip_output_ddtr_normalization": ([vp, i, ctypes.c_long, vp], i),
@@ -796,6 +798,16 @@ def zero_dm(d_data_tc, d_bandpass, nbits: int = 8, stream=None):
     nsamp, nchans = d_data_tc.shape
     fn = _lib_or_raise().aa_hip_zero_dm_u8 if d_data_tc.element_size() == 1 else _lib_or_raise().aa_hip_zero_dm_u16
     check(fn(_ptr(d_data_tc), int(nsamp), int(nchans), _ptr(d_bandpass), int(nbits), _stream_ptr(stream)), "zero_dm")
+
+
+def zero_dm_outliers(d_data_tc, d_bandpass, nbits: int = 8, stream=None):
+    """Zero-DM with outlier rejection in place on a [nsamp, nchans] uint8 or
+    uint16 device tensor (d_bandpass: nchans float32 device values)."""
+    nsamp, nchans = d_data_tc.shape
+    L = _lib_or_raise()
+    fn = L.aa_hip_zero_dm_outliers_u8 if d_data_tc.element_size() == 1 else L.aa_hip_zero_dm_outliers_u16
+    check(fn(_ptr(d_data_tc), int(nsamp), int(nchans), _ptr(d_bandpass), int(nbits), _stream_ptr(stream)),
+          "zero_dm_outliers")
 
 
 def input_ddtr_normalization(d_data_tc, nbits: int = 8, stream=None):

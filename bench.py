@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--normalization", action="store_true",
                     help="time the DDTR input normalisation on the config's [t][c] input and the output "
                          "normalisation on its largest range's plane (HBM roofline lines, not the contract line)")
+    ap.add_argument("--zero-dm-outliers", action="store_true",
+                    help="time zero-DM with outlier rejection on the config's [t][c] input (not the contract line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -99,6 +101,8 @@ def main():
         return bench_e2e(args, CONFIGS[args.config if args.config in CONFIGS else 2])
     if args.zero_dm:
         return bench_zero_dm(args, CONFIGS[args.config if args.config in CONFIGS else 2])
+    if args.zero_dm_outliers:
+        return bench_zero_dm(args, CONFIGS[args.config if args.config in CONFIGS else 2], outliers=True)
     if args.normalization:
         return bench_normalization(args, CONFIGS[args.config if args.config in CONFIGS else 2])
     if args.config == 3:
@@ -313,10 +317,12 @@ def bench_e2e(args, cfg):
     }), flush=True)
 
 
-def bench_zero_dm(args, cfg):
+def bench_zero_dm(args, cfg, outliers=False):
     """The zero-DM kernel on the config's [t][c] u8 input (4096 x 1,048,576
     for config 2), in place, timed with HIP events on its stream.  HBM-bound:
-    algorithmic bytes = the input read once and written once."""
+    algorithmic bytes = the input read once and written once.  With
+    `outliers`, zero-DM with outlier rejection (float64 statistics, up to six
+    passes over each time sample: compute-bound, reported as samples/s)."""
     import torch
     import py_astro_accelerate as aa
     dev = torch.device("cuda:0")
@@ -326,16 +332,25 @@ def bench_zero_dm(args, cfg):
     aa.synth_noise_u8(SEED, d, 128.0, 16.0)
     bp = torch.full((nchans,), 127.5, dtype=torch.float32, device=dev)
     stream = torch.cuda.Stream()
+    fn = aa.zero_dm_outliers if outliers else aa.zero_dm
     for _ in range(args.warmup):
-        aa.zero_dm(d, bp, nbits=8, stream=stream)
+        fn(d, bp, nbits=8, stream=stream)
     stream.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(args.steps):
-        aa.zero_dm(d, bp, nbits=8, stream=stream)
+        fn(d, bp, nbits=8, stream=stream)
     e1.record(stream)
     e1.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
+    if outliers:
+        sps = nchans * nsamp / (ms * 1e-3)
+        log(f"[zero-dm-outliers] {nchans} ch x {nsamp} samples u8: {ms:.3f} ms per pass, {sps / 1e9:.1f} G "
+            f"samples/s")
+        print(json.dumps({"metric": "zero-DM with outliers throughput", "value": sps / 1e9,
+                          "unit": "G samples/s", "ms_per_pass": ms,
+                          "config": {"workload": cfg["name"], "nchans": nchans, "nsamp": nsamp}}), flush=True)
+        return
     nbytes = 2.0 * nchans * nsamp
     gbs = nbytes / (ms * 1e-3) / 1e9
     log(f"[zero-dm] {nchans} ch x {nsamp} samples u8: {ms:.3f} ms per pass, {gbs:.0f} GB/s "

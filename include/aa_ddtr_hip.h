@@ -407,6 +407,22 @@ int aa_hip_zero_dm_u8(uint8_t *d_data_tc, int nsamp, int nchans, const float *d_
 int aa_hip_zero_dm_u16(uint16_t *d_data_tc, int nsamp, int nchans, const float *d_bandpass, int nbits,
                        void *stream);
 
+/* Zero-DM with outlier rejection, in place on [t][c] samples before the
+ * corner turn.  Replaces zero_dm_outliers(d_input, nchans, nsamp, nbits,
+ * d_normalization_factor) (include/aa_zero_dm_outliers.hpp:22,
+ * src/aa_zero_dm_outliers.cpp:47-67,
+ * src/aa_device_zero_dm_outliers_kernel.cu:190-233): per time sample, the
+ * mean and standard deviation over the channels, then up to 5 times over
+ * the channels strictly inside mean +- 3 sigma (stopping once the mean
+ * drops by less than 0.001), computed with the reference's streaming
+ * accumulators and merge order; x' = (x - mean) + d_bandpass[c] in float,
+ * clamped to [0, 2^nbits - 1], truncated (0 where the mean is NaN, i.e. no
+ * channel was inside the limits).  u8: nbits 1-8; u16: 1-16. */
+int aa_hip_zero_dm_outliers_u8(uint8_t *d_data_tc, long nsamp, int nchans, const float *d_bandpass, int nbits,
+                               void *stream);
+int aa_hip_zero_dm_outliers_u16(uint16_t *d_data_tc, long nsamp, int nchans, const float *d_bandpass, int nbits,
+                                void *stream);
+
 /* Input DDTR normalisation, in place on a [t][c] chunk before its corner
  * turn.  Replaces zero_dm_normalization_dm(d_input, nchans, nsamp, nbits)
  * as the reference's runner calls it (include/aa_zero_dm.hpp:26,

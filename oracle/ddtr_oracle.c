@@ -317,6 +317,127 @@ void oracle_zero_dm_u16(uint16_t *data_tc, int nsamp, int nchans, const float *b
     }
 }
 
+/* Zero-DM with outlier rejection (src/aa_device_zero_dm_outliers_kernel.cu:
+ * 15-233, launched by src/aa_zero_dm_outliers.cpp:47-67 with 128 threads per
+ * time sample, outlier_sigma 3).  Accumulator v (0..127) takes channels v,
+ * v+128, ... in order: the first with Initiate (:15-19), the rest with
+ * Add_one (:21-29).  The block reduction (Reduce_SM :42-70 for i = 64, 32,
+ * then Reduce_WARP :72-91 for q = 16..1) merges accumulator v with v+i for
+ * v < i, level by level, with Merge (:31-40) and the empty-accumulator
+ * rules written there.  Types follow the declarations (float M, S; double
+ * counts and factors); a product feeding an add is fused, as CUDA's default
+ * contraction compiles those lines.  mean = M/j, stdev = sqrt(S/j) in double
+ * (:118-121).  Passes: all channels (MSD_block), then up to 5 rejection
+ * passes over channels with lo < x < hi, lo/hi = mean -+ 3 stdev fused
+ * (:136-137), stopping when old_mean - mean < 0.001f (:203-207).  Output
+ * (:209-231): (x - mean) + bandpass[c] in float, clamped, truncated; a NaN
+ * (0/0 when no channel is inside the limits) converts to 0.  The reference
+ * clamps for nbits 4, 8 and 16; here every width clamps to 2^nbits - 1. */
+typedef struct { float M, S; int j; } zdo_acc;
+
+static void zdo_push(zdo_acc *a, float x)
+{
+    if (a->j == 0) {
+        a->M = x;
+        a->S = 0.0f;
+        a->j = 1;
+        return;
+    }
+    a->j += 1;
+    const double rj = (double)a->j;
+    a->M = a->M + x;
+    const float ft = (float)fma(rj, (double)x, -(double)a->M);
+    const double c = 1.0 / (rj * (rj - 1.0));
+    a->S = (float)fma(c * (double)ft, (double)ft, (double)a->S);
+}
+
+static void zdo_merge(zdo_acc *a, const zdo_acc *b)
+{
+    if (b->j <= 0) return;
+    if (a->j == 0) {
+        *a = *b;
+        return;
+    }
+    const double rb = (double)b->j, ra = (double)a->j;
+    const float ft = (float)fma(rb / ra, (double)a->M, -(double)b->M);
+    const double coef = ra / (rb * (ra + rb));
+    const float ss = a->S + b->S;
+    a->S = (float)fma(coef * (double)ft, (double)ft, (double)ss);
+    a->M = a->M + b->M;
+    a->j += b->j;
+}
+
+/* One statistics pass; reject = 0: every channel. */
+static void zdo_pass(const float *x, int nchans, int reject, float lo, float hi, float *mean, float *stdev)
+{
+    zdo_acc acc[128];
+    for (int v = 0; v < 128; v++) acc[v].M = acc[v].S = 0.0f, acc[v].j = 0;
+    for (int c = 0; c < nchans; c++)
+        if (!reject || (x[c] > lo && x[c] < hi)) zdo_push(&acc[c % 128], x[c]);
+    for (int i = 64; i > 0; i >>= 1)
+        for (int v = 0; v < i; v++) zdo_merge(&acc[v], &acc[v + i]);
+    *mean = (float)((double)acc[0].M / (double)acc[0].j);
+    *stdev = (float)sqrt((double)acc[0].S / (double)acc[0].j);
+}
+
+static void zdo_row(float *x, int nchans, float *mean_out)
+{
+    float mean, stdev;
+    zdo_pass(x, nchans, 0, 0.0f, 0.0f, &mean, &stdev);
+    for (int f = 0; f < 5; f++) {
+        const float old_mean = mean;
+        const float lo = fmaf(-3.0f, stdev, mean), hi = fmaf(3.0f, stdev, mean);
+        zdo_pass(x, nchans, 1, lo, hi, &mean, &stdev);
+        if (old_mean - mean < 0.001f) break;
+    }
+    *mean_out = mean;
+}
+
+static float zdo_value(float x, float mean, float bp, int nbits)
+{
+    const float r = (x - mean) + bp;
+    const float mx = (float)((1u << (nbits > 16 ? 16 : nbits)) - 1u);
+    if (r < 0.0f) return 0.0f;
+    if (r > mx) return mx;
+    if (r != r) return 0.0f;
+    return (float)(unsigned)r;
+}
+
+void oracle_zero_dm_outliers_u8(uint8_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits)
+{
+#pragma omp parallel
+    {
+        float *x = (float *)malloc((size_t)nchans * sizeof(float));
+#pragma omp for schedule(static)
+        for (int t = 0; t < nsamp; t++) {
+            uint8_t *row = data_tc + (size_t)t * (size_t)nchans;
+            for (int c = 0; c < nchans; c++) x[c] = (float)row[c];
+            float mean;
+            zdo_row(x, nchans, &mean);
+            for (int c = 0; c < nchans; c++) row[c] = (uint8_t)(unsigned)zdo_value(x[c], mean, bandpass[c], nbits);
+        }
+        free(x);
+    }
+}
+
+void oracle_zero_dm_outliers_u16(uint16_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits)
+{
+#pragma omp parallel
+    {
+        float *x = (float *)malloc((size_t)nchans * sizeof(float));
+#pragma omp for schedule(static)
+        for (int t = 0; t < nsamp; t++) {
+            uint16_t *row = data_tc + (size_t)t * (size_t)nchans;
+            for (int c = 0; c < nchans; c++) x[c] = (float)row[c];
+            float mean;
+            zdo_row(x, nchans, &mean);
+            for (int c = 0; c < nchans; c++)
+                row[c] = (uint16_t)(unsigned)zdo_value(x[c], mean, bandpass[c], nbits);
+        }
+        free(x);
+    }
+}
+
 /* Column normalisation of a [rows][cols] array in place, the reference's
  * normalization_transposed_kernel (src/aa_device_zero_dm_kernel.cu:100-164):
  * for each column, 32 float partial sums, partial y adding rows y, y+32,

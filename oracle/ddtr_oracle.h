@@ -96,6 +96,13 @@ void oracle_corner_turn_u16(const uint16_t *in_tc, int nsamp, int nchans,
  * [0, 2^nbits - 1] and truncated. */
 void oracle_zero_dm_u8(uint8_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits);
 void oracle_zero_dm_u16(uint16_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits);
+/* Zero-DM with outlier rejection on [t][c] samples in place
+ * (src/aa_device_zero_dm_outliers_kernel.cu:190-233): per time sample the
+ * mean over the channels inside mean +- 3 sigma (the reference's streaming
+ * statistics, up to 5 rejection passes); x' = (x - mean) + bandpass[c],
+ * clamped, truncated, NaN -> 0. */
+void oracle_zero_dm_outliers_u8(uint8_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits);
+void oracle_zero_dm_outliers_u16(uint16_t *data_tc, int nsamp, int nchans, const float *bandpass, int nbits);
 /* Column normalisation of a [rows][cols] array in place
  * (src/aa_device_zero_dm_kernel.cu:100-164): per column, s = (float sum in
  * the reference's order) / rows - factor; x' = x - s, clamped to

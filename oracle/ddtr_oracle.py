@@ -56,6 +56,8 @@ def lib() -> ctypes.CDLL:
             "oracle_bin_u8": ([vp, sz, i, i, vp, sz], None),
             "oracle_zero_dm_u8": ([vp, i, i, vp, i], None),
             "oracle_zero_dm_u16": ([vp, i, i, vp, i], None),
+            "oracle_zero_dm_outliers_u8": ([vp, i, i, vp, i], None),
+            "oracle_zero_dm_outliers_u16": ([vp, i, i, vp, i], None),
             "oracle_normalize_columns_u8": ([vp, sz, sz, f, i], None),
             "oracle_normalize_columns_u16": ([vp, sz, sz, f, i], None),
             "oracle_normalize_columns_f32": ([vp, sz, sz, f], None),
@@ -168,6 +170,18 @@ def zero_dm(data_tc: np.ndarray, bandpass=None, nbits: int = 8) -> np.ndarray:
     bp = np.full(nchans, ((1 << nbits) - 1.0) / 2.0, dtype=np.float32) if bandpass is None else \
         np.ascontiguousarray(bandpass, dtype=np.float32)
     fn = lib().oracle_zero_dm_u8 if out.dtype == np.uint8 else lib().oracle_zero_dm_u16
+    fn(_vp(out), nsamp, nchans, _vp(bp), nbits)
+    return out
+
+
+def zero_dm_outliers(data_tc: np.ndarray, bandpass=None, nbits: int = 8) -> np.ndarray:
+    """Zero-DM with outlier rejection, filtered copy of [nsamp, nchans] u8 or
+    u16 samples (default bandpass (2^nbits - 1) / 2 per channel)."""
+    out = np.ascontiguousarray(data_tc).copy()
+    nsamp, nchans = out.shape
+    bp = np.full(nchans, ((1 << nbits) - 1.0) / 2.0, dtype=np.float32) if bandpass is None else \
+        np.ascontiguousarray(bandpass, dtype=np.float32)
+    fn = lib().oracle_zero_dm_outliers_u8 if out.dtype == np.uint8 else lib().oracle_zero_dm_outliers_u16
     fn(_vp(out), nsamp, nchans, _vp(bp), nbits)
     return out
 

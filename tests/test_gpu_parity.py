@@ -234,12 +234,12 @@ def test_ffi_pipeline_rejects_out_of_range_u16(aa, cuda):
 
 
 def test_ffi_pipeline_rejects_unsupported_option(aa, cuda):
-    say("FFI pipeline rejects zero-DM with outliers")
+    say("FFI pipeline rejects old RFI")
     data = np.zeros((40000, 64), dtype=np.uint16)
     comps = aa.aa_py_pipeline_components()
     comps.dedispersion = True
     opts = aa.aa_py_pipeline_component_options()
-    opts.zero_dm_with_outliers = True
+    opts.old_rfi = True
     pipe = aa.aa_py_pipeline(comps, opts, _metadata(aa, 64, 40000, 1400.0, -4.0, 64e-6), data, 0)
     assert pipe.bind_ddtr_plan(aa.make_plan([(0, 50, 1, 1, 1)]))
     assert not pipe.run()
