@@ -85,6 +85,63 @@ __global__ __launch_bounds__(1024) void normalize_columns_kernel(T* data, long r
     }
 }
 
+// Float planes of a multiple of 4 columns: the same order, 4 columns per
+// lane (16-byte accesses), 256 columns per workgroup.
+__global__ __launch_bounds__(1024) void normalize_columns_f32x4_kernel(float* data, long rows, long cols,
+                                                                       float factor)
+{
+    __shared__ float4 part[16][64];
+    __shared__ float4 offs[64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const long nq = cols >> 2;
+    const long q = (long)blockIdx.x * 64 + lane;
+    const bool on = q < nq;
+    float4* d = reinterpret_cast<float4*>(data);
+    float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0;
+    if (on) {
+#pragma unroll 4
+        for (long r = w; r < rows; r += 32) {
+            const float4 x = d[r * nq + q];
+            a0.x = __fadd_rn(a0.x, x.x);
+            a0.y = __fadd_rn(a0.y, x.y);
+            a0.z = __fadd_rn(a0.z, x.z);
+            a0.w = __fadd_rn(a0.w, x.w);
+            if (r + 16 < rows) {
+                const float4 y = d[(r + 16) * nq + q];
+                a1.x = __fadd_rn(a1.x, y.x);
+                a1.y = __fadd_rn(a1.y, y.y);
+                a1.z = __fadd_rn(a1.z, y.z);
+                a1.w = __fadd_rn(a1.w, y.w);
+            }
+        }
+    }
+    part[w][lane] = make_float4(__fadd_rn(a0.x, a1.x), __fadd_rn(a0.y, a1.y), __fadd_rn(a0.z, a1.z),
+                                __fadd_rn(a0.w, a1.w));
+    __syncthreads();
+    if (w < 4) {   // wave k finishes component k of every lane's 4 columns
+        float p[16];
+#pragma unroll
+        for (int y = 0; y < 16; ++y) {
+            const float4 v = part[y][lane];
+            p[y] = w == 0 ? v.x : w == 1 ? v.y : w == 2 ? v.z : v.w;
+        }
+        const float sv = norm_offset(p, rows, factor);
+        float* o = reinterpret_cast<float*>(&offs[lane]);
+        o[w] = sv;
+    }
+    __syncthreads();
+    if (!on) return;
+    const float4 s = offs[lane];
+    for (long r = w; r < rows; r += 16) {
+        float4 x = d[r * nq + q];
+        x.x = __fsub_rn(x.x, s.x);
+        x.y = __fsub_rn(x.y, s.y);
+        x.z = __fsub_rn(x.z, s.z);
+        x.w = __fsub_rn(x.w, s.w);
+        d[r * nq + q] = x;
+    }
+}
+
 __global__ void norm_zero_kernel(uint32_t* p, long n)
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
@@ -156,7 +213,43 @@ __global__ void norm_offsets_kernel(const uint32_t* partial, long rows, long col
 }
 
 // Apply: x' = clamp(x - s[c]); blockIdx.x over column groups, rows strided
-// over blockIdx.y.
+// over blockIdx.y.  u8 rows of a multiple of 16 columns: 16 columns per
+// lane (one 16-byte access), four rows in flight.
+__device__ __forceinline__ uint32_t norm_word(uint32_t x, const float4& s, float maxval)
+{
+    return (uint32_t)norm_value((float)(x & 0xffu), s.x, maxval) |
+           (uint32_t)norm_value((float)((x >> 8) & 0xffu), s.y, maxval) << 8 |
+           (uint32_t)norm_value((float)((x >> 16) & 0xffu), s.z, maxval) << 16 |
+           (uint32_t)norm_value((float)(x >> 24), s.w, maxval) << 24;
+}
+
+__global__ __launch_bounds__(256) void norm_apply_u8x16_kernel(uint8_t* data, long rows, long cols,
+                                                               const float* __restrict__ offs, float maxval)
+{
+    const long nv = cols >> 4;
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    uint4* d = reinterpret_cast<uint4*>(data);
+    const float4* o = reinterpret_cast<const float4*>(offs) + 4 * v;
+    const float4 s0 = o[0], s1 = o[1], s2 = o[2], s3 = o[3];
+    const long step = gridDim.y;
+    long r = blockIdx.y;
+    for (; r + 3 * step < rows; r += 4 * step) {
+        uint4 x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = d[(r + k * step) * nv + v];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            d[(r + k * step) * nv + v] = make_uint4(norm_word(x[k].x, s0, maxval), norm_word(x[k].y, s1, maxval),
+                                                    norm_word(x[k].z, s2, maxval), norm_word(x[k].w, s3, maxval));
+    }
+    for (; r < rows; r += step) {
+        const uint4 x = d[r * nv + v];
+        d[r * nv + v] = make_uint4(norm_word(x.x, s0, maxval), norm_word(x.y, s1, maxval),
+                                   norm_word(x.z, s2, maxval), norm_word(x.w, s3, maxval));
+    }
+}
+
 __global__ void norm_apply_u8x4_kernel(uint8_t* data, long rows, long cols, const float* __restrict__ offs,
                                        float maxval)
 {
@@ -165,13 +258,7 @@ __global__ void norm_apply_u8x4_kernel(uint8_t* data, long rows, long cols, cons
     if (q >= nq) return;
     uint32_t* d = reinterpret_cast<uint32_t*>(data);
     const float4 s = reinterpret_cast<const float4*>(offs)[q];
-    for (long r = blockIdx.y; r < rows; r += gridDim.y) {
-        const uint32_t x = d[r * nq + q];
-        d[r * nq + q] = (uint32_t)norm_value((float)(x & 0xffu), s.x, maxval) |
-                        (uint32_t)norm_value((float)((x >> 8) & 0xffu), s.y, maxval) << 8 |
-                        (uint32_t)norm_value((float)((x >> 16) & 0xffu), s.z, maxval) << 16 |
-                        (uint32_t)norm_value((float)(x >> 24), s.w, maxval) << 24;
-    }
+    for (long r = blockIdx.y; r < rows; r += gridDim.y) d[r * nq + q] = norm_word(d[r * nq + q], s, maxval);
 }
 
 template <class T>
@@ -222,7 +309,10 @@ hipError_t normalize_int(T* data, long rows, long cols, float factor, int nbits,
     hipLaunchKernelGGL(norm_offsets_kernel, dim3(blocks_of(cols, 256)), dim3(256), 0, s, (const uint32_t*)partial,
                        rows, cols, factor, offs);
     const unsigned ry = (unsigned)(rows < 2048 ? rows : 2048);
-    if (x4 && (uintptr_t)offs % 16 == 0)
+    if (x4 && cols % 16 == 0 && (uintptr_t)data % 16 == 0 && (uintptr_t)offs % 16 == 0)
+        hipLaunchKernelGGL(norm_apply_u8x16_kernel, dim3(blocks_of(cols / 16, 256), ry), dim3(256), 0, s,
+                           (uint8_t*)data, rows, cols, (const float*)offs, maxval);
+    else if (x4 && (uintptr_t)offs % 16 == 0)
         hipLaunchKernelGGL(norm_apply_u8x4_kernel, dim3(blocks_of(cols / 4, 256), ry), dim3(256), 0, s,
                            (uint8_t*)data, rows, cols, (const float*)offs, maxval);
     else
@@ -252,8 +342,12 @@ hipError_t launch_normalize_columns_f32(float* data, long rows, long cols, float
 {
     if (!data || rows < 0 || cols <= 0) return hipErrorInvalidValue;
     if (rows == 0) return hipSuccess;
-    hipLaunchKernelGGL(normalize_columns_kernel<float>, dim3(blocks_of(cols, 64)), dim3(1024), 0, s, data, rows,
-                       cols, factor, 0.0f);
+    if (cols % 4 == 0 && (uintptr_t)data % 16 == 0)
+        hipLaunchKernelGGL(normalize_columns_f32x4_kernel, dim3(blocks_of(cols / 4, 64)), dim3(1024), 0, s, data,
+                           rows, cols, factor);
+    else
+        hipLaunchKernelGGL(normalize_columns_kernel<float>, dim3(blocks_of(cols, 64)), dim3(1024), 0, s, data,
+                           rows, cols, factor, 0.0f);
     return hipGetLastError();
 }
 
