@@ -297,6 +297,29 @@ private:
             release_device();
             return false;
         }
+        // With host copies and more than one range, a second output buffer
+        // lets each range's copy overlap the next range's kernels, when the
+        // memory budget left after the strategy's booking holds it.
+        if (m_host_out && m_strategy.get_nRanges() > 1) {
+            const size_t extra = std::max<size_t>(max_out, 1) * sizeof(float);
+            if (extra <= m_card.free_memory() && hipMalloc(&m_d_output2, extra) == hipSuccess) {
+                hipError_t oe = hipStreamCreate(&m_d2h_stream);
+                if (oe == hipSuccess) oe = hipEventCreateWithFlags(&m_ev_range, hipEventDisableTiming);
+                if (oe == hipSuccess && m_card.request_memory(extra)) {
+                    m_overlap = true;
+                } else {
+                    if (m_d2h_stream) (void)hipStreamDestroy(m_d2h_stream);
+                    if (m_ev_range) (void)hipEventDestroy(m_ev_range);
+                    (void)hipFree(m_d_output2);
+                    m_d2h_stream = nullptr;
+                    m_ev_range = nullptr;
+                    m_d_output2 = nullptr;
+                }
+            } else {
+                m_d_output2 = nullptr;
+                (void)hipGetLastError();
+            }
+        }
         return true;
     }
 
@@ -350,6 +373,17 @@ private:
             (void)hipStreamSynchronize(m_stream);
             (void)hipStreamDestroy(m_stream);
         }
+        if (m_d2h_stream) {
+            (void)hipStreamSynchronize(m_d2h_stream);
+            (void)hipStreamDestroy(m_d2h_stream);
+        }
+        if (m_ev_range) (void)hipEventDestroy(m_ev_range);
+        if (m_d_output2) (void)hipFree(m_d_output2);
+        m_d2h_stream = nullptr;
+        m_ev_range = nullptr;
+        m_d_output2 = nullptr;
+        m_overlap = false;
+        m_spec_chunk = m_spec_range = -1;
         if (m_ev_ingested) (void)hipEventDestroy(m_ev_ingested);
         if (m_ev_copied) (void)hipEventDestroy(m_ev_copied);
         if (m_d_input) (void)hipFree(m_d_input);
@@ -379,102 +413,95 @@ private:
         const int nchans = m_metadata.nchans();
         const int tp = m_strategy.t_processed()[(size_t)range][(size_t)chunk];
         (void)hipSetDevice(m_engine->device());
-        if (range == 0) {
-            const int tp0 = m_strategy.t_processed()[0][(size_t)chunk];
-            hipError_t he = hipSuccess;
-            if (m_prefetched_chunk == chunk) {   // copied during the previous chunk's ranges
-                he = hipStreamWaitEvent(m_stream, m_ev_copied, 0);
-            } else {
-                const size_t bytes = chunk_input_bytes(chunk);
-                const char *src = (const char *)m_input + (size_t)m_inc * (size_t)nchans * (size_t)m_bytes;
-                he = hipMemcpyAsync(m_d_input, src, bytes, hipMemcpyHostToDevice, m_stream);
-            }
-            if (he != hipSuccess) {
-                LOG(log_level::error, std::string("H2D copy: ") + hipGetErrorString(he));
-                return false;
-            }
-            if (m_zero_dm) {   // the reference filters the [t][c] chunk before its corner turn
-                const int rows = tp0 + m_strategy.maxshift();
-                he = m_bytes == 1
-                         ? aa_hip::launch_zero_dm_u8((uint8_t *)m_d_input, rows, nchans, m_d_bandpass,
-                                                     m_metadata.nbits(), m_stream)
-                         : aa_hip::launch_zero_dm_u16((uint16_t *)m_d_input, rows, nchans, m_d_bandpass,
-                                                      m_metadata.nbits(), m_stream);
+        // A range launched ahead by the previous call (never range 0) only
+        // needs its copy.
+        if (!(m_spec_chunk == chunk && m_spec_range == range)) {
+            if (range == 0) {
+                const int tp0 = m_strategy.t_processed()[0][(size_t)chunk];
+                hipError_t he = hipSuccess;
+                if (m_prefetched_chunk == chunk) {   // copied during the previous chunk's ranges
+                    he = hipStreamWaitEvent(m_stream, m_ev_copied, 0);
+                } else {
+                    const size_t bytes = chunk_input_bytes(chunk);
+                    const char *src = (const char *)m_input + (size_t)m_inc * (size_t)nchans * (size_t)m_bytes;
+                    he = hipMemcpyAsync(m_d_input, src, bytes, hipMemcpyHostToDevice, m_stream);
+                }
                 if (he != hipSuccess) {
-                    LOG(log_level::error, std::string("zero-DM: ") + hipGetErrorString(he));
+                    LOG(log_level::error, std::string("H2D copy: ") + hipGetErrorString(he));
                     return false;
                 }
-            } else if (m_zero_dm_outliers) {
-                const long rows = (long)tp0 + m_strategy.maxshift();
-                he = m_bytes == 1 ? aa_hip::launch_zero_dm_outliers_u8((uint8_t *)m_d_input, rows, nchans,
-                                                                       m_d_bandpass, m_metadata.nbits(), m_stream)
-                                  : aa_hip::launch_zero_dm_outliers_u16((uint16_t *)m_d_input, rows, nchans,
-                                                                        m_d_bandpass, m_metadata.nbits(),
-                                                                        m_stream);
-                if (he != hipSuccess) {
-                    LOG(log_level::error, std::string("zero-DM with outliers: ") + hipGetErrorString(he));
+                if (m_zero_dm) {   // the reference filters the [t][c] chunk before its corner turn
+                    const int rows = tp0 + m_strategy.maxshift();
+                    he = m_bytes == 1
+                             ? aa_hip::launch_zero_dm_u8((uint8_t *)m_d_input, rows, nchans, m_d_bandpass,
+                                                         m_metadata.nbits(), m_stream)
+                             : aa_hip::launch_zero_dm_u16((uint16_t *)m_d_input, rows, nchans, m_d_bandpass,
+                                                          m_metadata.nbits(), m_stream);
+                    if (he != hipSuccess) {
+                        LOG(log_level::error, std::string("zero-DM: ") + hipGetErrorString(he));
+                        return false;
+                    }
+                } else if (m_zero_dm_outliers) {
+                    const long rows = (long)tp0 + m_strategy.maxshift();
+                    he = m_bytes == 1 ? aa_hip::launch_zero_dm_outliers_u8((uint8_t *)m_d_input, rows, nchans,
+                                                                           m_d_bandpass, m_metadata.nbits(), m_stream)
+                                      : aa_hip::launch_zero_dm_outliers_u16((uint16_t *)m_d_input, rows, nchans,
+                                                                            m_d_bandpass, m_metadata.nbits(),
+                                                                            m_stream);
+                    if (he != hipSuccess) {
+                        LOG(log_level::error, std::string("zero-DM with outliers: ") + hipGetErrorString(he));
+                        return false;
+                    }
+                }
+                if (m_in_norm) {   // after zero-DM, before the corner turn (same rows)
+                    const long rows = (long)tp0 + m_strategy.maxshift();
+                    const int nb = m_metadata.nbits();
+                    const float factor = (float)((std::pow(2.0, nb) - 1.0) / 2.0);
+                    he = m_bytes == 1 ? aa_hip::launch_normalize_columns_u8((uint8_t *)m_d_input, rows, nchans, factor,
+                                                                            nb, m_d_norm_scratch, m_stream)
+                                      : aa_hip::launch_normalize_columns_u16((uint16_t *)m_d_input, rows, nchans,
+                                                                             factor, nb, m_d_norm_scratch, m_stream);
+                    if (he != hipSuccess) {
+                        LOG(log_level::error, std::string("input DDTR normalisation: ") + hipGetErrorString(he));
+                        return false;
+                    }
+                }
+                int st = m_engine->ingest(m_d_input, m_bytes == 1 ? 8 : 16, tp0, m_stream);
+                if (st != AA_HIP_SUCCESS) {
+                    LOG(log_level::error, std::string("corner turn: ") + aa_hip::last_error());
                     return false;
                 }
-            }
-            if (m_in_norm) {   // after zero-DM, before the corner turn (same rows)
-                const long rows = (long)tp0 + m_strategy.maxshift();
-                const int nb = m_metadata.nbits();
-                const float factor = (float)((std::pow(2.0, nb) - 1.0) / 2.0);
-                he = m_bytes == 1 ? aa_hip::launch_normalize_columns_u8((uint8_t *)m_d_input, rows, nchans, factor,
-                                                                        nb, m_d_norm_scratch, m_stream)
-                                  : aa_hip::launch_normalize_columns_u16((uint16_t *)m_d_input, rows, nchans,
-                                                                         factor, nb, m_d_norm_scratch, m_stream);
-                if (he != hipSuccess) {
-                    LOG(log_level::error, std::string("input DDTR normalisation: ") + hipGetErrorString(he));
-                    return false;
+                // The corner turn has taken the chunk out of m_d_input: the next
+                // chunk's copy starts behind it on the copy stream and overlaps
+                // this chunk's ranges (and their output copies).
+                if (chunk + 1 < m_strategy.num_tchunks()) {
+                    const long inc_next = m_inc + tp0;
+                    const char *src = (const char *)m_input + (size_t)inc_next * (size_t)nchans * (size_t)m_bytes;
+                    he = hipEventRecord(m_ev_ingested, m_stream);
+                    if (he == hipSuccess) he = hipStreamWaitEvent(m_copy_stream, m_ev_ingested, 0);
+                    if (he == hipSuccess)
+                        he = hipMemcpyAsync(m_d_input, src, chunk_input_bytes(chunk + 1), hipMemcpyHostToDevice,
+                                            m_copy_stream);
+                    if (he == hipSuccess) he = hipEventRecord(m_ev_copied, m_copy_stream);
+                    if (he != hipSuccess) {
+                        LOG(log_level::error, std::string("H2D prefetch: ") + hipGetErrorString(he));
+                        return false;
+                    }
+                    m_prefetched_chunk = chunk + 1;
+                }
+                if (m_bytes == 2) {
+                    st = m_engine->take_input_overflow(m_stream);
+                    if (st != 0) {
+                        LOG(log_level::error, st > 0 ? std::string("input sample above 255 in 8-bit data")
+                                                     : std::string(aa_hip::last_error()));
+                        aa_hip::set_last_error("input sample above 255 in 8-bit data");
+                        return false;
+                    }
                 }
             }
-            int st = m_engine->ingest(m_d_input, m_bytes == 1 ? 8 : 16, tp0, m_stream);
-            if (st != AA_HIP_SUCCESS) {
-                LOG(log_level::error, std::string("corner turn: ") + aa_hip::last_error());
-                return false;
-            }
-            // The corner turn has taken the chunk out of m_d_input: the next
-            // chunk's copy starts behind it on the copy stream and overlaps
-            // this chunk's ranges (and their output copies).
-            if (chunk + 1 < m_strategy.num_tchunks()) {
-                const long inc_next = m_inc + tp0;
-                const char *src = (const char *)m_input + (size_t)inc_next * (size_t)nchans * (size_t)m_bytes;
-                he = hipEventRecord(m_ev_ingested, m_stream);
-                if (he == hipSuccess) he = hipStreamWaitEvent(m_copy_stream, m_ev_ingested, 0);
-                if (he == hipSuccess)
-                    he = hipMemcpyAsync(m_d_input, src, chunk_input_bytes(chunk + 1), hipMemcpyHostToDevice,
-                                        m_copy_stream);
-                if (he == hipSuccess) he = hipEventRecord(m_ev_copied, m_copy_stream);
-                if (he != hipSuccess) {
-                    LOG(log_level::error, std::string("H2D prefetch: ") + hipGetErrorString(he));
-                    return false;
-                }
-                m_prefetched_chunk = chunk + 1;
-            }
-            if (m_bytes == 2) {
-                st = m_engine->take_input_overflow(m_stream);
-                if (st != 0) {
-                    LOG(log_level::error, st > 0 ? std::string("input sample above 255 in 8-bit data")
-                                                 : std::string(aa_hip::last_error()));
-                    aa_hip::set_last_error("input sample above 255 in 8-bit data");
-                    return false;
-                }
-            }
+            if (!launch_range(chunk, range)) return false;
         }
-        int st = m_engine->run_range(range, tp, m_d_output, (size_t)tp, m_stream);
-        if (st != AA_HIP_SUCCESS) {
-            LOG(log_level::error, std::string("dedispersion: ") + aa_hip::last_error());
-            return false;
-        }
-        if (m_out_norm) {   // each time sample's mean over the range's DMs moved to 127.5
-            const hipError_t he = aa_hip::launch_normalize_columns_f32(
-                m_d_output, m_strategy.ndms((size_t)range), tp, 127.5f, m_stream);
-            if (he != hipSuccess) {
-                LOG(log_level::error, std::string("output DDTR normalisation: ") + hipGetErrorString(he));
-                return false;
-            }
-        }
+        m_spec_chunk = m_spec_range = -1;
         if (m_host_out) {
             const int ib = m_strategy.dm((size_t)range).inBin;
             const size_t col = (size_t)(m_inc / ib);
@@ -484,18 +511,65 @@ private:
                 LOG(log_level::error, "output column offset exceeds the range's total samples.");
                 return false;
             }
-            hipError_t he = hipMemcpy2DAsync(m_host_blocks[(size_t)range] + col, total * sizeof(float), m_d_output,
-                                             (size_t)tp * sizeof(float), (size_t)tp * sizeof(float), (size_t)nd,
-                                             hipMemcpyDeviceToHost, m_stream);
+            hipStream_t cs = m_stream;
+            hipError_t he = hipSuccess;
+            if (m_overlap) {   // the copy waits for this range's kernels on its own stream
+                cs = m_d2h_stream;
+                he = hipEventRecord(m_ev_range, m_stream);
+                if (he == hipSuccess) he = hipStreamWaitEvent(cs, m_ev_range, 0);
+            }
+            if (he == hipSuccess)
+                he = hipMemcpy2DAsync(m_host_blocks[(size_t)range] + col, total * sizeof(float), out_buffer(range),
+                                      (size_t)tp * sizeof(float), (size_t)tp * sizeof(float), (size_t)nd,
+                                      hipMemcpyDeviceToHost, cs);
             if (he != hipSuccess) {
                 LOG(log_level::error, std::string("D2H copy: ") + hipGetErrorString(he));
                 return false;
             }
+            // The next range of this chunk starts computing into the other
+            // buffer while this range's copy runs; run() still returns only
+            // once this range's host data are complete.
+            if (m_overlap && range + 1 < (int)m_strategy.get_nRanges()) {
+                if (!launch_range(chunk, range + 1)) return false;
+                m_spec_chunk = chunk;
+                m_spec_range = range + 1;
+            }
+            he = hipStreamSynchronize(cs);
+            if (he != hipSuccess) {
+                LOG(log_level::error, std::string("device error: ") + hipGetErrorString(he));
+                return false;
+            }
+            return true;
         }
         hipError_t he = hipStreamSynchronize(m_stream);
         if (he != hipSuccess) {
             LOG(log_level::error, std::string("device error: ") + hipGetErrorString(he));
             return false;
+        }
+        return true;
+    }
+
+    // Output buffer of a range: ranges alternate between two buffers when
+    // the copies overlap the next range's kernels.
+    float *out_buffer(int range) const { return m_overlap && (range & 1) ? m_d_output2 : m_d_output; }
+
+    // Dedispersion of one (chunk, range) on m_stream into out_buffer(range),
+    // then the output normalisation when asked for.
+    bool launch_range(int chunk, int range) {
+        const int tp = m_strategy.t_processed()[(size_t)range][(size_t)chunk];
+        float *out = out_buffer(range);
+        int st = m_engine->run_range(range, tp, out, (size_t)tp, m_stream);
+        if (st != AA_HIP_SUCCESS) {
+            LOG(log_level::error, std::string("dedispersion: ") + aa_hip::last_error());
+            return false;
+        }
+        if (m_out_norm) {   // each time sample's mean over the range's DMs moved to 127.5
+            const hipError_t he =
+                aa_hip::launch_normalize_columns_f32(out, m_strategy.ndms((size_t)range), tp, 127.5f, m_stream);
+            if (he != hipSuccess) {
+                LOG(log_level::error, std::string("output DDTR normalisation: ") + hipGetErrorString(he));
+                return false;
+            }
         }
         return true;
     }
@@ -525,6 +599,11 @@ private:
     int m_prefetched_chunk = -1;
     bool m_zero_dm = false;                // zero-DM filter before each chunk's corner turn
     float *m_d_bandpass = nullptr;         // its per-channel normalisation
+    bool m_overlap = false;                // D2H copies overlap the next range's kernels
+    float *m_d_output2 = nullptr;          // the second output buffer (odd ranges) then
+    hipStream_t m_d2h_stream = nullptr;
+    hipEvent_t m_ev_range = nullptr;       // a range's kernels done
+    int m_spec_chunk = -1, m_spec_range = -1;   // (chunk, range) already launched
     bool m_zero_dm_outliers = false;       // zero-DM with outlier rejection instead
     bool m_in_norm = false;                // input DDTR normalisation of each chunk
     bool m_out_norm = false;               // output DDTR normalisation of each plane

@@ -194,22 +194,39 @@ def test_pipeline_output_normalization_matches_oracle(aa, oracle, cuda):
 
 @pytest.mark.gpu
 def test_pipeline_input_normalization_matches_oracle(aa, oracle, cuda):
-    """One time chunk: its t_processed + maxshift rows are normalised per
-    channel before the corner turn, then dedispersed."""
-    say("pipeline input_DDTR_normalization, one chunk")
-    nchans, nsamp, tsamp, fch1, foff = 128, 60000, 64e-6, 1400.0, -2.0
-    ranges = [(0, 60, 0.5, 1, 1), (60, 180, 1.0, 2, 2)]
-    s = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp)
-    assert s.ok and s.num_tchunks == 1
+    """Several time chunks: each chunk's t_processed + maxshift rows are
+    normalised per channel on their own (the next chunk's copy arriving
+    meanwhile), then corner-turned and dedispersed.  The oracle normalises
+    the same rows of a copy of the input for each chunk and keeps that
+    chunk's columns."""
+    say("pipeline input_DDTR_normalization, several chunks")
+    nchans, nsamp, tsamp, fch1, foff = 96, 200000, 1.28e-4, 1200.0, -2.5
+    ranges = [(0, 40, 0.5, 1, 1), (40, 80, 1.0, 2, 2), (80, 160, 2.0, 4, 4)]
+    free = None
+    for extra in range(200_000_000, 1_000_000, -250_000):
+        s0 = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp, free_memory=104857600 + extra)
+        if s0.ok and s0.num_tchunks >= 3:
+            free = 104857600 + extra
+            break
+    assert free is not None
+    s = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp, free_memory=free)
     data = oracle.gen_noise(43, nsamp, nchans)
     data[:, 5] = np.minimum(data[:, 5].astype(np.int32) + 50, 255).astype(np.uint8)   # a hot channel
+    data[90000:, 17] = data[90000:, 17] // 2                                          # a channel that drops
     oracle.inject_pulse(data, s.dmshifts, 90.0, tsamp, 30000, 6, 40)
-    rows = s.t_processed[0][0] + s.maxshift
-    normed = data.copy()
-    normed[:rows] = oracle.input_normalization(data[:rows])
-    want = oracle.run(s, normed)
+    tp = s.t_processed
+    want = [np.zeros((s.ndms[r], sum(tp[r])), np.float32) for r in range(s.nranges)]
+    for k in range(s.num_tchunks):
+        inc, rows = sum(tp[0][:k]), tp[0][k] + s.maxshift
+        dk = data.copy()
+        dk[inc:inc + rows] = oracle.input_normalization(data[inc:inc + rows])
+        out = oracle.run(s, dk)
+        for r in range(s.nranges):
+            c0 = sum(tp[r][:k])
+            want[r][:, c0:c0 + tp[r][k]] = out[r][:, c0:c0 + tp[r][k]]
     md = aa.make_metadata(nchans, nsamp, fch1, foff, tsamp)
-    _, planes = _pipeline(aa, ranges, md, data, 280 * 2**30, input_DDTR_normalization=True)
+    strat, planes = _pipeline(aa, ranges, md, data, free, input_DDTR_normalization=True)
+    assert strat.num_tchunks == s.num_tchunks >= 3
     for r in range(s.nranges):
         np.testing.assert_array_equal(planes[r], want[r])
 
