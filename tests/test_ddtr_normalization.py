@@ -155,7 +155,8 @@ def _pipeline(aa, ranges, md, data, free, **opts_on):
     for k, v in opts_on.items():
         setattr(opts, k, v)
     plan = aa.make_plan(ranges)
-    pipe = aa.aa_py_pipeline(comps, opts, md, np.ascontiguousarray(data), 0, max_device_bytes=free)
+    data = np.ascontiguousarray(data)
+    pipe = aa.aa_py_pipeline(comps, opts, md, data, 0, max_device_bytes=free if data.dtype == np.uint8 else 0)
     assert pipe.bind_ddtr_plan(plan)
     while pipe.run():
         pass
@@ -245,5 +246,30 @@ def test_pipeline_zero_dm_with_bandpass_average_matches_oracle(aa, oracle, cuda)
     want = oracle.run(s, oracle.zero_dm(data, bandpass=bp))
     md = aa.make_metadata(nchans, nsamp, fch1, foff, tsamp)
     _, planes = _pipeline(aa, ranges, md, data, 280 * 2**30, zero_dm=True, set_bandpass_average=True)
+    for r in range(s.nranges):
+        np.testing.assert_array_equal(planes[r], want[r])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_pipeline_all_value_options_compose(aa, oracle, cuda, dtype):
+    """zero_dm_with_outliers, input and output normalisation together, in the
+    reference's order (outlier zero-DM, input normalisation, corner turn,
+    dedispersion, output normalisation), on u8 and on u16 host input."""
+    say(f"pipeline outliers + input + output normalisation, {np.dtype(dtype).name} input")
+    nchans, nsamp, tsamp, fch1, foff = 128, 60000, 64e-6, 1400.0, -2.0
+    ranges = [(0, 60, 0.5, 1, 1), (60, 180, 1.0, 2, 2)]
+    s = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp)
+    assert s.ok and s.num_tchunks == 1
+    data = oracle.gen_noise(59, nsamp, nchans)
+    data[:, 9] = 250                      # a hot channel
+    oracle.inject_pulse(data, s.dmshifts, 90.0, tsamp, 30000, 6, 40)
+    rows = s.t_processed[0][0] + s.maxshift
+    pre = data.copy()
+    pre[:rows] = oracle.input_normalization(oracle.zero_dm_outliers(data[:rows]))
+    want = [oracle.output_normalization(p) for p in oracle.run(s, pre)]
+    md = aa.make_metadata(nchans, nsamp, fch1, foff, tsamp)
+    _, planes = _pipeline(aa, ranges, md, data.astype(dtype), 280 * 2**30, zero_dm_with_outliers=True,
+                          input_DDTR_normalization=True, output_DDTR_normalization=True)
     for r in range(s.nranges):
         np.testing.assert_array_equal(planes[r], want[r])
