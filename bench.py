@@ -90,8 +90,8 @@ def main():
                     help="time zero-DM with outlier rejection on the config's [t][c] input (not the contract line)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "dedispersion shards by DM range or beam with no exchange, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
+              "dedispersion would shard by DM range or beam with no exchange", file=sys.stderr)
         sys.exit(2)
 
     import torch
