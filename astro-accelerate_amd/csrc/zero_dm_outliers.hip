@@ -42,29 +42,21 @@ struct welford {
     int j;
 };
 
-__device__ __forceinline__ void wf_init(welford& a, float x)
-{
-    a.M = x;
-    a.S = 0.0f;
-    a.j = 1;
-}
-
 __device__ __forceinline__ void wf_add(welford& a, float x, const double* coef)
 {
     a.j += 1;
     const double rj = (double)a.j;
     a.M = a.M + x;
     const float ft = (float)fma(rj, (double)x, -(double)a.M);
-    const double c = a.j < kCoefTab ? coef[a.j] : 1.0 / (rj * (rj - 1.0));
+    const double c = a.j < kCoefTab ? coef[a.j] : 1.0 / (rj * (rj - 1.0));   // coef[1] = 0
     a.S = (float)fma(c * (double)ft, (double)ft, (double)a.S);
 }
 
+// Add_one on an empty accumulator with factor 0 (coef[1]) gives M = x,
+// S = 0, j = 1 exactly, i.e. Initiate: one code path for every channel.
 __device__ __forceinline__ void wf_push(welford& a, float x, const double* coef)
 {
-    if (a.j == 0)
-        wf_init(a, x);
-    else
-        wf_add(a, x, coef);
+    wf_add(a, x, coef);
 }
 
 // a <- Merge(a, b), with the reference's handling of empty accumulators.
