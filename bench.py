@@ -84,10 +84,12 @@ def main():
                     help="--e2e: with the pipeline's zero_dm option; alone: time the zero-DM kernel on the "
                          "config's [t][c] input (HBM roofline line, not the contract line)")
     ap.add_argument("--normalization", action="store_true",
-                    help="time the DDTR input normalisation on the config's [t][c] input and the output "
-                         "normalisation on its largest range's plane (HBM roofline lines, not the contract line)")
+                    help="--e2e: with the pipeline's input and output DDTR normalisation options; alone: time "
+                         "the input normalisation on the config's [t][c] input and the output normalisation on "
+                         "its largest range's plane (HBM roofline lines, not the contract line)")
     ap.add_argument("--zero-dm-outliers", action="store_true",
-                    help="time zero-DM with outlier rejection on the config's [t][c] input (not the contract line)")
+                    help="--e2e: with the pipeline's zero_dm_with_outliers option; alone: time zero-DM with "
+                         "outlier rejection on the config's [t][c] input (not the contract line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
@@ -291,6 +293,9 @@ def bench_e2e(args, cfg):
         opts = aa.aa_py_pipeline_component_options()
         opts.copy_ddtr_data_to_host = True
         opts.zero_dm = bool(args.zero_dm)
+        opts.zero_dm_with_outliers = bool(args.zero_dm_outliers)
+        opts.input_DDTR_normalization = bool(args.normalization)
+        opts.output_DDTR_normalization = bool(args.normalization)
         pipe = aa.aa_py_pipeline(comps, opts, md, buf, 0, max_device_bytes=cap)
         assert pipe.bind_ddtr_plan(plan)
         steps = 0
@@ -313,7 +318,9 @@ def bench_e2e(args, cfg):
         "value": obs_s / total, "unit": "x real time", "higher_is_better": True, "ok": ok,
         "read_s": t1 - t0, "pipeline_s": t2 - t1, "observation_s": obs_s, "time_chunks": nchunks,
         "config": {"workload": cfg["name"], "nchans": nchans, "nsamp": nsamp, "input": "SIGPROC 8-bit file",
-                   "host_buffers": "pinned", "device_memory_cap_bytes": cap, "zero_dm": bool(args.zero_dm)},
+                   "host_buffers": "pinned", "device_memory_cap_bytes": cap, "zero_dm": bool(args.zero_dm),
+                   "zero_dm_with_outliers": bool(args.zero_dm_outliers),
+                   "input_output_normalization": bool(args.normalization)},
     }), flush=True)
 
 
