@@ -23,7 +23,8 @@
 // shuffles down to lane 0).  The 1/(j(j-1)) factors of Add_one are a table in
 // LDS (correctly rounded doubles, the same values the division gives).
 // Compute-bound on the float64 VALU (about 15 instructions per channel and
-// pass, up to 6 passes); the row is re-read from L1/L2 for each pass.
+// pass, up to 6 passes); the row is re-read from L1/L2 for each pass, eight
+// channels per lane at a time.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -42,21 +43,25 @@ struct welford {
     int j;
 };
 
+// SMALL: j * x and M stay below 2^24 (8-bit samples), so j * x - M, an
+// integer, is exact in float and needs no float64 work.
+template <bool SMALL>
 __device__ __forceinline__ void wf_add(welford& a, float x, const double* coef)
 {
     a.j += 1;
     const double rj = (double)a.j;
     a.M = a.M + x;
-    const float ft = (float)fma(rj, (double)x, -(double)a.M);
+    const float ft = SMALL ? (float)a.j * x - a.M : (float)fma(rj, (double)x, -(double)a.M);
     const double c = a.j < kCoefTab ? coef[a.j] : 1.0 / (rj * (rj - 1.0));   // coef[1] = 0
     a.S = (float)fma(c * (double)ft, (double)ft, (double)a.S);
 }
 
 // Add_one on an empty accumulator with factor 0 (coef[1]) gives M = x,
 // S = 0, j = 1 exactly, i.e. Initiate: one code path for every channel.
+template <bool SMALL>
 __device__ __forceinline__ void wf_push(welford& a, float x, const double* coef)
 {
-    wf_add(a, x, coef);
+    wf_add<SMALL>(a, x, coef);
 }
 
 // a <- Merge(a, b), with the reference's handling of empty accumulators.
@@ -97,7 +102,7 @@ __device__ __forceinline__ void wf_reduce(welford v0, const welford& v1, int lan
     stdev = __shfl(s, 0);
 }
 
-template <class T>
+template <class T, bool SMALL>
 __global__ __launch_bounds__(256) void zero_dm_outliers_kernel(T* data, long nsamp, int nchans,
                                                                const float* __restrict__ bp, float maxval)
 {
@@ -110,27 +115,27 @@ __global__ __launch_bounds__(256) void zero_dm_outliers_kernel(T* data, long nsa
     for (long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6); t < nsamp; t += waves) {
         T* row = data + t * (long)nchans;
         welford v0{0.0f, 0.0f, 0}, v1{0.0f, 0.0f, 0};
-        for (int c = lane; c < nchans; c += 128) {
-            wf_push(v0, (float)row[c], coef);
-            if (c + 64 < nchans) wf_push(v1, (float)row[c + 64], coef);
-        }
-        float mean, stdev;
-        wf_reduce(v0, v1, lane, mean, stdev);
-        for (int f = 0; f < 5; ++f) {
+        float mean = 0.0f, stdev = 0.0f;
+        // Pass 0 takes every channel (limits -inf, +inf); then up to 5
+        // rejection passes.  Channels come in batches of 8 per lane
+        // (lane + 64 k), all loads issued before the first add.
+        float lo = -INFINITY, hi = INFINITY;
+        for (int f = 0; f < 6; ++f) {
             const float old_mean = mean;
-            const float lo = fmaf(-3.0f, stdev, mean), hi = fmaf(3.0f, stdev, mean);
             v0 = welford{0.0f, 0.0f, 0};
             v1 = welford{0.0f, 0.0f, 0};
-            for (int c = lane; c < nchans; c += 128) {
-                const float x0 = (float)row[c];
-                if (x0 > lo && x0 < hi) wf_push(v0, x0, coef);
-                if (c + 64 < nchans) {
-                    const float x1 = (float)row[c + 64];
-                    if (x1 > lo && x1 < hi) wf_push(v1, x1, coef);
-                }
+            for (int c = lane; c < nchans; c += 512) {
+                float x[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) x[k] = c + 64 * k < nchans ? (float)row[c + 64 * k] : NAN;
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (x[k] > lo && x[k] < hi) wf_push<SMALL>((k & 1) ? v1 : v0, x[k], coef);
             }
             wf_reduce(v0, v1, lane, mean, stdev);
-            if (old_mean - mean < 0.001f) break;   // wave-uniform
+            if (f > 0 && old_mean - mean < 0.001f) break;   // wave-uniform
+            lo = fmaf(-3.0f, stdev, mean);
+            hi = fmaf(3.0f, stdev, mean);
         }
         for (int c = lane; c < nchans; c += 64) {
             const float r = ((float)row[c] - mean) + bp[c];
@@ -147,8 +152,16 @@ hipError_t launch_outliers(T* data, long nsamp, int nchans, const float* bp, int
     if (nsamp == 0) return hipSuccess;
     const long blocks = (nsamp + 3) / 4;
     const unsigned grid = (unsigned)(blocks < 16384 ? blocks : 16384);
-    hipLaunchKernelGGL(zero_dm_outliers_kernel<T>, dim3(grid), dim3(256), 0, s, data, nsamp, nchans, bp,
-                       (float)((1u << nbits) - 1u));
+    const float maxval = (float)((1u << nbits) - 1u);
+    // Counts per accumulator reach ceil(nchans / 128); with 8-bit samples
+    // j * x and M stay below 2^24 while that count stays below 65793.
+    const bool small = sizeof(T) == 1 && (nchans + 127) / 128 < 65793;
+    if (small)
+        hipLaunchKernelGGL((zero_dm_outliers_kernel<T, true>), dim3(grid), dim3(256), 0, s, data, nsamp, nchans, bp,
+                           maxval);
+    else
+        hipLaunchKernelGGL((zero_dm_outliers_kernel<T, false>), dim3(grid), dim3(256), 0, s, data, nsamp, nchans,
+                           bp, maxval);
     return hipGetLastError();
 }
 
