@@ -138,6 +138,8 @@ lds_params lds_params_of(const range_launch &rl) {
     p.mdmstep = rl.mdmstep;
     p.fnchans = (float)rl.nchans;
     p.fbin = (float)rl.inBin;
+    const auto pow2 = [](long v) { return v > 0 && (v & (v - 1)) == 0; };
+    p.fscale = pow2(rl.nchans) && pow2(rl.inBin) ? (float)(1.0 / ((double)rl.nchans * (double)rl.inBin)) : 0.0f;
     p.out = rl.out;
     p.out_pitch = rl.out_pitch;
     p.row_quads = rl.row_quads;

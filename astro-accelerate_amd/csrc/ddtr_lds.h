@@ -29,12 +29,21 @@ struct lds_params {
     int t_processed;
     float mstartdm, mdmstep;
     float fnchans, fbin;
+    float fscale;            // 1 / (nchans * inBin) when both are powers of two (the two
+                             // divisions are then exact and equal this product), else 0
     float* out;
     size_t out_pitch;        // floats between output rows
     int row_quads;           // staged row width in uint4 (from lds_row_quads)
     const void* pair_recs;   // chunk shapes 2-4: device plan tables (from lds_plan_tables)
     int n_dm_tiles, n_t_tiles;  // filled by the launcher / lds_job_tiles
 };
+
+// The output value (float)sum / nchans / inBin, each division rounded as the
+// reference's; one multiplication when both divisors are powers of two.
+__device__ __forceinline__ float out_value(uint32_t sum, const lds_params& p)
+{
+    return p.fscale != 0.0f ? (float)sum * p.fscale : __fdiv_rn(__fdiv_rn((float)sum, p.fnchans), p.fbin);
+}
 
 // One range of a batched launch.  The launch has 8 queues (one per XCD);
 // queue x holds, job after job, the x-th eighth of every job's tiles (tiles

@@ -350,7 +350,7 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
 #pragma unroll
         for (int j = 0; j < kR; ++j) {
             const long tt = tl + j;
-            if (tt < p.t_processed) orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[j], p.fnchans), p.fbin);
+            if (tt < p.t_processed) orow[tt] = out_value(acc32[j], p);
         }
     }
 }
@@ -584,7 +584,7 @@ __device__ __forceinline__ void process_quad_tile(const lds_params& p, int dt, l
             for (int j = 0; j < kQR / 2; ++j) {
                 const long tt = t0 + h * kQSplit + lane + 64 * j;
                 if (tt < p.t_processed)
-                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * (kQR / 2) + j], p.fnchans), p.fbin);
+                    orow[tt] = out_value(acc32[h * (kQR / 2) + j], p);
             }
     }
 }

@@ -265,7 +265,7 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
             for (int x = 0; x < A; ++x) {
                 const long tt = t0 + h * SPLIT + kWBlk * (x >> 2) + 4 * lane + (x & 3);
                 if (tt < p.t_processed)
-                    orow[tt] = __fdiv_rn(__fdiv_rn((float)acc32[h * A + x], p.fnchans), p.fbin);
+                    orow[tt] = out_value(acc32[h * A + x], p);
             }
     }
 }
