@@ -251,12 +251,13 @@ def test_pipeline_zero_dm_with_bandpass_average_matches_oracle(aa, oracle, cuda)
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("zdm", ["outliers", "plain"])
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
-def test_pipeline_all_value_options_compose(aa, oracle, cuda, dtype):
-    """zero_dm_with_outliers, input and output normalisation together, in the
-    reference's order (outlier zero-DM, input normalisation, corner turn,
-    dedispersion, output normalisation), on u8 and on u16 host input."""
-    say(f"pipeline outliers + input + output normalisation, {np.dtype(dtype).name} input")
+def test_pipeline_all_value_options_compose(aa, oracle, cuda, dtype, zdm):
+    """Zero-DM (with outliers, or plain), input and output normalisation
+    together, in the reference's order (zero-DM, input normalisation, corner
+    turn, dedispersion, output normalisation), on u8 and on u16 host input."""
+    say(f"pipeline {zdm} zero-DM + input + output normalisation, {np.dtype(dtype).name} input")
     nchans, nsamp, tsamp, fch1, foff = 128, 60000, 64e-6, 1400.0, -2.0
     ranges = [(0, 60, 0.5, 1, 1), (60, 180, 1.0, 2, 2)]
     s = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp)
@@ -266,10 +267,12 @@ def test_pipeline_all_value_options_compose(aa, oracle, cuda, dtype):
     oracle.inject_pulse(data, s.dmshifts, 90.0, tsamp, 30000, 6, 40)
     rows = s.t_processed[0][0] + s.maxshift
     pre = data.copy()
-    pre[:rows] = oracle.input_normalization(oracle.zero_dm_outliers(data[:rows]))
+    zdm_fn = oracle.zero_dm_outliers if zdm == "outliers" else oracle.zero_dm
+    pre[:rows] = oracle.input_normalization(zdm_fn(data[:rows]))
     want = [oracle.output_normalization(p) for p in oracle.run(s, pre)]
     md = aa.make_metadata(nchans, nsamp, fch1, foff, tsamp)
-    _, planes = _pipeline(aa, ranges, md, data.astype(dtype), 280 * 2**30, zero_dm_with_outliers=True,
-                          input_DDTR_normalization=True, output_DDTR_normalization=True)
+    zopt = {"zero_dm_with_outliers": True} if zdm == "outliers" else {"zero_dm": True}
+    _, planes = _pipeline(aa, ranges, md, data.astype(dtype), 280 * 2**30, input_DDTR_normalization=True,
+                          output_DDTR_normalization=True, **zopt)
     for r in range(s.nranges):
         np.testing.assert_array_equal(planes[r], want[r])
