@@ -7,6 +7,7 @@
 // plus a library-owned device copy of the dmshift table.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -83,6 +84,14 @@ void load_chunk_data(unsigned short *device_pointer, unsigned short const *const
         g_const.device = dev;
     }
     (void)hipMemcpy(g_const.d_dmshifts, dmshifts, sizeof(float) * (size_t)nchans, hipMemcpyHostToDevice);
+    // The cached plan tables hold delays computed from the previous table:
+    // a new observation (other fch1/foff/nchans) invalidates all of them.
+    if (g_const.dmshifts.size() != (size_t)nchans ||
+        !std::equal(g_const.dmshifts.begin(), g_const.dmshifts.end(), dmshifts)) {
+        for (auto &kv : g_const.tables)
+            if (kv.second.d) (void)hipFree(kv.second.d);
+        g_const.tables.clear();
+    }
     g_const.dmshifts.assign(dmshifts, dmshifts + nchans);
     g_const.nchans = nchans;
     g_const.i_nsamp = length;

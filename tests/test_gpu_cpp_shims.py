@@ -47,18 +47,13 @@ def _p(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-@pytest.mark.parametrize("ranges", [
-    [(0, 120, 0.5, 1, 1), (120, 360, 1.0, 2, 2)],                       # pair shapes, then one halving
-    [(0, 60, 0.25, 1, 1)],                                               # one range, no binning
-])
-def test_reference_runner_sequence_matches_oracle(aa, oracle, cuda, ranges):
+def _run_sequence(aa, oracle, cuda, f, ranges, foff=-1.0, seed=9):
+    """One chunk through the runner's call sequence, each range checked."""
     import torch
-    say(f"C++ drop-in sequence {ranges}")
-    f = _fns(aa)
-    nchans, nsamp, tsamp, fch1, foff = 256, 131072, 64e-6, 1500.0, -1.0
+    nchans, nsamp, tsamp, fch1 = 256, 131072, 64e-6, 1500.0
     s = oracle.strategy(ranges, nchans, nsamp, fch1, foff, tsamp)
     assert s.ok and s.num_tchunks == 1
-    data = oracle.gen_noise(9, nsamp, nchans)
+    data = oracle.gen_noise(seed, nsamp, nchans)
     oracle.inject_pulse(data, s.dmshifts, 100.0, tsamp, 40000, 6, 40)
     want = oracle.run(s, data)
 
@@ -91,4 +86,23 @@ def test_reference_runner_sequence_matches_oracle(aa, oracle, cuda, ranges):
         assert st == 0
         torch.cuda.synchronize()
         got = d_out[:s.ndms[r] * tp[r]].cpu().numpy().reshape(s.ndms[r], tp[r])
-        np.testing.assert_array_equal(got, want[r][:, :tp[r]], err_msg=f"range {r}")
+        np.testing.assert_array_equal(got, want[r][:, :tp[r]], err_msg=f"foff {foff} range {r}")
+
+
+@pytest.mark.parametrize("ranges", [
+    [(0, 120, 0.5, 1, 1), (120, 360, 1.0, 2, 2)],                       # pair shapes, then one halving
+    [(0, 60, 0.25, 1, 1)],                                               # one range, no binning
+])
+def test_reference_runner_sequence_matches_oracle(aa, oracle, cuda, ranges):
+    say(f"C++ drop-in sequence {ranges}")
+    _run_sequence(aa, oracle, cuda, _fns(aa), ranges)
+
+
+def test_runner_sequence_two_observations_same_geometry(aa, oracle, cuda):
+    """A second observation with the same channel count, DM plan and tsamp
+    but another band (foff) must not reuse the first one's delays."""
+    say("C++ drop-in sequence: two observations, foff -1.0 then -0.75")
+    f = _fns(aa)
+    ranges = [(0, 120, 0.5, 1, 1), (120, 360, 1.0, 2, 2)]
+    _run_sequence(aa, oracle, cuda, f, ranges, foff=-1.0, seed=9)
+    _run_sequence(aa, oracle, cuda, f, ranges, foff=-0.75, seed=10)
