@@ -206,6 +206,11 @@ int aa_py_current_range(void *obj) { return obj ? static_cast<py_pipeline *>(obj
 int aa_py_current_time_chunk(void *obj) { return obj ? static_cast<py_pipeline *>(obj)->pipe.get_current_tchunk() : 0; }
 long aa_py_current_inc(void *obj) { return obj ? static_cast<py_pipeline *>(obj)->pipe.get_current_inc() : 0; }
 bool aa_py_cleanup(void *obj) { return obj && static_cast<py_pipeline *>(obj)->pipe.cleanup(); }
+bool aa_hip_pipeline_set_ddtr_consumer(void *pipeline, aa_hip_ddtr_consumer fn, void *user) {
+    if (!pipeline) return false;
+    static_cast<py_pipeline *>(pipeline)->pipe.set_ddtr_consumer(fn, user);
+    return true;
+}
 
 // --------------------------------------------------------- strategy (host)
 void *aa_py_ddtr_strategy_create(void const *plan, aa_py_filterbank_metadata_struct metadata, size_t free_memory,
@@ -486,6 +491,23 @@ int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nch
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
     if (t) (void)hipFree(t);
     return from_hip(e, "dedisperse");
+}
+
+size_t aa_hip_spd_workspace_bytes(void) { return aa_hip::spd_workspace_bytes(); }
+
+int aa_hip_spd_first_pass(const float *d_plane, size_t pitch, int ndms, int nsamp, int max_width, float *d_msd,
+                          float *d_snr, size_t snr_pitch, uint16_t *d_width, size_t width_pitch, void *d_work,
+                          size_t work_bytes, void *stream) {
+    static_assert(AA_HIP_SPD_MAX_WIDTH == SPD_MAX_WIDTH, "one width limit");
+    if (!d_plane || !d_msd || !d_work) return bad("NULL buffer");
+    if (ndms <= 0 || nsamp <= 0 || pitch < (size_t)nsamp) return bad("invalid plane geometry");
+    if (max_width < 1 || max_width > AA_HIP_SPD_MAX_WIDTH || max_width > nsamp) return bad("invalid max_width");
+    if ((d_snr && snr_pitch < (size_t)nsamp) || (d_width && width_pitch < (size_t)nsamp)) return bad("invalid output pitch");
+    if (work_bytes < aa_hip::spd_workspace_bytes()) return bad("workspace too small");
+    if ((uintptr_t)d_work % 256 != 0) return bad("workspace not 256-byte aligned");
+    return from_hip(aa_hip::launch_spd_first_pass(d_plane, pitch, ndms, nsamp, max_width, d_msd, d_snr, snr_pitch,
+                                                  d_width, width_pitch, d_work, (hipStream_t)stream),
+                    "single-pulse first pass");
 }
 
 int aa_hip_synth_noise_u8(uint64_t seed, int64_t n, double mean, double sigma, uint8_t *d_out_tc, void *stream) {

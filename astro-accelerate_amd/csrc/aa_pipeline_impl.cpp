@@ -91,6 +91,11 @@ public:
 
     aa_ddtr_strategy ddtr_strategy() { return m_bound ? m_strategy : aa_ddtr_strategy(); }
 
+    void set_ddtr_consumer(aa_hip_ddtr_consumer fn, void *user) {
+        m_consumer = fn;
+        m_consumer_user = fn ? user : nullptr;
+    }
+
     bool has(aa_pipeline::component_option o) const { return m_options.find(o) != m_options.end(); }
 
     bool ready() {
@@ -571,6 +576,31 @@ private:
                 return false;
             }
         }
+        if (m_consumer) {   // where the reference's runner calls analysis_GPU on the plane
+            const auto &rd = m_strategy.dm((size_t)range);
+            aa_hip_ddtr_plane pl;
+            pl.range = range;
+            pl.time_chunk = chunk;
+            pl.d_plane = out;
+            pl.ndms = m_strategy.ndms((size_t)range);
+            pl.nsamp = tp;
+            pl.pitch = (size_t)tp;
+            pl.dm_low = rd.low;
+            pl.dm_step = rd.step;
+            pl.inBin = rd.inBin;
+            pl.tsamp = (double)((float)m_metadata.tsamp() * (float)rd.inBin);
+            pl.inc = m_inc;
+            pl.device = m_engine->device();
+            pl.stream = (void *)m_stream;
+            const int rc = m_consumer(&pl, m_consumer_user);
+            (void)hipSetDevice(m_engine->device());
+            if (rc != 0) {
+                LOG(log_level::error, "DDTR plane consumer failed (" + std::to_string(rc) + ") on range " +
+                                          std::to_string(range) + ", chunk " + std::to_string(chunk));
+                aa_hip::set_last_error("DDTR plane consumer returned an error");
+                return false;
+            }
+        }
         return true;
     }
 
@@ -608,6 +638,8 @@ private:
     bool m_in_norm = false;                // input DDTR normalisation of each chunk
     bool m_out_norm = false;               // output DDTR normalisation of each plane
     void *m_d_norm_scratch = nullptr;      // input normalisation's column sums
+    aa_hip_ddtr_consumer m_consumer = nullptr;   // hand-off of each plane (analysis_GPU's place)
+    void *m_consumer_user = nullptr;
     float ***m_host_out = nullptr;
     std::vector<float *> m_host_blocks;
     std::vector<size_t> m_host_total;
@@ -637,5 +669,6 @@ int aa_ddtr_pipeline::get_current_tchunk() { return m_impl->current_chunk(); }
 long int aa_ddtr_pipeline::get_current_inc() { return m_impl->current_inc(); }
 bool aa_ddtr_pipeline::cleanup() { return m_impl->cleanup(); }
 float ***aa_ddtr_pipeline::output_buffer() { return m_impl->output_buffer(); }
+void aa_ddtr_pipeline::set_ddtr_consumer(aa_hip_ddtr_consumer fn, void *user) { m_impl->set_ddtr_consumer(fn, user); }
 
 }  // namespace astroaccelerate

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <thread>
 
@@ -329,5 +330,47 @@ int aa_hip_sigproc_read_u16(aa_hip_sigproc *h, const uint16_t **data, size_t *co
 }
 
 void aa_hip_sigproc_close(aa_hip_sigproc *h) { delete h; }
+
+// The reference's Python FFI of the reader (src/aa_py_sigproc_input.cpp:6-33):
+// the object is an aa_sigproc_input; the buffer is the u16-widened data.
+void *aa_py_sigproc_input(char const *path)
+{
+    if (!path) return nullptr;
+    return new (std::nothrow) astroaccelerate::aa_sigproc_input(path);
+}
+
+void aa_py_sigproc_input_del(void const *obj)
+{
+    delete static_cast<astroaccelerate::aa_sigproc_input const *>(obj);
+}
+
+aa_py_filterbank_metadata_struct aa_py_sigproc_input_read_metadata(void *obj)
+{
+    aa_py_filterbank_metadata_struct out{};
+    if (!obj) return out;
+    const astroaccelerate::aa_filterbank_metadata m =
+        static_cast<astroaccelerate::aa_sigproc_input *>(obj)->read_metadata();
+    out.m_tstart = m.tstart();
+    out.m_tsamp = m.tsamp();
+    out.m_fch1 = m.fch1();
+    out.m_foff = m.foff();
+    out.m_nbits = m.nbits();
+    out.m_nsamples = m.nsamples();
+    out.m_nchans = m.nchans();
+    out.m_src_raj = m.src_raj();
+    return out;
+}
+
+bool aa_py_sigproc_input_read_signal(void *obj)
+{
+    return obj && static_cast<astroaccelerate::aa_sigproc_input *>(obj)->read_signal();
+}
+
+unsigned short const *aa_py_sigproc_input_input_buffer(void const *obj)
+{
+    if (!obj) return nullptr;
+    const auto &b = static_cast<astroaccelerate::aa_sigproc_input const *>(obj)->input_buffer();
+    return b.empty() ? nullptr : b.data();
+}
 
 }  // extern "C"

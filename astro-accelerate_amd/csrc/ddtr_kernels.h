@@ -84,6 +84,15 @@ hipError_t launch_dedisperse_direct_u16(const uint16_t* L, size_t pitch, int nch
                                         float mstartdm, float mdmstep, int inBin,
                                         float* out, size_t out_pitch, hipStream_t s);
 
+// First single-pulse-search pass on a DM-time plane (boxcar MSD per width
+// 1..W, then best boxcar SNR and its width per start sample); `work` holds
+// spd_workspace_bytes() device bytes.  snr/width may be null.
+#define SPD_MAX_WIDTH 32
+size_t spd_workspace_bytes();
+hipError_t launch_spd_first_pass(const float* plane, size_t pitch, int ndms, int nsamp, int W, float* msd,
+                                 float* snr, size_t snr_pitch, uint16_t* width, size_t width_pitch, void* work,
+                                 hipStream_t s);
+
 // Synthetic 8-bit filterbank noise: N(mean, sigma^2) rounded and clipped to
 // [0,255], SplitMix64 counter stream keyed by seed (value depends only on
 // seed and the flat [t][c] index).

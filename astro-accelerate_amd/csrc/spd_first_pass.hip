@@ -1,0 +1,268 @@
+// First pass of the single-pulse search on one DM-time plane, the stage the
+// reference's runner hands each range's device plane to
+// (include/aa_permitted_pipelines_generic.hpp:848-889 -> analysis_GPU,
+// src/aa_device_analysis.cu:126-133 reading [nDMs][nTimesamples]):
+//
+//  * boxcar MSD (the role of MSD_plane_profile, src/aa_device_MSD_plane_profile.cu):
+//    mean and population sd of every in-row boxcar sum B_w(t) = sum_{i<w}
+//    x[d][t+i], t + w <= nsamp, for w = 1..W (W <= 32), over the whole
+//    plane.  Exact per width (the reference interpolates between decimated
+//    widths).  One read of the plane.
+//  * boxcar SNR (the first plane of SPDT, src/aa_device_SPS_long_kernel.cu:42-168):
+//    snr[d][t] = max over widths that fit of (B_w(t) - mean_w) / sd_w and
+//    width[d][t] = that w (ties: the smaller w; sd_w == 0 gives SNR 0).
+//    One more read of the plane, one write of snr (float) and width (u16).
+//
+// Layout and numerics.  A tile is kTile consecutive start samples of one DM
+// row plus the W-1 samples after them, staged in LDS.  Sums run in float on
+// centred values (x - c), so a boxcar sum holds the noise and not the
+// plane's offset: the MSD kernel centres each tile on its own mean, turns
+// the tile's per-width float sums into (count, mean, centred second moment)
+// in double and merges them into its workgroup's running moments with
+// Chan's pairwise update (a persistent grid of kBlocks workgroups, tiles
+// assigned by stride); one finalize workgroup merges the workgroups' moments
+// in block order, so the result does not depend on scheduling.  The SNR
+// kernel centres on the plane mean c = mean_1 and divides
+// (B'_w - (mean_w - w c)) by sd_w.
+//
+// Cost per start sample and width: one LDS read and 3-4 VALU operations
+// (VALU-bound at W = 32; the plane is read twice from HBM).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "ddtr_kernels.h"
+
+namespace aa_hip {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kK = 32;                    // start samples per thread per tile
+constexpr int kTile = kThreads * kK;      // 8192 start samples of one row
+constexpr int kMaxW = SPD_MAX_WIDTH;      // 32
+constexpr int kBlocks = 1024;             // persistent MSD grid
+
+struct spd_aux {              // finalize -> SNR kernel
+    float c;                  // centring value: mean of width-1 sums (the plane mean)
+    float pad[3];
+    float off[kMaxW];         // mean_w - w * c
+    float rsd[kMaxW];         // 1 / sd_w (0 when sd_w == 0)
+};
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Chan et al.'s pairwise update: (n_a, m_a, M2_a) += (n_b, m_b, M2_b).
+__device__ __forceinline__ void chan_merge(double& na, double& ma, double& m2a, double nb, double mb, double m2b)
+{
+    if (nb <= 0.0) return;
+    const double n = na + nb;
+    const double delta = mb - ma;
+    ma += delta * (nb / n);
+    m2a += m2b + delta * delta * (na * nb / n);
+    na = n;
+}
+
+// Per-workgroup moments over its tiles: partial[blk][w] = (n_w, mean of B_w,
+// sum of squared deviations of B_w).
+__global__ __launch_bounds__(kThreads) void spd_msd_kernel(const float* __restrict__ plane, long pitch, int ndms,
+                                                           int nsamp, int W, double* __restrict__ partial)
+{
+    __shared__ float s[kTile + kMaxW];
+    __shared__ float red[kThreads / 64][2 * kMaxW];
+    __shared__ float wsum[kThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const long per_row = ((long)nsamp + kTile - 1) / kTile;
+    const long ntiles = (long)ndms * per_row;
+    double accn = 0.0, accm = 0.0, acc2 = 0.0;   // thread tid < W: width tid + 1
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long d = tile / per_row;
+        const long t0 = (tile - d * per_row) * kTile;
+        const float* row = plane + d * pitch;
+        const long rest = (long)nsamp - t0;
+        const int nout = (int)(rest < kTile ? rest : kTile);                 // start samples in the tile
+        const int nload = (int)(rest < kTile + W - 1 ? rest : kTile + W - 1);  // samples staged
+        __syncthreads();   // the previous tile's readers of s and red are done
+        float part = 0.0f;
+        for (int i = tid; i < kTile + kMaxW; i += kThreads) {
+            const float v = i < nload ? row[t0 + i] : 0.0f;
+            s[i] = v;
+            if (i < nout) part += v;
+        }
+        part = wave_sum(part);
+        if (lane == 0) wsum[wv] = part;
+        __syncthreads();
+        const float c = (wsum[0] + wsum[1] + wsum[2] + wsum[3]) / (float)nout;
+        for (int i = tid; i < nload; i += kThreads) s[i] -= c;
+        __syncthreads();
+        float a1[kMaxW], a2[kMaxW];
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w) a1[w] = a2[w] = 0.0f;
+        if (nload == kTile + W - 1) {   // every width fits at every start sample
+#pragma unroll 2
+            for (int k = 0; k < kK; ++k) {
+                const int j = tid + kThreads * k;
+                float B = 0.0f;
+#pragma unroll
+                for (int w = 0; w < kMaxW; ++w) {
+                    if (w < W) {
+                        B += s[j + w];
+                        a1[w] += B;
+                        a2[w] = __builtin_fmaf(B, B, a2[w]);
+                    }
+                }
+            }
+        } else {   // the row's last tile: widths that fit
+            for (int k = 0; k < kK; ++k) {
+                const int j = tid + kThreads * k;
+                if (j >= nout) break;
+                float B = 0.0f;
+#pragma unroll
+                for (int w = 0; w < kMaxW; ++w) {
+                    if (w < W && j + w < nload) {
+                        B += s[j + w];
+                        a1[w] += B;
+                        a2[w] = __builtin_fmaf(B, B, a2[w]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w) {
+            if (w < W) {
+                const float r1 = wave_sum(a1[w]);
+                const float r2 = wave_sum(a2[w]);
+                if (lane == 0) {
+                    red[wv][w] = r1;
+                    red[wv][kMaxW + w] = r2;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < W) {
+            const int w = tid + 1;
+            const double S1 = (double)red[0][tid] + (double)red[1][tid] + (double)red[2][tid] + (double)red[3][tid];
+            const double S2 = (double)red[0][kMaxW + tid] + (double)red[1][kMaxW + tid] +
+                              (double)red[2][kMaxW + tid] + (double)red[3][kMaxW + tid];
+            const long fit = (long)nload - w + 1;               // start samples whose width-w boxcar is staged
+            const double n = (double)(fit < nout ? (fit > 0 ? fit : 0) : nout);
+            if (n > 0.0) {   // the tile's moments of B_w = B'_w + w c
+                const double mt = S1 / n;
+                const double m2t = S2 - S1 * mt;
+                chan_merge(accn, accm, acc2, n, mt + (double)w * (double)c, m2t > 0.0 ? m2t : 0.0);
+            }
+        }
+    }
+    if (tid < W) {
+        double* p = partial + ((long)blockIdx.x * kMaxW + tid) * 3;
+        p[0] = accn;
+        p[1] = accm;
+        p[2] = acc2;
+    }
+}
+
+// Merges the workgroups' moments in block order; writes (mean_w, sd_w) and
+// the SNR kernel's centring constants.
+__global__ void spd_msd_finalize_kernel(const double* __restrict__ partial, int nblocks, int W,
+                                        float* __restrict__ msd, spd_aux* __restrict__ aux)
+{
+    __shared__ double mean[kMaxW], sd[kMaxW];
+    const int w = threadIdx.x;
+    if (w < W) {
+        double n = 0.0, m = 0.0, m2 = 0.0;
+        for (int b = 0; b < nblocks; ++b) {
+            const double* p = partial + ((long)b * kMaxW + w) * 3;
+            chan_merge(n, m, m2, p[0], p[1], p[2]);
+        }
+        const double v = n > 0.0 ? m2 / n : 0.0;
+        mean[w] = m;
+        sd[w] = v > 0.0 ? sqrt(v) : 0.0;
+        msd[2 * w] = (float)mean[w];
+        msd[2 * w + 1] = (float)sd[w];
+    }
+    __syncthreads();
+    if (w < W) {
+        const double c = (double)(float)mean[0];
+        if (w == 0) aux->c = (float)c;
+        aux->off[w] = (float)(mean[w] - (double)(w + 1) * c);
+        aux->rsd[w] = sd[w] > 0.0 ? (float)(1.0 / sd[w]) : 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void spd_snr_kernel(const float* __restrict__ plane, long pitch, int nsamp,
+                                                           int W, const spd_aux* __restrict__ aux,
+                                                           float* __restrict__ snr, long snr_pitch,
+                                                           uint16_t* __restrict__ width, long width_pitch)
+{
+    __shared__ float s[kTile + kMaxW];
+    __shared__ float off[kMaxW], rsd[kMaxW];
+    const int tid = threadIdx.x;
+    const long per_row = ((long)nsamp + kTile - 1) / kTile;
+    const long d = blockIdx.x / per_row;
+    const long t0 = (blockIdx.x - d * per_row) * kTile;
+    const float* row = plane + d * pitch;
+    const long rest = (long)nsamp - t0;
+    const int nout = (int)(rest < kTile ? rest : kTile);
+    const int nload = (int)(rest < kTile + W - 1 ? rest : kTile + W - 1);
+    const float c = aux->c;
+    if (tid < kMaxW) {
+        off[tid] = aux->off[tid];
+        rsd[tid] = aux->rsd[tid];
+    }
+    for (int i = tid; i < kTile + kMaxW; i += kThreads) s[i] = i < nload ? row[t0 + i] - c : 0.0f;
+    __syncthreads();
+    float* so = snr ? snr + d * snr_pitch + t0 : nullptr;
+    uint16_t* wo = width ? width + d * width_pitch + t0 : nullptr;
+    for (int k = 0; k < kK; ++k) {
+        const int j = tid + kThreads * k;
+        if (j >= nout) break;
+        float B = 0.0f, best = 0.0f;
+        int bw = 0;
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w) {
+            if (w < W && j + w < nload) {
+                B += s[j + w];
+                const float v = (B - off[w]) * rsd[w];
+                if (bw == 0 || v > best) {
+                    best = v;
+                    bw = w + 1;
+                }
+            }
+        }
+        if (so) so[j] = best;
+        if (wo) wo[j] = (uint16_t)bw;
+    }
+}
+
+}  // namespace
+
+size_t spd_workspace_bytes() { return (size_t)kBlocks * kMaxW * 3 * sizeof(double) + 256 + sizeof(spd_aux); }
+
+hipError_t launch_spd_first_pass(const float* plane, size_t pitch, int ndms, int nsamp, int W, float* msd,
+                                 float* snr, size_t snr_pitch, uint16_t* width, size_t width_pitch, void* work,
+                                 hipStream_t s)
+{
+    if (!plane || !msd || !work || ndms <= 0 || nsamp <= 0 || W < 1 || W > kMaxW || W > nsamp ||
+        pitch < (size_t)nsamp)
+        return hipErrorInvalidValue;
+    if ((snr && snr_pitch < (size_t)nsamp) || (width && width_pitch < (size_t)nsamp)) return hipErrorInvalidValue;
+    double* partial = static_cast<double*>(work);
+    spd_aux* aux = reinterpret_cast<spd_aux*>(static_cast<char*>(work) + (size_t)kBlocks * kMaxW * 3 * sizeof(double));
+    const long per_row = ((long)nsamp + kTile - 1) / kTile;
+    const long ntiles = (long)ndms * per_row;
+    if (ntiles > 0x7fffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(spd_msd_kernel, dim3(kBlocks), dim3(kThreads), 0, s, plane, (long)pitch, ndms, nsamp, W,
+                       partial);
+    hipLaunchKernelGGL(spd_msd_finalize_kernel, dim3(1), dim3(64), 0, s, (const double*)partial, kBlocks, W, msd, aux);
+    if (snr || width)
+        hipLaunchKernelGGL(spd_snr_kernel, dim3((unsigned)ntiles), dim3(kThreads), 0, s, plane, (long)pitch, nsamp,
+                           W, (const spd_aux*)aux, snr, (long)snr_pitch, width, (long)width_pitch);
+    return hipGetLastError();
+}
+
+}  // namespace aa_hip

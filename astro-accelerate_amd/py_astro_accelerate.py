@@ -141,6 +141,14 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_py_current_time_chunk": ([vp], i),
         "aa_py_current_inc": ([vp], ctypes.c_long),
         "aa_py_cleanup": ([vp], b),
+        "aa_hip_pipeline_set_ddtr_consumer": ([vp, vp, vp], b),
+        "aa_py_sigproc_input": ([ctypes.c_char_p], vp),
+        "aa_py_sigproc_input_del": ([vp], None),
+        "aa_py_sigproc_input_read_metadata": ([vp], filterbank_metadata_struct),
+        "aa_py_sigproc_input_read_signal": ([vp], b),
+        "aa_py_sigproc_input_input_buffer": ([vp], ctypes.POINTER(ctypes.c_ushort)),
+        "aa_hip_spd_workspace_bytes": ([], sz),
+        "aa_hip_spd_first_pass": ([vp, sz, i, i, i, vp, vp, sz, vp, sz, vp, sz, vp], i),
         "aa_py_ddtr_strategy_create": ([vp, filterbank_metadata_struct, sz, b], vp),
         "aa_py_ddtr_strategy_ready": ([vp], b),
         "aa_py_ddtr_strategy_maxshift": ([vp], i),
@@ -333,6 +341,19 @@ class aa_py_filterbank_metadata:
             pass
 
 
+class aa_hip_ddtr_plane(ctypes.Structure):
+    """One range's device DM-time plane handed to a pipeline's consumer
+    (aa_hip_ddtr_plane in include/aa_ddtr_hip.h)."""
+    _fields_ = [("range", ctypes.c_int), ("time_chunk", ctypes.c_int), ("d_plane", ctypes.c_void_p),
+                ("ndms", ctypes.c_int), ("nsamp", ctypes.c_int), ("pitch", ctypes.c_size_t),
+                ("dm_low", ctypes.c_float), ("dm_step", ctypes.c_float), ("inBin", ctypes.c_int),
+                ("tsamp", ctypes.c_double), ("inc", ctypes.c_long), ("device", ctypes.c_int),
+                ("stream", ctypes.c_void_p)]
+
+
+DDTR_CONSUMER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(aa_hip_ddtr_plane), ctypes.c_void_p)
+
+
 class aa_py_pipeline:
     """The reference's aa_py_pipeline for the dedispersion component.
 
@@ -426,6 +447,27 @@ class aa_py_pipeline:
     def cleanUp(self):
         return bool(_lib_or_raise().aa_py_cleanup(self.m_obj))
 
+    def set_ddtr_consumer(self, fn):
+        """fn(plane: aa_hip_ddtr_plane) -> int, called once per (time chunk,
+        range) with the range's device plane, where the reference's runner
+        calls analysis_GPU; non-zero stops the pipeline.  None clears it."""
+        lib = _lib_or_raise()
+        if fn is None:
+            self._consumer = None
+            return bool(lib.aa_hip_pipeline_set_ddtr_consumer(self.m_obj, None, None))
+
+        def trampoline(pl, _user):
+            try:
+                return int(fn(pl.contents))
+            except Exception:   # an exception cannot cross the C frames: report it as a failure
+                import traceback
+                traceback.print_exc()
+                return -1
+
+        self._consumer = DDTR_CONSUMER(trampoline)   # kept alive with the pipeline
+        return bool(lib.aa_hip_pipeline_set_ddtr_consumer(self.m_obj, ctypes.cast(self._consumer, ctypes.c_void_p),
+                                                          None))
+
     def output_planes(self, strategy: "aa_py_ddtr_strategy") -> List[np.ndarray]:
         """Copies of the host output as numpy [ndms][total_t] arrays per range."""
         _, buf = self.get_buffer()
@@ -507,6 +549,43 @@ class aa_py_ddtr_strategy:
         low, _, step, ib, _ = self.dm(r)
         tsamp_r = np.float32(np.float32(self.tsamp) * np.float32(ib))
         return np.float32(np.float32(low) / tsamp_r), np.float32(np.float32(step) / tsamp_r)
+
+
+class aa_py_sigproc_input:
+    """The reference's Python SIGPROC reader (cmake/py_astro_accelerate.py.in,
+    class aa_py_sigproc_input, over src/aa_py_sigproc_input.cpp):
+    read_metadata() -> filterbank_metadata_struct, read_signal() -> bool,
+    input_buffer() -> ctypes POINTER(c_ushort) to nsamples x nchans u16
+    samples in [t][c] order, ready for aa_py_pipeline."""
+
+    def __init__(self, path: str):
+        self.m_obj = _lib_or_raise().aa_py_sigproc_input(ctypes.c_char_p(path.encode("utf-8")))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def close(self):
+        if getattr(self, "m_obj", None):
+            _lib_or_raise().aa_py_sigproc_input_del(self.m_obj)
+            self.m_obj = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def read_metadata(self) -> filterbank_metadata_struct:
+        return _lib_or_raise().aa_py_sigproc_input_read_metadata(self.m_obj)
+
+    def read_signal(self) -> bool:
+        return bool(_lib_or_raise().aa_py_sigproc_input_read_signal(self.m_obj))
+
+    def input_buffer(self):
+        return _lib_or_raise().aa_py_sigproc_input_input_buffer(self.m_obj)
 
 
 class aa_sigproc_input:
@@ -828,6 +907,48 @@ def output_ddtr_normalization(d_plane, stream=None):
     ndms, nsamp = d_plane.shape
     check(_lib_or_raise().aa_hip_output_ddtr_normalization(_ptr(d_plane), int(ndms), int(nsamp),
                                                            _stream_ptr(stream)), "output_ddtr_normalization")
+
+
+SPD_MAX_WIDTH = 32   # AA_HIP_SPD_MAX_WIDTH
+
+
+def spd_workspace(device=None):
+    """A device workspace for spd_first_pass (uint8 tensor, 256-byte aligned)."""
+    import torch
+    n = int(_lib_or_raise().aa_hip_spd_workspace_bytes())
+    return torch.empty(n, dtype=torch.uint8, device=device if device is not None else "cuda")
+
+
+def spd_first_pass_raw(d_plane_ptr: int, pitch: int, ndms: int, nsamp: int, max_width: int, d_msd, d_snr=None,
+                       d_width=None, work=None, stream_ptr=None):
+    """aa_hip_spd_first_pass on a raw device plane pointer (e.g. a consumer's
+    aa_hip_ddtr_plane); outputs are torch tensors: d_msd float32 [max_width][2],
+    d_snr float32 [ndms][>= nsamp], d_width int16 [ndms][>= nsamp]."""
+    if work is None:
+        work = spd_workspace(d_msd.device)
+    st = ctypes.c_void_p(stream_ptr) if stream_ptr is not None else _stream_ptr(None)
+    check(_lib_or_raise().aa_hip_spd_first_pass(
+        ctypes.c_void_p(d_plane_ptr), int(pitch), int(ndms), int(nsamp), int(max_width), _ptr(d_msd),
+        _ptr(d_snr) if d_snr is not None else None, d_snr.stride(0) if d_snr is not None else 0,
+        _ptr(d_width) if d_width is not None else None, d_width.stride(0) if d_width is not None else 0,
+        _ptr(work), work.numel(), st), "spd_first_pass")
+
+
+def spd_first_pass(d_plane, max_width: int = SPD_MAX_WIDTH, with_snr: bool = True, work=None, stream=None):
+    """First single-pulse-search pass on a float32 [ndms][nsamp] device plane:
+    returns (msd [max_width][2] = per-width mean and sd of the boxcar sums,
+    snr [ndms][nsamp] float32, width [ndms][nsamp] int16) -- snr and width
+    None when with_snr is False."""
+    import torch
+    ndms, nsamp = d_plane.shape
+    dev = d_plane.device
+    msd = torch.empty((max_width, 2), dtype=torch.float32, device=dev)
+    snr = torch.empty((ndms, nsamp), dtype=torch.float32, device=dev) if with_snr else None
+    width = torch.empty((ndms, nsamp), dtype=torch.int16, device=dev) if with_snr else None
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    spd_first_pass_raw(d_plane.data_ptr(), d_plane.stride(0), ndms, nsamp, max_width, msd, snr, width, work,
+                       s.cuda_stream)
+    return msd, snr, width
 
 
 def bandpass_average(data_tc: np.ndarray) -> np.ndarray:

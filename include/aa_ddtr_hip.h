@@ -112,9 +112,13 @@ typedef struct {
     bool fdas;
 } pipeline_components_struct;
 
-/* Same layout as include/aa_py_pipeline_api.hpp:32-48.  Only
- * copy_ddtr_data_to_host is honoured; every other option makes bind/ready
- * fail with AA_HIP_ERROR_UNSUPPORTED recorded in aa_hip_last_error(). */
+/* Same layout as include/aa_py_pipeline_api.hpp:32-48.  Honoured:
+ * zero_dm, zero_dm_with_outliers, input_DDTR_normalization,
+ * output_DDTR_normalization, set_bandpass_average and
+ * copy_ddtr_data_to_host, each as the reference's runner applies it.
+ * Refused: old_rfi (run() fails, "unsupported pipeline option" in
+ * aa_hip_last_error()).  The others concern output files or components
+ * outside dedispersion and are ignored (logged). */
 typedef struct {
     bool zero_dm;
     bool zero_dm_with_outliers;
@@ -323,6 +327,62 @@ int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nch
                          float mstartdm, float mdmstep, int inBin, float *d_out, size_t out_pitch,
                          int kernel, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Hand-off of DM-time planes to the single-pulse search and its first pass.
+ *
+ * The reference's runner passes each range's device plane d_DDTR_output,
+ * [nDMs][nTimesamples] floats, to analysis_GPU right after dedispersing it
+ * (include/aa_permitted_pipelines_generic.hpp:848-889; the plane is read at
+ * src/aa_device_analysis.cu:126-133).  Here a pipeline hands every range's
+ * plane to a consumer callback with its strides, and aa_hip_spd_first_pass
+ * is the first stage such a consumer runs.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int range;              /* DM range */
+    int time_chunk;         /* time chunk */
+    float *d_plane;         /* device plane: d_plane[dm * pitch + t] */
+    int ndms;               /* nDMs of the range */
+    int nsamp;              /* nTimesamples: t_processed[range][time_chunk] */
+    size_t pitch;           /* floats between DM rows */
+    float dm_low, dm_step;  /* strategy-redefined DM grid of the range */
+    int inBin;              /* binning of the range */
+    double tsamp;           /* sampling time of the plane (metadata tsamp x inBin) */
+    long inc;               /* first input sample of the chunk (column inc / inBin of the range) */
+    int device;             /* the device holding the plane (current during the call) */
+    void *stream;           /* hipStream_t the plane was written on */
+} aa_hip_ddtr_plane;
+/* Called once per (time chunk, range), after the range's dedispersion and
+ * output normalisation are enqueued and before its host copy.  Work the
+ * consumer enqueues on plane->stream is ordered before the copy and before
+ * the buffer is reused; work on other streams must wait for plane->stream
+ * and finish before the callback returns.  A non-zero return stops the
+ * pipeline with an error (as a failed analysis_GPU does). */
+typedef int (*aa_hip_ddtr_consumer)(const aa_hip_ddtr_plane *plane, void *user);
+/* No reference counterpart over the FFI (the reference calls analysis_GPU
+ * itself): sets (fn != NULL) or clears the consumer of a pipeline from
+ * aa_py_pipeline_api / _u8 / aa_hip_py_pipeline_api_u8_capped. */
+bool aa_hip_pipeline_set_ddtr_consumer(void *pipeline, aa_hip_ddtr_consumer fn, void *user);
+
+/* First pass of the single-pulse search on one plane (d_plane[dm * pitch + t],
+ * ndms x nsamp floats).  For widths w = 1..max_width (max_width <=
+ * AA_HIP_SPD_MAX_WIDTH and <= nsamp) and every start sample t with t + w <=
+ * nsamp, B_w(t) = sum of x[dm][t .. t+w-1]:
+ *   d_msd[2(w-1)], d_msd[2(w-1)+1] = mean and population sd of all B_w over
+ *     the plane (the per-width MSD that src/aa_device_MSD_plane_profile.cu
+ *     estimates; exact here), float2 per width as the reference's d_MSD;
+ *   d_snr[dm * snr_pitch + t] = max over the widths that fit at t of
+ *     (B_w(t) - mean_w) / sd_w, and d_width[dm * width_pitch + t] = that w
+ *     (the first boxcar plane of src/aa_device_SPS_long_kernel.cu:42-168;
+ *     ties go to the smaller width, sd_w == 0 gives SNR 0).
+ * d_snr and d_width may be NULL (MSD only).  d_work: device scratch of
+ * aa_hip_spd_workspace_bytes() bytes, 256-byte aligned.  Asynchronous on
+ * `stream`; the result does not depend on scheduling. */
+#define AA_HIP_SPD_MAX_WIDTH 32
+size_t aa_hip_spd_workspace_bytes(void);
+int aa_hip_spd_first_pass(const float *d_plane, size_t pitch, int ndms, int nsamp, int max_width, float *d_msd,
+                          float *d_snr, size_t snr_pitch, uint16_t *d_width, size_t width_pitch, void *d_work,
+                          size_t work_bytes, void *stream);
+
 /* Synthetic input on the device (no reference counterpart; benchmarks and
  * tests).  Noise: N(mean, sigma^2) rounded to nearest and clipped to
  * [0,255], a SplitMix64 counter stream keyed by `seed` over the flat [t][c]
@@ -473,6 +533,22 @@ int aa_hip_sigproc_open(const char *path, aa_hip_sigproc **out, aa_hip_filterban
 int aa_hip_sigproc_read_u8(aa_hip_sigproc *h, int pinned, const uint8_t **data, size_t *bytes);
 int aa_hip_sigproc_read_u16(aa_hip_sigproc *h, const uint16_t **data, size_t *count);
 void aa_hip_sigproc_close(aa_hip_sigproc *h);
+
+/* The reference's Python FFI of the SIGPROC reader: an aa_sigproc_input
+ * object (include/aa_sigproc_input.hpp) behind a void pointer.
+ * Replaces src/aa_py_sigproc_input.cpp:6-8 (NULL on a NULL path). */
+void *aa_py_sigproc_input(char const *path);
+/* Replaces src/aa_py_sigproc_input.cpp:10-12. */
+void aa_py_sigproc_input_del(void const *obj);
+/* Replaces src/aa_py_sigproc_input.cpp:14-25 (also fills m_src_raj, which
+ * the reference leaves unset); all zero when the header cannot be read. */
+aa_py_filterbank_metadata_struct aa_py_sigproc_input_read_metadata(void *obj);
+/* Replaces src/aa_py_sigproc_input.cpp:27-29: reads the data widened to u16
+ * ([t][c], any nbits); false if called before read_metadata or twice. */
+bool aa_py_sigproc_input_read_signal(void *obj);
+/* Replaces src/aa_py_sigproc_input.cpp:31-33: nsamples x nchans u16 samples
+ * (NULL before read_signal); valid until aa_py_sigproc_input_del. */
+unsigned short const *aa_py_sigproc_input_input_buffer(void const *obj);
 
 #ifdef __cplusplus
 }

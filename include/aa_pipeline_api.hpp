@@ -16,6 +16,7 @@
 #include <memory>
 #include <type_traits>
 
+#include "aa_ddtr_hip.h"
 #include "aa_ddtr_plan.hpp"
 #include "aa_ddtr_strategy.hpp"
 #include "aa_device_info.hpp"
@@ -54,6 +55,10 @@ public:
     long int get_current_inc();
     bool cleanup();
     float ***output_buffer();
+    // Hands every (time chunk, range) plane to fn on the device, where the
+    // reference's runner calls analysis_GPU
+    // (include/aa_permitted_pipelines_generic.hpp:848-889); NULL clears it.
+    void set_ddtr_consumer(aa_hip_ddtr_consumer fn, void *user);
 
 private:
     std::unique_ptr<aa_ddtr_pipeline_impl> m_impl;
@@ -88,6 +93,9 @@ public:
     long int get_current_inc() { return m_p.get_current_inc(); }
     bool cleanup() { return m_p.cleanup(); }
     float ***output_buffer() { return m_p.output_buffer(); }
+    // No reference counterpart: the device plane of every (time chunk, range)
+    // goes to fn (see aa_hip_ddtr_consumer), the single-pulse search's entry.
+    void set_ddtr_consumer(aa_hip_ddtr_consumer fn, void *user) { m_p.set_ddtr_consumer(fn, user); }
     // Not implemented in the reference either (include/aa_pipeline_api.hpp:1188-1195).
     bool handoff(aa_pipeline_api &) { return true; }
 
