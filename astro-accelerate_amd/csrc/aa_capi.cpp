@@ -379,6 +379,20 @@ int aa_hip_corner_turn_u16(const uint16_t *d_in_tc, int nsamp, int nchans, uint8
         "corner turn");
 }
 
+int aa_hip_corner_turn_levels(const void *d_in_tc, int sample_bytes, int nsamp, int nchans,
+                              uint8_t *const *d_levels, const size_t *pitches, int nlev, int *d_err, int *fused,
+                              void *stream) {
+    if (!d_in_tc || !d_levels || !pitches) return bad("NULL buffer");
+    if (nlev < 0 || nlev > aa_hip::kCtMaxLevels) return bad("nlev must be 0..3");
+    if (sample_bytes == 2 && !d_err) return bad("u16 input needs an error flag");
+    int done = 0;
+    const int st = from_hip(aa_hip::launch_corner_turn_levels(d_in_tc, sample_bytes, nsamp, nchans, d_levels, pitches,
+                                                              nlev, d_err, (hipStream_t)stream, &done),
+                            "corner turn");
+    if (fused) *fused = done;
+    return st;
+}
+
 int aa_hip_bin2_u8(const uint8_t *d_in, size_t in_pitch, int nchans, int n_out, uint8_t *d_out, size_t out_pitch,
                    void *stream) {
     if (!d_in || !d_out) return bad("NULL buffer");

@@ -249,21 +249,24 @@ int ddtr_engine::ingest(const void *d_input_tc, int input_bits, int t_processed0
     const long len0 = (long)t_processed0 + m_maxshift;
     if (t_processed0 <= 0 || len0 > m_levels[0].capacity)
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "t_processed0 exceeds the handle's capacity");
+    if (input_bits != 8 && input_bits != 16) return fail(AA_HIP_ERROR_UNSUPPORTED, "input_bits must be 8 or 16");
     device_guard guard(m_device);
-    hipError_t e;
-    if (input_bits == 8) {
-        e = launch_corner_turn_u8((const uint8_t *)d_input_tc, (int)len0, m_nchans, m_levels[0].ptr,
-                                  m_levels[0].pitch, s);
-    } else if (input_bits == 16) {
-        e = launch_corner_turn_u16((const uint16_t *)d_input_tc, (int)len0, m_nchans, m_levels[0].ptr,
-                                   m_levels[0].pitch, m_d_err, s);
-    } else {
-        return fail(AA_HIP_ERROR_UNSUPPORTED, "input_bits must be 8 or 16");
+    // The corner turn writes the binned levels the plan needs in the same
+    // pass when the shapes allow it; otherwise ensure_level bins on demand.
+    const int nlev = std::min((int)m_levels.size() - 1, kCtMaxLevels);
+    uint8_t *ptrs[kCtMaxLevels + 1];
+    size_t pitches[kCtMaxLevels + 1];
+    for (int k = 0; k <= nlev; ++k) {
+        ptrs[k] = m_levels[(size_t)k].ptr;
+        pitches[k] = m_levels[(size_t)k].pitch;
     }
+    int fused = 0;
+    const hipError_t e = launch_corner_turn_levels(d_input_tc, input_bits / 8, len0, m_nchans, ptrs, pitches, nlev,
+                                                   m_d_err, s, &fused);
     if (e != hipSuccess) return hip_fail(e, "corner turn");
     for (auto &lv : m_levels) lv.len = 0;
-    m_levels[0].len = len0;
-    m_resident = 0;
+    for (int k = 0; k <= fused; ++k) m_levels[(size_t)k].len = len0 >> k;
+    m_resident = fused;
     return AA_HIP_SUCCESS;
 }
 

@@ -31,6 +31,16 @@ hipError_t launch_corner_turn_u8(const uint8_t* in_tc, int nsamp, int nchans,
 hipError_t launch_corner_turn_u16(const uint16_t* in_tc, int nsamp, int nchans,
                                   uint8_t* out_ct, size_t out_pitch, int* d_err,
                                   hipStream_t s);
+// [t][c] (u8, or u16 carrying 8-bit samples: sample_bytes 1 or 2) -> level 0
+// [c][t] at ptrs[0], plus the x2 floor-average levels 1..nlev (nlev <= 3)
+// L_k[t] = (L_{k-1}[2t] + L_{k-1}[2t+1]) >> 1, t < nsamp >> k, in the same
+// pass.  Rows of level k are pitches[k] bytes apart.  *fused = the number of
+// binned levels written: nlev on the wide path (nchans a multiple of 16,
+// 16-byte aligned input, rows and pitches), else 0 (level 0 only).
+constexpr int kCtMaxLevels = 3;
+hipError_t launch_corner_turn_levels(const void* in_tc, int sample_bytes, long nsamp, int nchans,
+                                     uint8_t* const* ptrs, const size_t* pitches, int nlev, int* d_err,
+                                     hipStream_t s, int* fused);
 // u16 [t][c] -> u16 [c][t] (the reference's in-place corner turn, via temp).
 hipError_t launch_transpose_u16(const uint16_t* in, int rows, int cols,
                                 uint16_t* out, hipStream_t s);

@@ -63,6 +63,146 @@ __global__ __launch_bounds__(256) void corner_turn_kernel(const Tin* __restrict_
     }
 }
 
+// Wide corner turn fused with the x2 binning cascade.  A 256-thread
+// work-group turns a tile of 256 samples x 128 channels:
+//  * load: lane (cq = lane & 7, tg = lane >> 3) reads samples 8 tg .. 8 tg + 7
+//    of channels 16 cq .. 16 cq + 15 with eight 16-byte loads (one 128-byte
+//    line of each [t][c] row per 8 lanes);
+//  * in registers: each 4 x 4 byte block (4 samples x 4 channels) is
+//    transposed with 8 v_perm, so a channel's 8 samples are two dwords;
+//  * LDS [c][t] (row stride kCtRow bytes): one 8-byte write per channel;
+//  * store: lane (tq = lane & 15, cr = lane >> 4) writes 16 samples of
+//    channels cr + 16 j (j < 8) to level 0 with 16-byte stores (16 lanes =
+//    256 contiguous bytes of a [c][t] row), and their floor averages to the
+//    next NLEV levels (8, 4, 2 bytes per lane), each level k holding
+//    L_k[t] = floor((L_{k-1}[2t] + L_{k-1}[2t+1]) / 2) for t < nsamp >> k.
+// HBM traffic: every input byte read once, every level byte written once.
+constexpr int kCtT = 256, kCtC = 128, kCtRow = kCtT + 8;
+
+struct ct_levels {
+    uint8_t* ptr[4];
+    size_t pitch[4];
+};
+
+__device__ __forceinline__ uint32_t perm_b32(uint32_t hi, uint32_t lo, uint32_t sel)
+{
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
+
+// floor averages of adjacent bytes of a and b -> 4 bytes (a's two, then b's two)
+__device__ __forceinline__ uint32_t halve_pairs(uint32_t a, uint32_t b)
+{
+    const uint32_t sa = (((a & 0x00FF00FFu) + ((a >> 8) & 0x00FF00FFu)) >> 1) & 0x00FF00FFu;
+    const uint32_t sb = (((b & 0x00FF00FFu) + ((b >> 8) & 0x00FF00FFu)) >> 1) & 0x00FF00FFu;
+    return perm_b32(sb, sa, 0x06040200u);
+}
+
+// 16 bytes of channel data from a load of Tin samples; u16 values above 255
+// set *bad.
+template <typename Tin>
+__device__ __forceinline__ uint4 ct_load16(const Tin* p, bool& bad);
+
+template <>
+__device__ __forceinline__ uint4 ct_load16<uint8_t>(const uint8_t* p, bool&)
+{
+    return *reinterpret_cast<const uint4*>(p);
+}
+
+template <>
+__device__ __forceinline__ uint4 ct_load16<uint16_t>(const uint16_t* p, bool& bad)
+{
+    const uint4 a = reinterpret_cast<const uint4*>(p)[0];
+    const uint4 b = reinterpret_cast<const uint4*>(p)[1];
+    bad |= ((a.x | a.y | a.z | a.w | b.x | b.y | b.z | b.w) & 0xFF00FF00u) != 0u;
+    return make_uint4(perm_b32(a.y, a.x, 0x06040200u), perm_b32(a.w, a.z, 0x06040200u),
+                      perm_b32(b.y, b.x, 0x06040200u), perm_b32(b.w, b.z, 0x06040200u));
+}
+
+template <typename Tin, int NLEV>
+__global__ __launch_bounds__(256) void corner_turn_wide_kernel(const Tin* __restrict__ in, long nsamp, int nchans,
+                                                               ct_levels lv, int* __restrict__ err)
+{
+    __shared__ __align__(16) uint8_t tile[kCtC * kCtRow];
+    const int tid = threadIdx.x;
+    const long t0 = (long)blockIdx.x * kCtT;
+    const int c0 = blockIdx.y * kCtC;
+    {
+        const int cq = tid & 7, tg = tid >> 3;
+        const int c = c0 + 16 * cq;
+        const long tb = t0 + 8 * tg;
+        bool bad = false;
+        uint4 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            v[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (c < nchans && tb + k < nsamp) v[k] = ct_load16<Tin>(in + (size_t)(tb + k) * (size_t)nchans + c, bad);
+        }
+        if (sizeof(Tin) > 1 && bad) atomicOr(err, 1);
+        // word q of v[k] holds channels 4q..4q+3 of sample k
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t a[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a[k] = q == 0 ? v[k].x : q == 1 ? v[k].y : q == 2 ? v[k].z : v[k].w;
+            uint32_t lo[4], hi[4];   // channel 4q + j: samples 0..3 in lo[j], 4..7 in hi[j]
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t* A = a + 4 * h;
+                const uint32_t p01l = perm_b32(A[1], A[0], 0x05010400u), p01h = perm_b32(A[1], A[0], 0x07030602u);
+                const uint32_t p23l = perm_b32(A[3], A[2], 0x05010400u), p23h = perm_b32(A[3], A[2], 0x07030602u);
+                uint32_t* B = h == 0 ? lo : hi;
+                B[0] = perm_b32(p23l, p01l, 0x05040100u);
+                B[1] = perm_b32(p23l, p01l, 0x07060302u);
+                B[2] = perm_b32(p23h, p01h, 0x05040100u);
+                B[3] = perm_b32(p23h, p01h, 0x07060302u);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                *reinterpret_cast<uint2*>(tile + (16 * cq + 4 * q + j) * kCtRow + 8 * tg) = make_uint2(lo[j], hi[j]);
+        }
+    }
+    __syncthreads();
+    const int tq = tid & 15, cr = tid >> 4;
+    const long t = t0 + 16 * tq;
+#pragma unroll 2
+    for (int j = 0; j < 8; ++j) {
+        const int cl = cr + 16 * j;
+        const int c = c0 + cl;
+        if (c >= nchans || t >= nsamp) continue;
+        const uint8_t* src = tile + cl * kCtRow + 16 * tq;
+        const uint2 w01 = *reinterpret_cast<const uint2*>(src);
+        const uint2 w23 = *reinterpret_cast<const uint2*>(src + 8);
+        uint32_t w[4] = {w01.x, w01.y, w23.x, w23.y};
+        uint8_t* o0 = lv.ptr[0] + (size_t)c * lv.pitch[0] + (size_t)t;
+        if (t + 16 <= nsamp) {
+            *reinterpret_cast<uint4*>(o0) = make_uint4(w[0], w[1], w[2], w[3]);
+        } else {
+            for (int i = 0; i < (int)(nsamp - t); ++i) o0[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+        }
+        // level k: 16 >> k outputs at t >> k, valid below nsamp >> k
+#pragma unroll
+        for (int k = 1; k <= NLEV; ++k) {
+            if (k == 1) {
+                w[0] = halve_pairs(w[0], w[1]);
+                w[1] = halve_pairs(w[2], w[3]);
+            } else {
+                w[0] = halve_pairs(w[0], w[1]);   // k = 3: w[1] is 0, the result's low half
+                w[1] = 0u;
+            }
+            const long tk = t >> k;
+            const long nk = nsamp >> k;
+            uint8_t* ok = lv.ptr[k] + (size_t)c * lv.pitch[k] + (size_t)tk;
+            if (tk + (16 >> k) <= nk) {
+                if (k == 1) *reinterpret_cast<uint2*>(ok) = make_uint2(w[0], w[1]);
+                else if (k == 2) *reinterpret_cast<uint32_t*>(ok) = w[0];
+                else *reinterpret_cast<uint16_t*>(ok) = (uint16_t)w[0];
+            } else {
+                for (int i = 0; i < (int)(nk - tk); ++i) ok[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void transpose_u16_kernel(const uint16_t* __restrict__ in, int rows,
                                                             int cols, uint16_t* __restrict__ out)
 {
@@ -339,26 +479,76 @@ __global__ __launch_bounds__(256) void zero_dm_kernel(T* data, int nsamp, int nc
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
+template <typename Tin>
+static void launch_ct_wide(const Tin* in, long nsamp, int nchans, const ct_levels& lv, int nlev, int* d_err,
+                           hipStream_t s)
+{
+    const dim3 grid(ceil_div(nsamp, kCtT), ceil_div(nchans, kCtC));
+    switch (nlev) {
+    case 0: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 0>), grid, dim3(256), 0, s, in, nsamp, nchans, lv, d_err); break;
+    case 1: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 1>), grid, dim3(256), 0, s, in, nsamp, nchans, lv, d_err); break;
+    case 2: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 2>), grid, dim3(256), 0, s, in, nsamp, nchans, lv, d_err); break;
+    default: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 3>), grid, dim3(256), 0, s, in, nsamp, nchans, lv, d_err); break;
+    }
+}
+
+static bool ct_wide_fits(const void* in, int nchans, uint8_t* const* ptrs, const size_t* pitches, int nlev)
+{
+    if (nchans % 16 != 0 || (uintptr_t)in % 16 != 0) return false;
+    for (int k = 0; k <= nlev; ++k)
+        if ((uintptr_t)ptrs[k] % 16 != 0 || pitches[k] % 16 != 0) return false;
+    return true;
+}
+
+hipError_t launch_corner_turn_levels(const void* in_tc, int sample_bytes, long nsamp, int nchans,
+                                     uint8_t* const* ptrs, const size_t* pitches, int nlev, int* d_err,
+                                     hipStream_t s, int* fused)
+{
+    *fused = 0;
+    if (nsamp < 0 || nchans <= 0 || nlev < 0 || nlev > kCtMaxLevels || !in_tc || !ptrs || !pitches ||
+        (sample_bytes != 1 && sample_bytes != 2) || (sample_bytes == 2 && !d_err) || nsamp > 0x7fffffffL)
+        return hipErrorInvalidValue;
+    for (int k = 0; k <= nlev; ++k)
+        if (!ptrs[k] || (size_t)(nsamp >> k) > pitches[k]) return hipErrorInvalidValue;
+    if (nsamp == 0) {
+        *fused = nlev;
+        return hipSuccess;
+    }
+    if (!ct_wide_fits(in_tc, nchans, ptrs, pitches, nlev)) {
+        const dim3 grid(ceil_div(nsamp, kTile), ceil_div(nchans, kTile));
+        if (sample_bytes == 1)
+            hipLaunchKernelGGL(corner_turn_kernel<uint8_t>, grid, dim3(256), 0, s, (const uint8_t*)in_tc, (int)nsamp,
+                               nchans, ptrs[0], pitches[0], (int*)nullptr);
+        else
+            hipLaunchKernelGGL(corner_turn_kernel<uint16_t>, grid, dim3(256), 0, s, (const uint16_t*)in_tc,
+                               (int)nsamp, nchans, ptrs[0], pitches[0], d_err);
+        return hipGetLastError();
+    }
+    ct_levels lv{};
+    for (int k = 0; k <= nlev; ++k) {
+        lv.ptr[k] = ptrs[k];
+        lv.pitch[k] = pitches[k];
+    }
+    if (sample_bytes == 1) launch_ct_wide((const uint8_t*)in_tc, nsamp, nchans, lv, nlev, d_err, s);
+    else launch_ct_wide((const uint16_t*)in_tc, nsamp, nchans, lv, nlev, d_err, s);
+    *fused = nlev;
+    return hipGetLastError();
+}
+
 hipError_t launch_corner_turn_u8(const uint8_t* in_tc, int nsamp, int nchans, uint8_t* out_ct,
                                  size_t out_pitch, hipStream_t s)
 {
     if (nsamp < 0 || nchans <= 0 || (size_t)nsamp > out_pitch) return hipErrorInvalidValue;
-    if (nsamp == 0) return hipSuccess;
-    dim3 grid(ceil_div(nsamp, kTile), ceil_div(nchans, kTile));
-    hipLaunchKernelGGL(corner_turn_kernel<uint8_t>, grid, dim3(256), 0, s, in_tc, nsamp, nchans,
-                       out_ct, out_pitch, (int*)nullptr);
-    return hipGetLastError();
+    int fused = 0;
+    return launch_corner_turn_levels(in_tc, 1, nsamp, nchans, &out_ct, &out_pitch, 0, nullptr, s, &fused);
 }
 
 hipError_t launch_corner_turn_u16(const uint16_t* in_tc, int nsamp, int nchans, uint8_t* out_ct,
                                   size_t out_pitch, int* d_err, hipStream_t s)
 {
     if (nsamp < 0 || nchans <= 0 || (size_t)nsamp > out_pitch || !d_err) return hipErrorInvalidValue;
-    if (nsamp == 0) return hipSuccess;
-    dim3 grid(ceil_div(nsamp, kTile), ceil_div(nchans, kTile));
-    hipLaunchKernelGGL(corner_turn_kernel<uint16_t>, grid, dim3(256), 0, s, in_tc, nsamp, nchans,
-                       out_ct, out_pitch, d_err);
-    return hipGetLastError();
+    int fused = 0;
+    return launch_corner_turn_levels(in_tc, 2, nsamp, nchans, &out_ct, &out_pitch, 0, d_err, s, &fused);
 }
 
 hipError_t launch_transpose_u16(const uint16_t* in, int rows, int cols, uint16_t* out, hipStream_t s)

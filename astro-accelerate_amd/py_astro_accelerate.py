@@ -170,6 +170,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_corner_turn_u8": ([vp, i, i, vp, sz, vp], i),
         "aa_hip_corner_turn_u16": ([vp, i, i, vp, sz, vp, vp], i),
         "aa_hip_bin2_u8": ([vp, sz, i, i, vp, sz, vp], i),
+        "aa_hip_corner_turn_levels": ([vp, i, i, i, ctypes.POINTER(vp), ctypes.POINTER(sz), i, vp, pi, vp], i),
         "aa_hip_dedisperse_u8": ([vp, sz, i, i, PFLOAT, vp, i, i, f, f, i, vp, sz, i, vp], i),
         "aa_hip_synth_noise_u8": ([u64, i64, d, d, vp, vp], i),
         "aa_hip_inject_pulse_u8": ([vp, i, i, vp, d, d, i, i, i, vp], i),
@@ -853,6 +854,20 @@ def corner_turn_u16(d_in_tc, nsamp: int, nchans: int, d_out, out_pitch: int, d_e
     check(_lib_or_raise().aa_hip_corner_turn_u16(_ptr(d_in_tc), int(nsamp), int(nchans), _ptr(d_out),
                                                  int(out_pitch), _ptr(d_err), _stream_ptr(stream)),
           "corner_turn_u16")
+
+
+def corner_turn_levels(d_in_tc, sample_bytes: int, nsamp: int, nchans: int, d_levels, pitches, d_err=None,
+                       stream=None) -> int:
+    """Corner turn plus len(d_levels) - 1 x2 binned levels in one pass; returns
+    the number of binned levels written (0 when the shape takes the byte path)."""
+    n = len(d_levels)
+    ptrs = (ctypes.c_void_p * n)(*[x.data_ptr() for x in d_levels])
+    pit = (ctypes.c_size_t * n)(*[int(p) for p in pitches])
+    fused = ctypes.c_int(-1)
+    check(_lib_or_raise().aa_hip_corner_turn_levels(_ptr(d_in_tc), int(sample_bytes), int(nsamp), int(nchans), ptrs,
+                                                    pit, n - 1, _ptr(d_err) if d_err is not None else None,
+                                                    ctypes.byref(fused), _stream_ptr(stream)), "corner_turn_levels")
+    return fused.value
 
 
 def bin2_u8(d_in, in_pitch: int, nchans: int, n_out: int, d_out, out_pitch: int, stream=None):

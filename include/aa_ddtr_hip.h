@@ -305,6 +305,20 @@ int aa_hip_corner_turn_u8(const uint8_t *d_in_tc, int nsamp, int nchans, uint8_t
  * *d_err (device int) to 1; the samples are narrowed regardless. */
 int aa_hip_corner_turn_u16(const uint16_t *d_in_tc, int nsamp, int nchans, uint8_t *d_out_ct,
                            size_t out_pitch, int *d_err, void *stream);
+/* Replaces corner_turn_SM_inplace (src/aa_corner_turn.cpp:118-121) followed
+ * by the runner's bin_gpu cascade (include/aa_permitted_pipelines_generic.hpp:
+ * 774-812, src/aa_bin_gpu.cpp:8-44) in one pass over the input: u8
+ * (sample_bytes 1) or u16 holding 8-bit samples (sample_bytes 2; a value
+ * above 255 sets *d_err) [t][c] -> level 0 u8 [c][t] at d_levels[0] and the
+ * x2 levels k = 1..nlev (nlev <= 3) at d_levels[k]:
+ * L_k[c][t] = floor((L_{k-1}[c][2t] + L_{k-1}[c][2t+1]) / 2), t < nsamp >> k.
+ * Row k holds pitches[k] >= nsamp >> k bytes.  *fused (may be NULL) is set to
+ * the number of binned levels written: nlev when nchans is a multiple of 16
+ * and the input, rows and pitches are 16-byte aligned, else 0 (then only
+ * level 0 is written and the levels come from aa_hip_bin2_u8). */
+int aa_hip_corner_turn_levels(const void *d_in_tc, int sample_bytes, int nsamp, int nchans,
+                              uint8_t *const *d_levels, const size_t *pitches, int nlev, int *d_err,
+                              int *fused, void *stream);
 /* Replaces bin_gpu (src/aa_bin_gpu.cpp:8-44) for the 8-bit path, one x2
  * level: out[c][t] = floor((in[c][2t] + in[c][2t+1]) / 2), t < n_out. */
 int aa_hip_bin2_u8(const uint8_t *d_in, size_t in_pitch, int nchans, int n_out, uint8_t *d_out,

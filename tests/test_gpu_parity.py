@@ -57,6 +57,46 @@ def test_corner_turn_u16_and_flag(aa, oracle, cuda):
     assert int(err.item()) == 1
 
 
+@pytest.mark.parametrize("nsamp,nchans,nlev,sample_bytes", [
+    (4109, 144, 3, 1),     # ragged time tile (4109 = 16 x 256 + 13), partial channel tile (144 = 128 + 16)
+    (256, 128, 1, 1),      # exactly one tile
+    (1000, 16, 2, 1),      # one 16-channel group
+    (70001, 4096, 3, 1),   # 4096 channels (32 channel tiles), odd length
+    (3001, 112, 3, 2),     # u16 input holding 8-bit samples
+    (777, 100, 2, 1),      # 100 channels: not a multiple of 16 -> byte path, levels left to bin2
+])
+def test_corner_turn_levels(aa, oracle, cuda, nsamp, nchans, nlev, sample_bytes):
+    """The corner turn fused with the x2 cascade: level 0 is the transpose and
+    level k the floor-average of level k-1, bit for bit with the oracle, and
+    nothing is written past each level's nsamp >> k samples."""
+    import torch
+    say(f"corner_turn_levels nsamp={nsamp} nchans={nchans} nlev={nlev} bytes={sample_bytes}")
+    rng = np.random.default_rng(nsamp + 31 * nchans)
+    dt = np.uint8 if sample_bytes == 1 else np.uint16
+    x = rng.integers(0, 256, size=(nsamp, nchans), dtype=dt)
+    xd = to_dev(x if sample_bytes == 1 else x.view(np.int16), cuda)
+    pitches = [((nsamp >> k) + 4096 + 255) // 256 * 256 for k in range(nlev + 1)]
+    levels = [torch.full((nchans, p), 0xAB, dtype=torch.uint8, device=cuda) for p in pitches]
+    err = torch.zeros(1, dtype=torch.int32, device=cuda)
+    fused = aa.corner_turn_levels(xd, sample_bytes, nsamp, nchans, levels, pitches, err)
+    assert fused == (nlev if nchans % 16 == 0 else 0)
+    want = oracle.corner_turn(x, nsamp, nchans)
+    for k in range(fused + 1):
+        got = levels[k].cpu().numpy()
+        n = nsamp >> k
+        np.testing.assert_array_equal(got[:, :n], want, err_msg=f"level {k}")
+        assert np.all(got[:, n:] == 0xAB), f"level {k} written past its {n} samples"
+        if k < nlev:
+            want = oracle.bin2(want, want.shape[1] // 2)
+    for k in range(fused + 1, nlev + 1):
+        assert np.all(levels[k].cpu().numpy() == 0xAB), "levels beyond the fused ones untouched"
+    assert int(err.item()) == 0
+    if sample_bytes == 2:
+        x[nsamp // 2, nchans - 1] = 256
+        aa.corner_turn_levels(to_dev(x.view(np.int16), cuda), 2, nsamp, nchans, levels, pitches, err)
+        assert int(err.item()) == 1
+
+
 @pytest.mark.parametrize("n_out,in_pitch,out_pitch", [(1000, 2048, 1024), (1001, 2003, 1001), (3, 8, 4)])
 def test_bin2(aa, oracle, cuda, n_out, in_pitch, out_pitch):
     import torch
