@@ -5,6 +5,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "aa_ddtr_hip.h"
@@ -264,14 +265,14 @@ bool aa_py_ddtr_strategy_dmshifts(void const *obj, float *out, int nchans) {
 }
 
 // --------------------------------------------------------- device layer
-int aa_hip_ddtr_create(int device, int nchans, const float *dmshifts, int nranges, const float *dm_low,
+static int ddtr_create(int device, int nchans, const float *dmshifts, int nranges, const float *dm_low,
                        const float *dm_step, const int *ndms, const int *inBin, double tsamp, int maxshift,
-                       int max_t_processed0, aa_hip_ddtr **out) {
+                       int max_t_processed0, aa_hip_ddtr **out, int sample_bits) {
     if (!out) return bad("out is NULL");
     *out = nullptr;
     aa_hip::ddtr_engine *e = nullptr;
     const int st = aa_hip::ddtr_engine::create(device, nchans, dmshifts, nranges, dm_low, dm_step, ndms, inBin,
-                                               tsamp, maxshift, max_t_processed0, &e);
+                                               tsamp, maxshift, max_t_processed0, &e, sample_bits);
     if (st != AA_HIP_SUCCESS) return st;
     aa_hip_ddtr *h = new (std::nothrow) aa_hip_ddtr;
     if (!h) {
@@ -281,6 +282,20 @@ int aa_hip_ddtr_create(int device, int nchans, const float *dmshifts, int nrange
     h->engine.reset(e);
     *out = h;
     return AA_HIP_SUCCESS;
+}
+
+int aa_hip_ddtr_create(int device, int nchans, const float *dmshifts, int nranges, const float *dm_low,
+                       const float *dm_step, const int *ndms, const int *inBin, double tsamp, int maxshift,
+                       int max_t_processed0, aa_hip_ddtr **out) {
+    return ddtr_create(device, nchans, dmshifts, nranges, dm_low, dm_step, ndms, inBin, tsamp, maxshift,
+                       max_t_processed0, out, 8);
+}
+
+int aa_hip_ddtr_create_u16(int device, int nchans, const float *dmshifts, int nranges, const float *dm_low,
+                           const float *dm_step, const int *ndms, const int *inBin, double tsamp, int maxshift,
+                           int max_t_processed0, aa_hip_ddtr **out) {
+    return ddtr_create(device, nchans, dmshifts, nranges, dm_low, dm_step, ndms, inBin, tsamp, maxshift,
+                       max_t_processed0, out, 16);
 }
 
 int aa_hip_ddtr_destroy(aa_hip_ddtr *h) {

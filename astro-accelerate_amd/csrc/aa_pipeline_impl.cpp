@@ -129,10 +129,17 @@ public:
                 LOG(log_level::notice, "pipeline option " + std::to_string((int)o) + " has no effect here (ignored).");
         }
         const int nb = m_metadata.nbits();
-        if (nb != 1 && nb != 2 && nb != 4 && nb != 8) {
-            LOG(log_level::error, "only 1, 2, 4 or 8-bit filterbank data (one byte per sample in memory) is "
-                                  "supported (nbits=" + std::to_string(nb) + ").");
+        const bool narrow = nb == 1 || nb == 2 || nb == 4 || nb == 8;
+        const bool wide = nb == 16 || nb == 32;   // 32-bit files arrive scaled to u16 (the reader's rule)
+        if (!narrow && !wide) {
+            LOG(log_level::error, "filterbank data of 1, 2, 4, 8, 16 or 32 bits is supported (nbits=" +
+                                      std::to_string(nb) + ").");
             aa_hip::set_last_error("unsupported nbits");
+            return false;
+        }
+        if (wide && m_bytes != 2) {
+            LOG(log_level::error, std::to_string(nb) + "-bit data needs a 16-bit (unsigned short) input buffer.");
+            aa_hip::set_last_error("unsupported nbits for an 8-bit input buffer");
             return false;
         }
         if (!m_input) {
@@ -236,9 +243,12 @@ private:
         }
         const std::vector<float> dmshifts = m_strategy.dmshifts();
         aa_hip::ddtr_engine *e = nullptr;
+        // Samples wider than 8 bits keep u16 levels and float channel sums.
+        m_wide = m_metadata.nbits() > 8;
         int st = aa_hip::ddtr_engine::create(m_card.currently_selected_card_id(), nchans, dmshifts.data(), nranges,
                                              low.data(), step.data(), ndms.data(), inbin.data(),
-                                             m_metadata.tsamp(), m_strategy.maxshift(), max_tp0, &e);
+                                             m_metadata.tsamp(), m_strategy.maxshift(), max_tp0, &e,
+                                             m_wide ? 16 : 8);
         if (st != AA_HIP_SUCCESS) {
             LOG(log_level::error, std::string("pipeline setup: ") + aa_hip::last_error());
             m_finished_setup_failed = true;
@@ -441,7 +451,7 @@ private:
                              ? aa_hip::launch_zero_dm_u8((uint8_t *)m_d_input, rows, nchans, m_d_bandpass,
                                                          m_metadata.nbits(), m_stream)
                              : aa_hip::launch_zero_dm_u16((uint16_t *)m_d_input, rows, nchans, m_d_bandpass,
-                                                          m_metadata.nbits(), m_stream);
+                                                          sample_nbits(), m_stream);
                     if (he != hipSuccess) {
                         LOG(log_level::error, std::string("zero-DM: ") + hipGetErrorString(he));
                         return false;
@@ -451,7 +461,7 @@ private:
                     he = m_bytes == 1 ? aa_hip::launch_zero_dm_outliers_u8((uint8_t *)m_d_input, rows, nchans,
                                                                            m_d_bandpass, m_metadata.nbits(), m_stream)
                                       : aa_hip::launch_zero_dm_outliers_u16((uint16_t *)m_d_input, rows, nchans,
-                                                                            m_d_bandpass, m_metadata.nbits(),
+                                                                            m_d_bandpass, sample_nbits(),
                                                                             m_stream);
                     if (he != hipSuccess) {
                         LOG(log_level::error, std::string("zero-DM with outliers: ") + hipGetErrorString(he));
@@ -460,7 +470,7 @@ private:
                 }
                 if (m_in_norm) {   // after zero-DM, before the corner turn (same rows)
                     const long rows = (long)tp0 + m_strategy.maxshift();
-                    const int nb = m_metadata.nbits();
+                    const int nb = sample_nbits();
                     const float factor = (float)((std::pow(2.0, nb) - 1.0) / 2.0);
                     he = m_bytes == 1 ? aa_hip::launch_normalize_columns_u8((uint8_t *)m_d_input, rows, nchans, factor,
                                                                             nb, m_d_norm_scratch, m_stream)
@@ -494,7 +504,7 @@ private:
                     }
                     m_prefetched_chunk = chunk + 1;
                 }
-                if (m_bytes == 2) {
+                if (m_bytes == 2 && !m_wide) {
                     st = m_engine->take_input_overflow(m_stream);
                     if (st != 0) {
                         LOG(log_level::error, st > 0 ? std::string("input sample above 255 in 8-bit data")
@@ -609,6 +619,10 @@ private:
     aa_filterbank_metadata m_metadata;
     const void *m_input;
     int m_bytes;
+    bool m_wide = false;   // nbits > 8: u16 levels, float channel sums
+    // Sample width of the data in device memory: 32-bit files are scaled to
+    // u16 by the reader, so the conditioning kernels see 16-bit samples.
+    int sample_nbits() const { return std::min(m_metadata.nbits(), 16); }
     aa_device_info &m_card;
     aa_ddtr_plan m_plan;
     aa_ddtr_strategy m_strategy;

@@ -37,6 +37,12 @@ struct shim_tables {
     void *d = nullptr;
     int row_quads = 0;
 };
+// The 16-bit kernel's window of one range, kept while its geometry holds.
+struct shim_window {
+    int nchans = 0, ndms = 0;
+    float mstartdm = 0.0f, mdmstep = 0.0f;
+    int win = 0;
+};
 
 struct shim_constants {
     int nchans = 0;
@@ -48,8 +54,10 @@ struct shim_constants {
     int device = -1;
     uint8_t *d_u8 = nullptr;      // the u16 input narrowed to u8 rows for the LDS kernels
     size_t u8_capacity = 0;
-    int *d_err = nullptr;         // set when a u16 sample exceeds 255
+    size_t u8_pitch = 0;
+    int *d_err = nullptr;         // set when an 8-bit call meets a sample above 255
     std::map<int, shim_tables> tables;
+    std::map<int, shim_window> windows;
 };
 std::mutex g_mutex;
 shim_constants g_const;
@@ -91,11 +99,42 @@ void load_chunk_data(unsigned short *device_pointer, unsigned short const *const
         for (auto &kv : g_const.tables)
             if (kv.second.d) (void)hipFree(kv.second.d);
         g_const.tables.clear();
+        g_const.windows.clear();
     }
     g_const.dmshifts.assign(dmshifts, dmshifts + nchans);
     g_const.nchans = nchans;
     g_const.i_nsamp = length;
     g_const.t_processed = t_processed;
+    // dedisperse() allocates nothing per call: the u8 rows of 8-bit data
+    // (16-byte aligned, 4096 bytes of slack past each row) and the overflow
+    // flag are sized here, for the chunk's longest rows.
+    const size_t pitch = ((size_t)length + 4096 + 255) / 256 * 256;
+    if (g_const.u8_capacity < pitch * (size_t)nchans) {
+        if (g_const.d_u8) (void)hipFree(g_const.d_u8);
+        g_const.d_u8 = nullptr;
+        g_const.u8_capacity = 0;
+        if (hipMalloc(&g_const.d_u8, pitch * (size_t)nchans) == hipSuccess)
+            g_const.u8_capacity = pitch * (size_t)nchans;
+        else
+            LOG(log_level::error, "load_chunk_data: cannot allocate the 8-bit rows");
+    }
+    g_const.u8_pitch = pitch;
+    if (!g_const.d_err) {
+        if (hipMalloc(&g_const.d_err, sizeof(int)) != hipSuccess || hipMemset(g_const.d_err, 0, sizeof(int)) != hipSuccess) {
+            LOG(log_level::error, "load_chunk_data: cannot allocate the input flag");
+            g_const.d_err = nullptr;
+        }
+    } else {
+        // The previous chunk's 8-bit calls flag samples above 255 without
+        // waiting for the device; the flag is read here, where the copy
+        // above has already synchronised.
+        int flag = 0;
+        if (hipMemcpy(&flag, g_const.d_err, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess && flag) {
+            LOG(log_level::error, "dedisperse: samples above 255 met with nbits <= 8 in the previous chunk "
+                                  "(their sums were taken modulo 256); pass the data's nbits (16)");
+            (void)hipMemset(g_const.d_err, 0, sizeof(int));
+        }
+    }
 }
 
 // src/aa_device_load_data.cu:58-61
@@ -107,11 +146,18 @@ void set_dedispersion_constants(int t_processed, int maxshift) {
 
 // src/aa_dedisperse.cpp:5-163.  The failsafe kernel's semantics for every
 // geometry: its grid covers t < 8*(t_processed/8) and d < 64*(ndms/64), the
-// output row stride is the current i_t_processed_s.  The u16 rows are
-// narrowed to a u8 copy (they hold 8-bit values) and dedispersed by the LDS
-// kernels, the first chunk shape that fits, with the range's plan tables
-// cached between calls; a sample above 255 sends the call to the direct
-// kernel on the u16 rows instead.
+// output row stride is the current i_t_processed_s.
+//  * nbits <= 8: the u16 rows hold 8-bit values; they are narrowed to the u8
+//    rows sized by load_chunk_data (on every call: bin_gpu rewrites the rows
+//    between ranges) and dedispersed by the LDS kernels, the first chunk
+//    shape that fits, with the range's plan tables built on its first call
+//    and kept while its geometry holds.  A sample above 255 is flagged on
+//    the device and reported by the next load_chunk_data.
+//  * nbits > 8: the u16 rows are dedispersed as they are, float sums in
+//    channel order (the 16-bit LDS kernel, or the direct kernel when a tile's
+//    delay spread exceeds its rows).
+// Nothing waits for the device and nothing is allocated per call beyond a
+// range's first.
 int dedisperse(int i, int t_processed, int *inBin, float *dmshifts, unsigned short *d_input, float *d_output,
                float *d_dm_shifts, int nchans, float *tsamp, float *dm_low, float *dm_step, int const *const ndms,
                int nbits, int failsafe) {
@@ -120,8 +166,8 @@ int dedisperse(int i, int t_processed, int *inBin, float *dmshifts, unsigned sho
     (void)nchans;
     (void)failsafe;
     std::lock_guard<std::mutex> lock(g_mutex);
-    if (nbits != 8 && nbits != 16) {
-        LOG(log_level::error, "dedisperse: only 8-bit samples (u16 containers) are supported");
+    if (nbits < 1 || nbits > 32) {
+        LOG(log_level::error, "dedisperse: nbits must be 1..32");
         return 1;
     }
     if (!g_const.d_dmshifts || g_const.nchans <= 0) {
@@ -139,30 +185,34 @@ int dedisperse(int i, int t_processed, int *inBin, float *dmshifts, unsigned sho
         return 1;
     }
     if (d_cov == 0 || t_cov == 0) return 0;
-    // u8 rows, 16-byte aligned, with 4096 bytes of slack past each row
-    const size_t pitch = ((size_t)n + 4096 + 255) / 256 * 256;
     hipError_t e = hipSuccess;
-    if (g_const.u8_capacity < pitch * (size_t)nch) {
-        if (g_const.d_u8) (void)hipFree(g_const.d_u8);
-        g_const.d_u8 = nullptr;
-        g_const.u8_capacity = 0;
-        e = hipMalloc(&g_const.d_u8, pitch * (size_t)nch);
-        if (e == hipSuccess) g_const.u8_capacity = pitch * (size_t)nch;
+    if (nbits > 8) {
+        shim_window &w = g_const.windows[i];
+        if (w.win == 0 || w.nchans != nch || w.ndms != d_cov || w.mstartdm != mstartdm || w.mdmstep != mdmstep) {
+            w = shim_window{nch, d_cov, mstartdm, mdmstep,
+                            aa_hip::wide_window(g_const.dmshifts.data(), nch, d_cov, mstartdm, mdmstep)};
+        }
+        if (w.win > 0)
+            e = aa_hip::launch_dedisperse_wide_u16(d_input, (size_t)n, n, nch, g_const.d_dmshifts, d_cov, t_cov,
+                                                   mstartdm, mdmstep, inBin[i], d_output, (size_t)g_const.t_processed,
+                                                   w.win, nullptr);
+        else
+            e = aa_hip::launch_dedisperse_direct_u16(d_input, (size_t)n, nch, g_const.d_dmshifts, d_cov, t_cov,
+                                                     mstartdm, mdmstep, inBin[i], d_output,
+                                                     (size_t)g_const.t_processed, nullptr);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) LOG(log_level::error, std::string("dedisperse: ") + hipGetErrorString(e));
+        return e == hipSuccess ? 0 : 1;
     }
-    if (e == hipSuccess && !g_const.d_err) e = hipMalloc(&g_const.d_err, sizeof(int));
-    int err = 1;
-    if (e == hipSuccess) e = hipMemsetAsync(g_const.d_err, 0, sizeof(int), nullptr);
-    if (e == hipSuccess) e = aa_hip::launch_narrow_u16(d_input, n, nch, n, g_const.d_u8, pitch, g_const.d_err, nullptr);
-    if (e == hipSuccess) e = hipMemcpy(&err, g_const.d_err, sizeof(int), hipMemcpyDeviceToHost);
+    const size_t pitch = g_const.u8_pitch;
+    if (!g_const.d_u8 || !g_const.d_err || pitch * (size_t)nch > g_const.u8_capacity || (long)pitch < n + 4096) {
+        LOG(log_level::error, "dedisperse: the 8-bit rows of load_chunk_data are missing or too short");
+        return 1;
+    }
+    e = aa_hip::launch_narrow_u16(d_input, n, nch, n, g_const.d_u8, pitch, g_const.d_err, nullptr);
     if (e != hipSuccess) {
         LOG(log_level::error, std::string("dedisperse: ") + hipGetErrorString(e));
         return 1;
-    }
-    if (err) {   // u16 values above 8 bits: the direct kernel sums them as they are
-        e = aa_hip::launch_dedisperse_direct_u16(d_input, (size_t)n, nch, g_const.d_dmshifts, d_cov, t_cov, mstartdm,
-                                                 mdmstep, inBin[i], d_output, (size_t)g_const.t_processed, nullptr);
-        if (e == hipSuccess) e = hipGetLastError();
-        return e == hipSuccess ? 0 : 1;
     }
     aa_hip::range_launch rl;
     if (aa_hip::resolve_launch(AA_HIP_KERNEL_AUTO, g_const.d_u8, pitch, nch, g_const.d_dmshifts,

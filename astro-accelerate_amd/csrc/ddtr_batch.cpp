@@ -42,6 +42,8 @@ int ddtr_batch::create(int nbeams, ddtr_engine *const *beams, ddtr_batch **out) 
         if (!beams[b]) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "beam " + std::to_string(b) + " is NULL");
         if (beams[b]->device() != beams[0]->device())
             return fail(AA_HIP_ERROR_UNSUPPORTED, "all beams of a batch must be on one device");
+        if (beams[b]->sample_bits() != 8)
+            return fail(AA_HIP_ERROR_UNSUPPORTED, "batches take 8-bit plans (beam " + std::to_string(b) + " is 16-bit)");
     }
     auto *p = new (std::nothrow) ddtr_batch();
     if (!p) return fail(AA_HIP_ERROR_OUT_OF_MEMORY, "host allocation");

@@ -149,6 +149,15 @@ lds_params lds_params_of(const range_launch &rl) {
 }
 
 hipError_t launch_range(const range_launch &rl, hipStream_t s) {
+    if (rl.sample_bytes == 2) {
+        const uint16_t *L = reinterpret_cast<const uint16_t *>(rl.L);
+        if (rl.wide_win > 0)
+            return launch_dedisperse_wide_u16(L, rl.pitch / 2, rl.row_len, rl.nchans, rl.d_dmshifts, rl.ndms,
+                                              rl.t_processed, rl.mstartdm, rl.mdmstep, rl.inBin, rl.out, rl.out_pitch,
+                                              rl.wide_win, s);
+        return launch_dedisperse_direct_u16(L, rl.pitch / 2, rl.nchans, rl.d_dmshifts, rl.ndms, rl.t_processed,
+                                            rl.mstartdm, rl.mdmstep, rl.inBin, rl.out, rl.out_pitch, s);
+    }
     if (rl.variant >= 0) return launch_dedisperse_lds(lds_params_of(rl), rl.variant, s);
     return launch_dedisperse_direct_u8(rl.L, rl.pitch, rl.nchans, rl.d_dmshifts, rl.ndms, rl.t_processed,
                                        rl.mstartdm, rl.mdmstep, rl.inBin, rl.out, rl.out_pitch, s);
@@ -157,9 +166,10 @@ hipError_t launch_range(const range_launch &rl, hipStream_t s) {
 int ddtr_engine::create(int device, int nchans, const float *dmshifts, int nranges,
                         const float *dm_low, const float *dm_step, const int *ndms,
                         const int *inBin, double tsamp, int maxshift, int max_t_processed0,
-                        ddtr_engine **out) {
+                        ddtr_engine **out, int sample_bits) {
     if (!out) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "out is NULL");
     *out = nullptr;
+    if (sample_bits != 8 && sample_bits != 16) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "sample_bits must be 8 or 16");
     if (nchans <= 0 || nranges <= 0 || !dmshifts || !dm_low || !dm_step || !ndms || !inBin ||
         maxshift < 0 || max_t_processed0 <= 0 || !(tsamp > 0.0))
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "invalid plan geometry");
@@ -185,6 +195,7 @@ int ddtr_engine::create(int device, int nchans, const float *dmshifts, int nrang
     device_guard guard(device);
     ddtr_engine *e = new ddtr_engine();
     e->m_device = device;
+    e->m_sample_bytes = sample_bits / 8;
     e->m_nchans = nchans;
     e->m_tsamp = (float)tsamp;
     e->m_maxshift = maxshift;
@@ -202,7 +213,7 @@ int ddtr_engine::create(int device, int nchans, const float *dmshifts, int nrang
     for (int k = 0; k < nlevels && st == hipSuccess; ++k) {
         level lv;
         lv.capacity = cap;
-        lv.pitch = round_up((size_t)cap + kRowPad, kPitchAlign);
+        lv.pitch = round_up(((size_t)cap + kRowPad) * (size_t)e->m_sample_bytes, kPitchAlign);
         const size_t bytes = lv.pitch * (size_t)nchans;
         st = hipMalloc(&lv.ptr, bytes);
         if (st == hipSuccess) st = hipMemset(lv.ptr, 0, bytes);
@@ -251,6 +262,16 @@ int ddtr_engine::ingest(const void *d_input_tc, int input_bits, int t_processed0
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "t_processed0 exceeds the handle's capacity");
     if (input_bits != 8 && input_bits != 16) return fail(AA_HIP_ERROR_UNSUPPORTED, "input_bits must be 8 or 16");
     device_guard guard(m_device);
+    if (m_sample_bytes == 2) {   // u16 levels: samples keep their width
+        if (input_bits != 16) return fail(AA_HIP_ERROR_UNSUPPORTED, "a 16-bit plan takes 16-bit input");
+        const hipError_t e = launch_transpose_u16_pitch((const uint16_t *)d_input_tc, len0, m_nchans,
+                                                        (uint16_t *)m_levels[0].ptr, m_levels[0].pitch / 2, s);
+        if (e != hipSuccess) return hip_fail(e, "corner turn");
+        for (auto &lv : m_levels) lv.len = 0;
+        m_levels[0].len = len0;
+        m_resident = 0;
+        return AA_HIP_SUCCESS;
+    }
     // The corner turn writes the binned levels the plan needs in the same
     // pass when the shapes allow it; otherwise ensure_level bins on demand.
     const int nlev = std::min((int)m_levels.size() - 1, kCtMaxLevels);
@@ -271,6 +292,7 @@ int ddtr_engine::ingest(const void *d_input_tc, int input_bits, int t_processed0
 }
 
 int ddtr_engine::take_input_overflow(hipStream_t s) {
+    if (m_sample_bytes == 2) return 0;   // u16 levels hold any 16-bit value
     device_guard guard(m_device);
     int flag = 0;
     hipError_t e = hipMemcpyAsync(&flag, m_d_err, sizeof(int), hipMemcpyDeviceToHost, s);
@@ -288,7 +310,10 @@ int ddtr_engine::ensure_level(int k, hipStream_t s) {
         level &dst = m_levels[(size_t)m_resident + 1];
         const long n_out = src.len / 2;
         if (n_out > dst.capacity) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "binning level overflow");
-        hipError_t e = launch_bin2_u8(src.ptr, src.pitch, m_nchans, (int)n_out, dst.ptr, dst.pitch, s);
+        hipError_t e = m_sample_bytes == 2
+                           ? launch_bin2_u16((const uint16_t *)src.ptr, src.pitch / 2, m_nchans, n_out,
+                                             (uint16_t *)dst.ptr, dst.pitch / 2, s)
+                           : launch_bin2_u8(src.ptr, src.pitch, m_nchans, (int)n_out, dst.ptr, dst.pitch, s);
         if (e != hipSuccess) return hip_fail(e, "binning");
         dst.len = n_out;
         ++m_resident;
@@ -317,6 +342,29 @@ int ddtr_engine::prepare_range(int r, int t_processed, float *d_out, size_t out_
     range_scalars(r, &mstartdm, &mdmstep);
     st = check_delays(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep, t_processed, i_nsamp);
     if (st != AA_HIP_SUCCESS) return st;
+    if (m_sample_bytes == 2) {   // u16 levels: dedisperse_u16, or the direct kernel when its window is too wide
+        *rl = range_launch{};
+        rl->L = lv.ptr;
+        rl->pitch = lv.pitch;
+        rl->nchans = m_nchans;
+        rl->d_dmshifts = m_d_dmshifts;
+        rl->ndms = rd.ndms;
+        rl->t_processed = t_processed;
+        rl->mstartdm = mstartdm;
+        rl->mdmstep = mdmstep;
+        rl->inBin = rd.inBin;
+        rl->out = d_out;
+        rl->out_pitch = out_pitch;
+        rl->sample_bytes = 2;
+        rl->row_len = i_nsamp;
+        if (m_wide_win.size() != m_ranges.size()) m_wide_win.assign(m_ranges.size(), 0);
+        int &w = m_wide_win[(size_t)r];
+        if (w == 0) w = wide_window(m_dmshifts.data(), m_nchans, rd.ndms, mstartdm, mdmstep);
+        rl->wide_win = m_kernel == AA_HIP_KERNEL_DIRECT ? -1 : w;
+        if (rl->wide_win < 0 && m_kernel != AA_HIP_KERNEL_AUTO && m_kernel != AA_HIP_KERNEL_DIRECT)
+            return fail(AA_HIP_ERROR_UNSUPPORTED, "the 16-bit LDS kernel does not fit this range's delay spread");
+        return AA_HIP_SUCCESS;
+    }
     if (m_lds.size() != m_ranges.size()) m_lds.assign(m_ranges.size(), lds_choice{-2, 0});
     lds_choice &c = m_lds[(size_t)r];
     if (c.variant == -2) {
@@ -372,6 +420,7 @@ int ddtr_engine::set_range_variant(int r, int variant) {
 int ddtr_engine::autotune_range(int r, int t_processed, float *d_out, size_t out_pitch, int reps, hipStream_t s,
                                 int *best, float *ms) {
     if (!best || !ms || reps <= 0) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "bad autotune arguments");
+    if (m_sample_bytes == 2) return fail(AA_HIP_ERROR_UNSUPPORTED, "16-bit plans have one kernel (nothing to tune)");
     if (r < 0 || r >= (int)m_ranges.size()) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "bad range index");
     device_guard guard(m_device);
     const int saved_kernel = m_kernel;

@@ -61,15 +61,21 @@ struct range_launch {
     int variant = -1;
     int row_quads = 0;
     const void *pair_recs = nullptr;  // chunk shapes 2-4: device plan tables
+    int sample_bytes = 1;             // 2: u16 rows (samples wider than 8 bits; pitch still in bytes)
+    long row_len = 0;                 // u16 rows: valid samples per row
+    int wide_win = -1;                // u16 rows: dedisperse_u16's window, -1 = direct kernel
 };
 
 class ddtr_engine {
 public:
-    // Returns AA_HIP_* status; on success *out owns the engine.
+    // Returns AA_HIP_* status; on success *out owns the engine.  sample_bits
+    // 8: u8 levels (8-bit samples, or u16 input holding them); 16: u16
+    // levels for samples wider than 8 bits (float sums in channel order).
     static int create(int device, int nchans, const float *dmshifts, int nranges,
                       const float *dm_low, const float *dm_step, const int *ndms,
                       const int *inBin, double tsamp, int maxshift, int max_t_processed0,
-                      ddtr_engine **out);
+                      ddtr_engine **out, int sample_bits = 8);
+    int sample_bits() const { return m_sample_bytes * 8; }
     ~ddtr_engine();
 
     int set_kernel(int kernel);
@@ -125,6 +131,7 @@ private:
     };
 
     int m_device = 0;
+    int m_sample_bytes = 1;  // bytes per level sample: 1 (u8) or 2 (u16)
     int m_nchans = 0;
     float m_tsamp = 0.0f;  // metadata tsamp narrowed to float
     int m_maxshift = 0;
@@ -137,6 +144,7 @@ private:
     int m_resident = -1;  // highest level valid for the current chunk
     std::vector<lds_choice> m_lds;     // per range: chosen LDS shape (-2 = not computed, -1 = direct)
     std::vector<int> m_variant_pref;   // per range: preferred LDS shape, -1 = any
+    std::vector<int> m_wide_win;       // per range (u16 levels): dedisperse_u16 window, 0 = not computed
     struct plan_tables {
         int variant = -1;  // chunk shape the tables are for
         void *ptr = nullptr;

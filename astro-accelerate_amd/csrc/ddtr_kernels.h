@@ -76,6 +76,7 @@ hipError_t launch_normalize_columns_u8(uint8_t* data, long rows, long cols, floa
 hipError_t launch_normalize_columns_u16(uint16_t* data, long rows, long cols, float factor, int nbits,
                                         void* scratch, hipStream_t s);
 hipError_t launch_normalize_columns_f32(float* data, long rows, long cols, float factor, hipStream_t s);
+
 hipError_t launch_bin2_u8(const uint8_t* in, size_t in_pitch, int nchans, int n_out,
                           uint8_t* out, size_t out_pitch, hipStream_t s);
 // Reference bin_gpu semantics on u16 rows of length nsamp packed into rows
@@ -93,6 +94,24 @@ hipError_t launch_dedisperse_direct_u16(const uint16_t* L, size_t pitch, int nch
                                         const float* d_dmshifts, int ndms, int t_processed,
                                         float mstartdm, float mdmstep, int inBin,
                                         float* out, size_t out_pitch, hipStream_t s);
+
+// Samples wider than 8 bits (u16 rows, row pitches in samples): float sums
+// in channel order, as the reference's cache kernel forms them.
+// wide_window: the staged window (tile + the largest delay spread of a
+// 16-DM tile, even) dedisperse_u16 needs for the range, or -1 when it
+// exceeds the kernel's LDS rows (the direct kernel then runs).  Host only.
+int wide_window(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep);
+// row_len: valid samples per row (reads beyond it see zeros); win from
+// wide_window.  Any even pitch and 4-byte aligned L take single dword loads.
+hipError_t launch_dedisperse_wide_u16(const uint16_t* L, size_t pitch, long row_len, int nchans,
+                                      const float* d_dmshifts, int ndms, int t_processed, float mstartdm,
+                                      float mdmstep, int inBin, float* out, size_t out_pitch, int win, hipStream_t s);
+// [t][c] u16 -> [c][t] u16, out rows out_pitch samples apart.
+hipError_t launch_transpose_u16_pitch(const uint16_t* in_tc, long nsamp, int nchans, uint16_t* out_ct,
+                                      size_t out_pitch, hipStream_t s);
+// out[c][t] = (in[c][2t] + in[c][2t+1]) >> 1, t < n_out, u16 rows.
+hipError_t launch_bin2_u16(const uint16_t* in, size_t in_pitch, int nchans, long n_out, uint16_t* out,
+                           size_t out_pitch, hipStream_t s);
 
 // First single-pulse-search pass on a DM-time plane (boxcar MSD per width
 // 1..W, then best boxcar SNR and its width per start sample); `work` holds

@@ -298,7 +298,10 @@ __global__ void swap_ref_u16_kernel(uint16_t* __restrict__ data, const float* __
 
 // ---------------------------------------------------------- direct dedisperse
 // One thread per (DM, output sample).  Used as the universally valid
-// variant and as the shim behind the u16 drop-in dedisperse().
+// variant and as the shim behind the u16 drop-in dedisperse().  u8 sums are
+// exact integers; u16 samples are summed as the reference's cache kernel
+// does, one float accumulator in channel order (it rounds once the sum
+// passes 2^24).
 template <typename T>
 __global__ __launch_bounds__(256) void dedisperse_direct_kernel(
     const T* __restrict__ L, size_t pitch, int nchans, const float* __restrict__ dmshifts,
@@ -310,12 +313,22 @@ __global__ __launch_bounds__(256) void dedisperse_direct_kernel(
     if (t >= t_processed || d >= ndms) return;
     const float shift_temp = __fmaf_rn((float)d, mdmstep, mstartdm);
     const T* p = L + t;
-    uint32_t sum = 0;
-    for (int c = 0; c < nchans; ++c) {
-        const int delay = ddtr_delay(dmshifts[c], shift_temp);
-        sum += (uint32_t)p[(size_t)c * pitch + (size_t)delay];
+    float total;
+    if constexpr (sizeof(T) == 1) {
+        uint32_t sum = 0;
+        for (int c = 0; c < nchans; ++c) {
+            const int delay = ddtr_delay(dmshifts[c], shift_temp);
+            sum += (uint32_t)p[(size_t)c * pitch + (size_t)delay];
+        }
+        total = (float)sum;
+    } else {
+        total = 0.0f;
+        for (int c = 0; c < nchans; ++c) {
+            const int delay = ddtr_delay(dmshifts[c], shift_temp);
+            total = __fadd_rn(total, (float)p[(size_t)c * pitch + (size_t)delay]);
+        }
     }
-    out[(size_t)d * out_pitch + (size_t)t] = __fdiv_rn(__fdiv_rn((float)sum, fnchans), fbin);
+    out[(size_t)d * out_pitch + (size_t)t] = __fdiv_rn(__fdiv_rn(total, fnchans), fbin);
 }
 
 // ----------------------------------------------------------- synthetic input

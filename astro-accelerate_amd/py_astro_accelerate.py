@@ -161,6 +161,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_py_ddtr_strategy_dmshifts": ([vp, PFLOAT, i], b),
         "aa_hip_ddtr_create": ([i, i, PFLOAT, i, PFLOAT, PFLOAT, pi, pi, d, i, i,
                                 ctypes.POINTER(vp)], i),
+        "aa_hip_ddtr_create_u16": ([i, i, PFLOAT, i, PFLOAT, PFLOAT, pi, pi, d, i, i,
+                                    ctypes.POINTER(vp)], i),
         "aa_hip_ddtr_destroy": ([vp], i),
         "aa_hip_ddtr_set_kernel": ([vp, i], i),
         "aa_hip_ddtr_device_bytes": ([vp], sz),
@@ -717,10 +719,13 @@ class DdtrPlan:
     ``execute_chunk`` runs corner turn + binning cascade + every range on
     torch device tensors: the input is [t][c] uint8 (or int16/uint16-holding
     8-bit samples), outputs are float32 [ndms_r][>= t_processed_r] tensors.
+    With ``sample_bits=16`` the plan takes [t][c] 16-bit samples of any value
+    (an int16 tensor holding their bit patterns) and sums them in float in
+    channel order, as the reference does for data wider than 8 bits.
     """
 
     def __init__(self, dmshifts: np.ndarray, ranges: Sequence[tuple], tsamp: float, maxshift: int,
-                 max_t_processed0: int, device: int = 0):
+                 max_t_processed0: int, device: int = 0, sample_bits: int = 8):
         lib = _lib_or_raise()
         self.dmshifts = np.ascontiguousarray(dmshifts, dtype=np.float32)
         self.nchans = int(self.dmshifts.size)
@@ -731,21 +736,25 @@ class DdtrPlan:
         ndms = (ctypes.c_int * n)(*[int(r[2]) for r in self.ranges])
         inbin = (ctypes.c_int * n)(*[int(r[3]) for r in self.ranges])
         h = ctypes.c_void_p()
-        check(lib.aa_hip_ddtr_create(int(device), self.nchans, self.dmshifts.ctypes.data_as(PFLOAT), n, low, step,
-                                     ndms, inbin, float(tsamp), int(maxshift), int(max_t_processed0),
-                                     ctypes.byref(h)), "aa_hip_ddtr_create")
+        if sample_bits not in (8, 16):
+            raise ValueError("sample_bits must be 8 or 16")
+        create = lib.aa_hip_ddtr_create if sample_bits == 8 else lib.aa_hip_ddtr_create_u16
+        check(create(int(device), self.nchans, self.dmshifts.ctypes.data_as(PFLOAT), n, low, step,
+                     ndms, inbin, float(tsamp), int(maxshift), int(max_t_processed0),
+                     ctypes.byref(h)), "aa_hip_ddtr_create")
+        self.sample_bits = sample_bits
         self.handle = h
         self.tsamp = tsamp
         self.maxshift = int(maxshift)
 
     @classmethod
-    def from_strategy(cls, s: aa_py_ddtr_strategy, device: int = 0) -> "DdtrPlan":
+    def from_strategy(cls, s: aa_py_ddtr_strategy, device: int = 0, sample_bits: int = 8) -> "DdtrPlan":
         ranges = []
         for r in range(s.nranges):
             low, _, step, ib, _ = s.dm(r)
             ranges.append((low, step, s.ndms(r), ib))
         max_tp0 = max(s.t_processed(0, k) for k in range(s.num_tchunks))
-        return cls(s.dmshifts(), ranges, s.tsamp, s.maxshift, max_tp0, device)
+        return cls(s.dmshifts(), ranges, s.tsamp, s.maxshift, max_tp0, device, sample_bits)
 
     def close(self):
         if getattr(self, "handle", None):

@@ -227,6 +227,19 @@ int aa_hip_ddtr_create(int device, int nchans, const float *dmshifts, int nrange
                        const float *dm_low, const float *dm_step, const int *ndms,
                        const int *inBin, double tsamp, int maxshift, int max_t_processed0,
                        aa_hip_ddtr **out);
+/* As aa_hip_ddtr_create, for samples wider than 8 bits (16-bit filterbanks,
+ * and 32-bit ones the reader scales to u16; src/aa_sigproc_input.cpp:400-433):
+ * the cascade levels stay u16 and each output is the float channel sum in
+ * channel order, as the reference's cache kernel forms it
+ * (src/aa_device_dedispersion_kernel.cu:422-450; it rounds once a sum passes
+ * 2^24).  Ingest takes input_bits 16 only; the kernel choice is the 16-bit
+ * LDS kernel (AA_HIP_KERNEL_AUTO) or the direct kernel (DIRECT, and AUTO
+ * where a tile's delay spread exceeds the LDS rows).  Not usable in
+ * batches. */
+int aa_hip_ddtr_create_u16(int device, int nchans, const float *dmshifts, int nranges,
+                           const float *dm_low, const float *dm_step, const int *ndms,
+                           const int *inBin, double tsamp, int maxshift, int max_t_processed0,
+                           aa_hip_ddtr **out);
 int aa_hip_ddtr_destroy(aa_hip_ddtr *h);
 /* Selects the dedispersion kernel variant (aa_hip_kernel) for this handle. */
 int aa_hip_ddtr_set_kernel(aa_hip_ddtr *h, int kernel);

@@ -767,3 +767,120 @@ void oracle_dedisperse_rows(const uint8_t *L, size_t pitch, int nchans, const fl
     }
     free(delays);
 }
+
+/* ------------------------------------------------------------------------
+ * Samples wider than 8 bits (16-bit files, and 32-bit files the reader
+ * scales to u16; src/aa_sigproc_input.cpp:400-433).  The device rows stay
+ * u16 and the cache kernel's float accumulator rounds once channel sums pass
+ * 2^24, so the sum is kept exactly as that kernel forms it: one float
+ * accumulator per (DM, sample), channels added in order c = 0 .. nchans-1
+ * (src/aa_device_dedispersion_kernel.cu:422-450, `local_kernel +=
+ * (float)d_input[shift]`), then / nchans / inBin.
+ * ------------------------------------------------------------------------ */
+
+/* [t][c] u16 -> [c][t] u16 (src/aa_corner_turn.cpp:118-121). */
+void oracle_transpose_u16(const uint16_t *in_tc, int nsamp, int nchans, uint16_t *out_ct, size_t out_pitch)
+{
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nchans; c++)
+        for (int t = 0; t < nsamp; t++) out_ct[(size_t)c * out_pitch + t] = in_tc[(size_t)t * nchans + c];
+}
+
+/* One x2 level of u16 rows: the bin kernel's ((float)a + (float)b) / 2.0f
+ * (exact for u16 operands) truncated back by swap
+ * (src/aa_device_binning_kernel.cu:33-45, src/aa_device_corner_turn_kernel.cu:91-98). */
+void oracle_bin_u16(const uint16_t *in, size_t in_pitch, int nchans, int n_out, uint16_t *out, size_t out_pitch)
+{
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nchans; c++) {
+        const uint16_t *row = in + (size_t)c * in_pitch;
+        uint16_t *orow = out + (size_t)c * out_pitch;
+        for (int t = 0; t < n_out; t++)
+            orow[t] = (uint16_t)(((float)row[2 * t] + (float)row[2 * t + 1]) / 2.0f);
+    }
+}
+
+void oracle_dedisperse_rows_u16(const uint16_t *L, size_t pitch, int nchans, const float *dmshifts,
+                                const int *dm_idx, int nrows, int t_processed, float mstartdm, float mdmstep,
+                                int inBin, float *out, size_t out_pitch)
+{
+    const int nblk = (t_processed + ORACLE_TBLOCK - 1) / ORACLE_TBLOCK;
+    int *delays = (int *)malloc(sizeof(int) * (size_t)nchans * (size_t)(nrows > 0 ? nrows : 1));
+    for (int k = 0; k < nrows; k++)
+        for (int c = 0; c < nchans; c++)
+            delays[(size_t)k * nchans + c] = oracle_delay(dmshifts[c], dm_idx[k], mstartdm, mdmstep);
+    const float fn = (float)nchans, fb = (float)inBin;
+#pragma omp parallel
+    {
+        float acc[ORACLE_TBLOCK];
+#pragma omp for schedule(dynamic, 1)
+        for (long item = 0; item < (long)nrows * nblk; item++) {
+            const int k = (int)(item / nblk);
+            const int t0 = (int)(item % nblk) * ORACLE_TBLOCK;
+            const int n = t_processed - t0 < ORACLE_TBLOCK ? t_processed - t0 : ORACLE_TBLOCK;
+            const int *dk = delays + (size_t)k * nchans;
+            for (int t = 0; t < n; t++) acc[t] = 0.0f;
+            for (int c = 0; c < nchans; c++) {   /* channel order: the float sum rounds */
+                const uint16_t *row = L + (size_t)c * pitch + (size_t)(t0 + dk[c]);
+                for (int t = 0; t < n; t++) acc[t] = acc[t] + (float)row[t];
+            }
+            float *orow = out + (size_t)k * out_pitch + t0;
+            for (int t = 0; t < n; t++) orow[t] = (acc[t] / fn) / fb;
+        }
+    }
+    free(delays);
+}
+
+/* oracle_ddtr_run_u8's sequencing on u16 levels (input_tc: u16 [t][c]). */
+int oracle_ddtr_run_u16(const oracle_strategy *s, const uint16_t *input_tc, float **out)
+{
+    if (!s->ok) return -1;
+    const int nchans = s->nchans;
+    const float tsamp_original = (float)s->tsamp;
+    long inc = 0;
+    for (int r = 0; r < s->nranges; r++) {
+        int ib = s->dm[r].inBin;
+        if (ib < 1 || (ib & (ib - 1)) != 0) return -2;
+        if (r > 0 && ib < s->dm[r - 1].inBin) return -3;
+    }
+    for (int k = 0; k < s->num_tchunks; k++) {
+        const int tp0 = oracle_strategy_t_processed(s, 0, k);
+        const long len0 = (long)tp0 + s->maxshift;
+        if (inc + len0 > s->nsamp) return -4;
+        size_t pitch = (size_t)len0;
+        uint16_t *level = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)nchans * pitch);
+        oracle_transpose_u16(input_tc + (size_t)inc * nchans, (int)len0, nchans, level, pitch);
+        long level_len = len0;
+        int bin = 1;
+        for (int r = 0; r < s->nranges; r++) {
+            const int ib = s->dm[r].inBin;
+            while (bin < ib) {
+                long n_out = level_len / 2;
+                uint16_t *next = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)nchans * (size_t)(n_out > 0 ? n_out : 1));
+                oracle_bin_u16(level, pitch, nchans, (int)n_out, next, (size_t)n_out);
+                free(level);
+                level = next;
+                pitch = (size_t)n_out;
+                level_len = n_out;
+                bin *= 2;
+            }
+            const int tp = oracle_strategy_t_processed(s, r, k);
+            const int maxshift_r = s->maxshift / ib;
+            if ((long)tp + maxshift_r > level_len) { free(level); return -5; }
+            float mstartdm, mdmstep;
+            oracle_range_scalars(tsamp_original, ib, s->dm[r].low, s->dm[r].step, &mstartdm, &mdmstep);
+            long total_r = 0;
+            for (int j = 0; j < s->num_tchunks; j++) total_r += oracle_strategy_t_processed(s, r, j);
+            const long col = inc / ib;
+            if (col + tp > total_r) { free(level); return -6; }
+            int *idx = (int *)malloc(sizeof(int) * (size_t)(s->ndms[r] > 0 ? s->ndms[r] : 1));
+            for (int d = 0; d < s->ndms[r]; d++) idx[d] = d;
+            oracle_dedisperse_rows_u16(level, pitch, nchans, s->dmshifts, idx, s->ndms[r], tp, mstartdm, mdmstep, ib,
+                                       out[r] + col, (size_t)total_r);
+            free(idx);
+        }
+        free(level);
+        inc += tp0;
+    }
+    return 0;
+}

@@ -137,6 +137,17 @@ void oracle_dedisperse_rows(const uint8_t *L, size_t pitch, int nchans, const fl
 int oracle_ddtr_run_u8(const oracle_strategy *s, const uint8_t *input_tc,
                        float **out);
 
+/* Samples wider than 8 bits, kept as u16 rows: the transpose, one x2 level
+ * (truncated float mean, exact for u16), and the float channel sum in
+ * channel order, (acc / nchans) / inBin, as the reference's cache kernel
+ * forms it; oracle_ddtr_run_u16 runs oracle_ddtr_run_u8's sequence on them. */
+void oracle_transpose_u16(const uint16_t *in_tc, int nsamp, int nchans, uint16_t *out_ct, size_t out_pitch);
+void oracle_bin_u16(const uint16_t *in, size_t in_pitch, int nchans, int n_out, uint16_t *out, size_t out_pitch);
+void oracle_dedisperse_rows_u16(const uint16_t *L, size_t pitch, int nchans, const float *dmshifts,
+                                const int *dm_idx, int nrows, int t_processed, float mstartdm, float mdmstep,
+                                int inBin, float *out, size_t out_pitch);
+int oracle_ddtr_run_u16(const oracle_strategy *s, const uint16_t *input_tc, float **out);
+
 /* Synthetic data: SplitMix64 counter PRNG + Box-Muller (double), rounded to
  * nearest and clipped to [0,255].  value(t,c) depends only on (seed, t, c). */
 void oracle_gen_noise_u8(uint64_t seed, int nsamp, int nchans, double mean,

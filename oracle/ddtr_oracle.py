@@ -65,6 +65,10 @@ def lib() -> ctypes.CDLL:
             "oracle_dedisperse_range": ([vp, sz, i, i, pf, i, i, f, f, i, pf, sz], None),
             "oracle_dedisperse_rows": ([vp, sz, i, pf, pi, i, i, f, f, i, pf, sz], None),
             "oracle_ddtr_run_u8": ([ctypes.POINTER(_Strategy), vp, ctypes.POINTER(pf)], i),
+            "oracle_transpose_u16": ([vp, i, i, vp, sz], None),
+            "oracle_bin_u16": ([vp, sz, i, i, vp, sz], None),
+            "oracle_dedisperse_rows_u16": ([vp, sz, i, pf, pi, i, i, f, f, i, pf, sz], None),
+            "oracle_ddtr_run_u16": ([ctypes.POINTER(_Strategy), vp, ctypes.POINTER(pf)], i),
             "oracle_gen_noise_u8": ([ctypes.c_uint64, i, i, d, d, vp], None),
             "oracle_inject_pulse_u8": ([vp, i, i, pf, d, d, i, i, i], None),
             "oracle_aa_mask": ([f, i, pf], i),
@@ -143,11 +147,14 @@ def strategy(ranges: Sequence[Sequence[float]], nchans: int, nsamp: int, fch1: f
 
 
 def run(s: Strategy, input_tc: np.ndarray) -> List[np.ndarray]:
-    """Whole DDTR of a u8 [t][c] input: one float32 [ndms][total_t] plane per range."""
-    assert input_tc.dtype == np.uint8 and input_tc.flags.c_contiguous
+    """Whole DDTR of a u8 [t][c] input (or a u16 one: samples wider than 8
+    bits, float channel sums in channel order): one float32 [ndms][total_t]
+    plane per range."""
+    assert input_tc.dtype in (np.uint8, np.uint16) and input_tc.flags.c_contiguous
     planes = [np.zeros((s.ndms[r], sum(s.t_processed[r])), dtype=np.float32) for r in range(s.nranges)]
     ptrs = (ctypes.POINTER(ctypes.c_float) * s.nranges)(*[_pf(p) for p in planes])
-    st = lib().oracle_ddtr_run_u8(ctypes.byref(s._raw), _vp(input_tc), ptrs)
+    fn = lib().oracle_ddtr_run_u8 if input_tc.dtype == np.uint8 else lib().oracle_ddtr_run_u16
+    st = fn(ctypes.byref(s._raw), _vp(input_tc), ptrs)
     if st != 0:
         raise RuntimeError(f"oracle run failed ({st})")
     return planes
@@ -247,6 +254,33 @@ def dedisperse_rows(level: np.ndarray, pitch: int, nchans: int, dmshifts_: np.nd
     out = np.zeros((idx.size, t_processed), dtype=np.float32)
     lib().oracle_dedisperse_rows(_vp(level), pitch, nchans, _pf(ds), _pi(idx), idx.size, t_processed,
                                  mstartdm, mdmstep, inBin, _pf(out), t_processed)
+    return out
+
+
+def transpose_u16(input_tc: np.ndarray, nsamp: int, nchans: int) -> np.ndarray:
+    """[t][c] u16 -> [c][t] u16 (samples wider than 8 bits keep their width)."""
+    x = np.ascontiguousarray(input_tc, dtype=np.uint16)
+    out = np.zeros((nchans, nsamp), dtype=np.uint16)
+    lib().oracle_transpose_u16(_vp(x), nsamp, nchans, _vp(out), nsamp)
+    return out
+
+
+def bin2_u16(level: np.ndarray, n_out: int) -> np.ndarray:
+    level = np.ascontiguousarray(level, dtype=np.uint16)
+    out = np.zeros((level.shape[0], n_out), dtype=np.uint16)
+    lib().oracle_bin_u16(_vp(level), level.shape[1], level.shape[0], n_out, _vp(out), n_out)
+    return out
+
+
+def dedisperse_rows_u16(level: np.ndarray, pitch: int, nchans: int, dmshifts_: np.ndarray, dm_idx: Sequence[int],
+                        t_processed: int, mstartdm: float, mdmstep: float, inBin: int) -> np.ndarray:
+    """DM rows of u16 [c][t] rows (pitch in samples): float sums in channel order."""
+    lv = np.ascontiguousarray(level, dtype=np.uint16)
+    ds = np.ascontiguousarray(dmshifts_, dtype=np.float32)
+    idx = np.ascontiguousarray(dm_idx, dtype=np.int32)
+    out = np.zeros((idx.size, t_processed), dtype=np.float32)
+    lib().oracle_dedisperse_rows_u16(_vp(lv), pitch, nchans, _pf(ds), _pi(idx), idx.size, t_processed,
+                                     mstartdm, mdmstep, inBin, _pf(out), t_processed)
     return out
 
 
