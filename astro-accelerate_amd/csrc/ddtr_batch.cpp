@@ -55,9 +55,9 @@ int ddtr_batch::create(int nbeams, ddtr_engine *const *beams, ddtr_batch **out) 
     hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->m_device);
     // one 1024-thread work-group per CU (its LDS use fills the CU)
     p->m_grid = 8 * std::max(1, (ncu + 7) / 8);
-    for (auto &f : p->m_fam) {
-        if (e == hipSuccess) e = hipMalloc(&f.d_bounds, 9 * sizeof(long));
-        if (e == hipSuccess) e = hipMalloc(&f.d_counters, 8 * sizeof(int));
+    for (int f = 0; f < kFamilies; ++f) {
+        if (e == hipSuccess) e = hipMalloc(&p->m_fam[f].d_bounds, 9 * sizeof(long));
+        if (e == hipSuccess && f == 0) e = hipMalloc(&p->m_fam[f].d_counters, 8 * sizeof(int));
     }
     if (e != hipSuccess) {
         const int st = hip_fail(e, "batch create");
@@ -77,6 +77,14 @@ ddtr_batch::~ddtr_batch() {
         if (f.d_counters) (void)hipFree(f.d_counters);
     }
     if (m_capture_stream) (void)hipStreamDestroy(m_capture_stream);
+}
+
+int ddtr_batch::family_of(int variant, int row_quads) {
+    if (variant == 0 || variant == 1) return 0;
+    const int J = variant == 4 ? 4 : variant == 5 ? 3 : variant == 2 ? 2 : 0;
+    const int ni = std::max(J, (row_quads + 63) / 64);   // the instantiation launch_dedisperse_wide picks
+    if (J == 0 || row_quads <= 0 || ni > 5) return -1;
+    return 1 + 4 * (J - 2) + (ni - 2);
 }
 
 void ddtr_batch::release_graph() {
@@ -105,15 +113,13 @@ int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float
             st = e->prepare_range(r, t_processed[flat], d_out[flat], out_pitch[flat], s, &rl);
             if (st != AA_HIP_SUCCESS) return st;
             if (rl.ndms <= 0 || rl.t_processed <= 0) continue;
-            // A persistent kernel holding the pair shape beside the byte-row
-            // shapes, or by itself, measured 10 % slower per tile than its
-            // launches per range: those ranges keep their own launches.
-            if (rl.variant == 0 || rl.variant == 1) {
+            const int f = rl.sample_bytes == 1 ? family_of(rl.variant, rl.row_quads) : -1;
+            if (f >= 0) {
                 lds_job j{};
                 j.p = lds_params_of(rl);
                 j.variant = rl.variant;
                 lds_job_tiles(j.p, rl.variant);
-                jobs[0].push_back(j);
+                jobs[f].push_back(j);
             } else {
                 direct.push_back(rl);
             }
@@ -145,13 +151,22 @@ int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float
     return AA_HIP_SUCCESS;
 }
 
-// The persistent launch (when it has jobs), then the separate launches.
-// Reads only device-side job tables: capturable.
+// The persistent byte-row launch and the pair families' launches (those
+// with jobs), then the separate launches.  Reads only device-side job tables
+// (the host copies of the queue bounds size the grids): capturable.
 hipError_t ddtr_batch::launch_families(hipStream_t s) {
     hipError_t e = hipSuccess;
     if (!m_fam[0].jobs.empty())
         e = launch_dedisperse_lds_batch(m_fam[0].d_jobs, (int)m_fam[0].jobs.size(), m_fam[0].d_bounds,
                                         m_fam[0].d_counters, m_grid, s);
+    for (int f = 1; e == hipSuccess && f < kFamilies; ++f) {
+        if (m_fam[f].jobs.empty()) continue;
+        const int J = 2 + (f - 1) / 4, ni = 2 + (f - 1) % 4;
+        long longest = 0;
+        for (int x = 0; x < 8; ++x) longest = std::max(longest, m_fam[f].bounds[(size_t)x + 1] - m_fam[f].bounds[(size_t)x]);
+        e = launch_dedisperse_wide_batch(m_fam[f].d_jobs, (int)m_fam[f].jobs.size(), m_fam[f].d_bounds, longest, J, ni,
+                                         s);
+    }
     for (size_t i = 0; e == hipSuccess && i < m_direct.size(); ++i) e = launch_range(m_direct[i], s);
     return e;
 }

@@ -5,11 +5,14 @@
 //
 // Per call: every beam's chunk is corner-turned into its engine's level 0 and
 // binned as its ranges need (the engines' own kernels).  The (beam, range)
-// jobs on the two byte-row chunk shapes run in one persistent launch over a
-// shared tile list (dedisperse_lds_batch_kernel); ranges on the pair or quad
-// pre-summation shapes, or on the direct kernel (no LDS shape fits), run as
-// launches of their own after it (captured in the same graph).  Results are
-// bit-identical to running each engine's execute_chunk.
+// jobs then run in launches over shared tile queues (8, one per XCD), one
+// launch per kernel instantiation: the two byte-row chunk shapes together
+// in a persistent launch (dedisperse_lds_batch_kernel), and the pair
+// pre-summation shapes grouped by tile width and row width, one tile per
+// work-group (dedisperse_wide_batch_kernel<NI, J>).  Ranges
+// on the quad shape or on the direct kernel (no LDS shape fits) run as
+// launches of their own after them (captured in the same graph).  Results
+// are bit-identical to running each engine's execute_chunk.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,7 +38,7 @@ public:
                 bool use_graph, hipStream_t s);
 
     // jobs (ranges) and tiles in the persistent launches, ranges launched
-    // on their own
+    // on their own, persistent launches
     int njobs() const {
         size_t n = 0;
         for (const auto &f : m_fam) n += f.jobs.size();
@@ -47,6 +50,11 @@ public:
         return n;
     }
     int ndirect() const { return (int)m_direct.size(); }
+    int nlaunches() const {
+        int n = 0;
+        for (const auto &f : m_fam) n += f.jobs.empty() ? 0 : 1;
+        return n;
+    }
     int nranges_total() const { return m_nranges_total; }
 
 private:
@@ -60,7 +68,10 @@ private:
     int m_grid = 0;
     std::vector<ddtr_engine *> m_beams;
     int m_nranges_total = 0;
-    // The persistent launch's jobs (byte-row shapes).
+    // One batched launch's jobs.  Family 0: the byte-row shapes (chunk
+    // shapes 0 and 1, dedisperse_lds_batch_kernel); family 1 + 4 (J - 2) +
+    // (NI - 2): the pair shapes of tile width J (2: shape 2, 3: shape 5, 4:
+    // shape 4) and NI quads per lane of a row (dedisperse_wide_batch_kernel).
     struct family {
         std::vector<lds_job> jobs;
         std::vector<long> bounds;  // 9 queue bounds (host copy)
@@ -68,9 +79,10 @@ private:
         lds_job *d_jobs = nullptr;
         size_t jobs_cap = 0;
         long *d_bounds = nullptr;  // 9 queue bounds
-        int *d_counters = nullptr; // 8 queue heads
+        int *d_counters = nullptr; // 8 queue heads (family 0)
     };
-    static constexpr int kFamilies = 1;
+    static constexpr int kFamilies = 1 + 3 * 4;
+    static int family_of(int variant, int row_quads);   // -1: launched on its own
     family m_fam[kFamilies];
     std::vector<range_launch> m_direct;
     hipStream_t m_capture_stream = nullptr;  // for callers on the legacy default stream

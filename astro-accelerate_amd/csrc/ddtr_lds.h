@@ -52,7 +52,7 @@ __device__ __forceinline__ float out_value(uint32_t sum, const lds_params& p)
 // part in queue x (queue positions from 0).
 struct lds_job {
     lds_params p;
-    int variant;             // chunk shape 0 or 1 (the shapes dedisperse_lds_batch holds)
+    int variant;             // chunk shape
     int pad;
     long qend[8];
 };
@@ -93,6 +93,15 @@ void lds_job_tiles(lds_params& p, int variant);
 // graph).  grid: multiple of 8.
 hipError_t launch_dedisperse_lds_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, int* d_counters,
                                        int grid, hipStream_t s);
+
+// One launch over pair-shape jobs (chunk shapes 2, 4, 5) that share the
+// tile width J (2, 3, 4 blocks: shapes 2, 5, 4) and the row width NI = ni
+// quads per lane (the instantiation a launch per range would use: ni of at
+// least J); jobs in queue order and d_bounds as for
+// launch_dedisperse_lds_batch; max_queue: the longest queue (tiles).  Each
+// work-group takes one tile of its XCD's queue.
+hipError_t launch_dedisperse_wide_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, long max_queue,
+                                        int J, int ni, hipStream_t s);
 
 
 }  // namespace aa_hip

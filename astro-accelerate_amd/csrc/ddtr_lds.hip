@@ -362,24 +362,6 @@ __device__ __forceinline__ int batch_xcd()
     return (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 7u);
 }
 
-// Job and tile of position `pos` of queue x (see lds_job): the first job
-// whose part in queue x ends after pos.
-__device__ __forceinline__ const lds_job& batch_job_of(const lds_job* __restrict__ jobs, int njobs, int x, long pos,
-                                                       long* local)
-{
-    int lo = 0, hi = njobs - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (jobs[mid].qend[x] > pos) hi = mid;
-        else lo = mid + 1;
-    }
-    const lds_job& J = jobs[lo];
-    const long n = (long)J.p.n_dm_tiles * J.p.n_t_tiles;
-    const long b = x * n / 8, e = (x + 1) * n / 8;
-    *local = b + pos - (J.qend[x] - (e - b));
-    return J;
-}
-
 // ---------------------------------------------------------------------------
 // Quad pre-summation (chunk shape 3).  As the pair shape, with rows that sum
 // the four channels of a group: in a 16-DM tile the three delays of channels

@@ -227,6 +227,25 @@ __device__ __forceinline__ void pair_flush(uint32_t (&acc16)[R / 2], uint32_t (&
     }
 }
 
+// Job and tile of position `pos` of batch queue x (see lds_job): the first
+// job whose part in queue x ends after pos; *local = the tile's index in
+// that job.
+__device__ __forceinline__ const lds_job& batch_job_of(const lds_job* __restrict__ jobs, int njobs, int x, long pos,
+                                                       long* local)
+{
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (jobs[mid].qend[x] > pos) hi = mid;
+        else lo = mid + 1;
+    }
+    const lds_job& J = jobs[lo];
+    const long n = (long)J.p.n_dm_tiles * J.p.n_t_tiles;
+    const long b = x * n / 8, e = (x + 1) * n / 8;
+    *local = b + pos - (J.qend[x] - (e - b));
+    return J;
+}
+
 }  // namespace
 
 // Launches the 16-byte-read pair kernel with J blocks per lane (4: shape 4,

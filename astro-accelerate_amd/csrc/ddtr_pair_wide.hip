@@ -280,7 +280,63 @@ __global__ __launch_bounds__(kThreads) void dedisperse_wide_kernel(lds_params p)
     process_wide_tile<NI, J>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * (2 * kWBlk * J));
 }
 
+// Many pair-shape ranges (of many beams) sharing J and NI in one launch.
+// The 8 queues are those of the byte-row batch kernel: queue x holds the
+// x-th eighth of every job, job after job, so the XCDs work through the jobs
+// side by side as launches per range would.  Work-group g runs on XCD g % 8
+// (work-groups are dispatched to the XCDs round robin) and processes
+// position g / 8 of queue g % 8: the hardware dispatcher hands each XCD its
+// queue's tiles in order, so neighbouring DM tiles of one time tile run
+// together and share their input rows in that XCD's L2, as in a launch per
+// range.  One tile per work-group: a loop over tiles in each work-group
+// (a persistent grid) keeps its loop state beside the tile's and, at
+// J = 4, raises the kernel from 56 SGPRs / 121 VGPRs to 106 / 128 with
+// scratch spills.  The job table is read through the constant address
+// space, so its fields load into scalar registers as kernel arguments do.
+template <int NI, int J>
+__global__ __launch_bounds__(kThreads) void dedisperse_wide_batch_kernel(const lds_job* __restrict__ jobs, int njobs,
+                                                                         const long* __restrict__ bounds)
+{
+    typedef const __attribute__((address_space(4))) lds_job const_job;
+    const int x = (int)(blockIdx.x & 7u);
+    const long pos = (long)(blockIdx.x >> 3);
+    if (pos >= bounds[x + 1] - bounds[x]) return;   // whole work-group leaves before any barrier
+    long local;
+    const lds_job& j0 = batch_job_of(jobs, njobs, x, pos, &local);
+    const const_job& jb = ((const const_job*)jobs)[&j0 - jobs];
+    process_wide_tile<NI, J>((const lds_params&)jb.p, (int)(local % jb.p.n_dm_tiles),
+                             (local / jb.p.n_dm_tiles) * (2L * kWBlk * J));
+}
+
 }  // namespace
+
+hipError_t launch_dedisperse_wide_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, long max_queue,
+                                        int J, int ni, hipStream_t s)
+{
+    if (njobs <= 0 || max_queue <= 0) return hipSuccess;
+    if (!d_jobs || !d_bounds || max_queue > 0x0fffffffL) return hipErrorInvalidValue;
+    const long grid = 8 * max_queue;
+    const dim3 g((unsigned)grid), b(kThreads);
+#define AA_WIDE_BATCH(NI_, J_) \
+    hipLaunchKernelGGL((dedisperse_wide_batch_kernel<NI_, J_>), g, b, 0, s, d_jobs, njobs, d_bounds)
+    if (J == 4) {
+        if (ni <= 4) AA_WIDE_BATCH(4, 4);
+        else AA_WIDE_BATCH(5, 4);
+    } else if (J == 3) {
+        if (ni <= 3) AA_WIDE_BATCH(3, 3);
+        else if (ni == 4) AA_WIDE_BATCH(4, 3);
+        else AA_WIDE_BATCH(5, 3);
+    } else if (J == 2) {
+        if (ni <= 2) AA_WIDE_BATCH(2, 2);
+        else if (ni == 3) AA_WIDE_BATCH(3, 2);
+        else if (ni == 4) AA_WIDE_BATCH(4, 2);
+        else AA_WIDE_BATCH(5, 2);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef AA_WIDE_BATCH
+    return hipGetLastError();
+}
 
 hipError_t launch_dedisperse_wide(const lds_params& q, int J, int ni, unsigned grid, hipStream_t s)
 {

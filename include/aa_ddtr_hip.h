@@ -291,10 +291,12 @@ int aa_hip_ddtr_autotune_range(aa_hip_ddtr *h, int range, int t_processed, float
  * it and share one device.  execute: per beam b, d_inputs_tc[b] is its
  * [t][c] u8 chunk; t_processed, d_out and out_pitch are flattened
  * beam-major over each beam's ranges (as execute_chunk takes them per beam).
- * Every beam is corner-turned and binned, then the ranges on the two
- * byte-row LDS shapes run in one persistent launch over a shared tile list;
- * ranges on the pair or quad pre-summation shapes, or that no chunk shape
- * fits, run launches of their own after it (in the same graph).  With use_graph != 0 the
+ * Every beam is corner-turned and binned, then all beams' ranges run in one
+ * launch per kernel instantiation over shared tile queues: the two byte-row
+ * LDS shapes in one persistent launch, the pair pre-summation shapes in one
+ * launch per tile width and row width; ranges on the quad shape, or that no
+ * chunk shape fits, run launches of their own after them (in the same
+ * graph).  With use_graph != 0 the
  * sequence is captured into a hipGraph on the first call for a given set of
  * buffers and replayed by later calls with the same buffers.  Results are
  * bit-identical to aa_hip_ddtr_execute_chunk per beam.  Calls on one batch
@@ -305,8 +307,8 @@ int aa_hip_ddtr_batch_destroy(aa_hip_ddtr_batch *h);
 int aa_hip_ddtr_batch_execute(aa_hip_ddtr_batch *h, const void *const *d_inputs_tc,
                               const int *t_processed, float *const *d_out,
                               const size_t *out_pitch, int use_graph, void *stream);
-/* Jobs (ranges) in the persistent launch, their tiles, and ranges launched
- * on their own (pre-summation shapes or direct kernel) in the last execute. */
+/* Jobs (ranges) in the batched launches, their tiles, and ranges launched
+ * on their own (quad shape or direct kernel) in the last execute. */
 int aa_hip_ddtr_batch_stats(const aa_hip_ddtr_batch *h, int *njobs, long *ntiles, int *ndirect);
 
 /* Replaces corner_turn_SM_inplace (src/aa_corner_turn.cpp:118-121) for the
