@@ -524,6 +524,23 @@ int aa_hip_dedisperse_u8(const uint8_t *d_in, size_t pitch, int row_len, int nch
 
 size_t aa_hip_spd_workspace_bytes(void) { return aa_hip::spd_workspace_bytes(); }
 
+size_t aa_hip_spd_noalloc_scratch_bytes(void) { return aa_hip::spd_static_scratch_bytes(); }
+
+int aa_hip_spd_first_pass_noalloc(const float *d_plane, size_t pitch, int ndms, int nsamp, int max_width,
+                                  float *d_msd, float *d_snr, size_t snr_pitch, uint16_t *d_width,
+                                  size_t width_pitch, void *stream) {
+    if (!d_plane || !d_msd) return bad("NULL buffer");
+    if (ndms <= 0 || nsamp <= 0 || pitch < (size_t)nsamp) return bad("invalid plane geometry");
+    if (max_width < 1 || max_width > AA_HIP_SPD_MAX_WIDTH || max_width > nsamp)
+        return bad("invalid max_width: the static workspace holds moments for widths 1..32 (and max_width <= nsamp)");
+    if ((d_snr && snr_pitch < (size_t)nsamp) || (d_width && width_pitch < (size_t)nsamp)) return bad("invalid output pitch");
+    if (aa_hip::spd_workspace_bytes() > aa_hip::spd_static_scratch_bytes())
+        return bad("the first pass needs more workspace than the static scratch array holds");
+    return from_hip(aa_hip::launch_spd_first_pass_static(d_plane, pitch, ndms, nsamp, max_width, d_msd, d_snr,
+                                                         snr_pitch, d_width, width_pitch, (hipStream_t)stream),
+                    "single-pulse first pass (no allocation)");
+}
+
 int aa_hip_spd_first_pass(const float *d_plane, size_t pitch, int ndms, int nsamp, int max_width, float *d_msd,
                           float *d_snr, size_t snr_pitch, uint16_t *d_width, size_t width_pitch, void *d_work,
                           size_t work_bytes, void *stream) {

@@ -121,6 +121,13 @@ size_t spd_workspace_bytes();
 hipError_t launch_spd_first_pass(const float* plane, size_t pitch, int ndms, int nsamp, int W, float* msd,
                                  float* snr, size_t snr_pitch, uint16_t* width, size_t width_pitch, void* work,
                                  hipStream_t s);
+// The same pass with its workspace in a file-scope device array (one per
+// device, spd_static_scratch_bytes() bytes): allocates nothing.  Calls that
+// overlap on one device share that array and race.
+size_t spd_static_scratch_bytes();
+hipError_t launch_spd_first_pass_static(const float* plane, size_t pitch, int ndms, int nsamp, int W, float* msd,
+                                        float* snr, size_t snr_pitch, uint16_t* width, size_t width_pitch,
+                                        hipStream_t s);
 
 // Synthetic 8-bit filterbank noise: N(mean, sigma^2) rounded and clipped to
 // [0,255], SplitMix64 counter stream keyed by seed (value depends only on

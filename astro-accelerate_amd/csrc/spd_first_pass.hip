@@ -31,6 +31,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "ddtr_kernels.h"
 
@@ -263,6 +264,53 @@ hipError_t launch_spd_first_pass(const float* plane, size_t pitch, int ndms, int
         hipLaunchKernelGGL(spd_snr_kernel, dim3((unsigned)ntiles), dim3(kThreads), 0, s, plane, (long)pitch, nsamp,
                            W, (const spd_aux*)aux, snr, (long)snr_pitch, width, (long)width_pitch);
     return hipGetLastError();
+}
+
+// ------------------------------------------------ allocation-free variant
+// Workspace of launch_spd_first_pass_static: a file-scope device array, one
+// copy per device, held until the process ends.  The first pass needs the
+// same fixed workspace for every plane (the persistent MSD grid's per-width
+// moments and the finalize constants), so this size covers every call whose
+// width limit is at most SPD_MAX_WIDTH.  Every byte a call reads is written
+// earlier in the same call, so nothing needs clearing between calls.
+constexpr size_t kSpdScratchBytes = (size_t)kBlocks * kMaxW * 3 * sizeof(double) + 256 + sizeof(spd_aux);
+static __device__ __align__(256) unsigned char scratch[kSpdScratchBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kScratchDevices] = {};   // device address of `scratch`, per device
+
+// The current device's address of `scratch`, looked up once per device.
+hipError_t scratch_address(void** out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+size_t spd_static_scratch_bytes() { return kSpdScratchBytes; }
+
+hipError_t launch_spd_first_pass_static(const float* plane, size_t pitch, int ndms, int nsamp, int W, float* msd,
+                                        float* snr, size_t snr_pitch, uint16_t* width, size_t width_pitch,
+                                        hipStream_t s)
+{
+    if (spd_workspace_bytes() > kSpdScratchBytes) return hipErrorInvalidValue;
+    void* work = nullptr;
+    const hipError_t e = scratch_address(&work);
+    if (e != hipSuccess) return e;
+    return launch_spd_first_pass(plane, pitch, ndms, nsamp, W, msd, snr, snr_pitch, width, width_pitch, work, s);
 }
 
 }  // namespace aa_hip

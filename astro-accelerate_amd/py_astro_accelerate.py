@@ -149,6 +149,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_py_sigproc_input_input_buffer": ([vp], ctypes.POINTER(ctypes.c_ushort)),
         "aa_hip_spd_workspace_bytes": ([], sz),
         "aa_hip_spd_first_pass": ([vp, sz, i, i, i, vp, vp, sz, vp, sz, vp, sz, vp], i),
+        "aa_hip_spd_noalloc_scratch_bytes": ([], sz),
+        "aa_hip_spd_first_pass_noalloc": ([vp, sz, i, i, i, vp, vp, sz, vp, sz, vp], i),
         "aa_py_ddtr_strategy_create": ([vp, filterbank_metadata_struct, sz, b], vp),
         "aa_py_ddtr_strategy_ready": ([vp], b),
         "aa_py_ddtr_strategy_maxshift": ([vp], i),
@@ -973,6 +975,21 @@ def spd_first_pass(d_plane, max_width: int = SPD_MAX_WIDTH, with_snr: bool = Tru
     spd_first_pass_raw(d_plane.data_ptr(), d_plane.stride(0), ndms, nsamp, max_width, msd, snr, width, work,
                        s.cuda_stream)
     return msd, snr, width
+
+
+def spd_first_pass_noalloc(d_plane, msd, snr=None, width=None, max_width: int = SPD_MAX_WIDTH, stream=None):
+    """aa_hip_spd_first_pass_noalloc: the first pass into caller-given output
+    tensors (msd float32 [max_width][2], snr float32 [ndms][>= nsamp] or None,
+    width int16 [ndms][>= nsamp] or None), with the library's static device
+    workspace; nothing is allocated."""
+    import torch
+    ndms, nsamp = d_plane.shape
+    s = stream if stream is not None else torch.cuda.current_stream(d_plane.device)
+    check(_lib_or_raise().aa_hip_spd_first_pass_noalloc(
+        ctypes.c_void_p(d_plane.data_ptr()), d_plane.stride(0), int(ndms), int(nsamp), int(max_width), _ptr(msd),
+        _ptr(snr) if snr is not None else None, snr.stride(0) if snr is not None else 0,
+        _ptr(width) if width is not None else None, width.stride(0) if width is not None else 0,
+        ctypes.c_void_p(s.cuda_stream)), "spd_first_pass_noalloc")
 
 
 def bandpass_average(data_tc: np.ndarray) -> np.ndarray:

@@ -411,6 +411,21 @@ size_t aa_hip_spd_workspace_bytes(void);
 int aa_hip_spd_first_pass(const float *d_plane, size_t pitch, int ndms, int nsamp, int max_width, float *d_msd,
                           float *d_snr, size_t snr_pitch, uint16_t *d_width, size_t width_pitch, void *d_work,
                           size_t work_bytes, void *stream);
+/* No counterpart in the reference: aa_hip_spd_first_pass for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments without the
+ * workspace, same results bit for bit.  The workspace is a file-scope device
+ * array of aa_hip_spd_noalloc_scratch_bytes() bytes (786,960: the fixed
+ * workspace of the pass for any plane size and max_width <= 32), one copy per
+ * device, held until the process ends; its address is looked up on the first
+ * call on each device (the current device).  max_width outside 1..32 is
+ * rejected.  Calls on one device share the array: calls that overlap (two
+ * streams, two host threads without ordering) race on it and must be
+ * serialised by the caller.  Allocates nothing, asynchronous on `stream`. */
+size_t aa_hip_spd_noalloc_scratch_bytes(void);
+int aa_hip_spd_first_pass_noalloc(const float *d_plane, size_t pitch, int ndms, int nsamp, int max_width,
+                                  float *d_msd, float *d_snr, size_t snr_pitch, uint16_t *d_width,
+                                  size_t width_pitch, void *stream);
 
 /* Synthetic input on the device (no reference counterpart; benchmarks and
  * tests).  Noise: N(mean, sigma^2) rounded to nearest and clipped to
