@@ -30,9 +30,11 @@ struct aa_hip_refine {
     int device = 0;
     int ncand = 0;
     int grid = 0;
+    int loss_kind = 0;
     aa_hip::refine_cand* d_cands = nullptr;
     int* d_order = nullptr;
     int* d_counter = nullptr;
+    double* d_power = nullptr;   // channel powers of every candidate (loss 1)
 };
 
 namespace {
@@ -228,6 +230,7 @@ int aa_hip_refine_create(int device, int ncand, const int64_t* x_offsets, const 
     if (ncand <= 0 || !x_offsets || !ds_offsets || !nchans || !nsamp)
         return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "invalid candidate table");
     std::vector<aa_hip::refine_cand> cands((size_t)ncand);
+    int64_t power_entries = 0;
     for (int b = 0; b < ncand; ++b) {
         if (nchans[b] <= 0 || nchans[b] > aa_hip::kRefineMaxChannels || nsamp[b] <= 0 ||
             nsamp[b] > aa_hip::kRefineMaxSamples || x_offsets[b] < 0 || ds_offsets[b] < 0)
@@ -235,7 +238,8 @@ int aa_hip_refine_create(int device, int ncand, const int64_t* x_offsets, const 
                                                       ": windows up to " + std::to_string(aa_hip::kRefineMaxChannels) +
                                                       " channels x " + std::to_string(aa_hip::kRefineMaxSamples) +
                                                       " samples, non-negative offsets");
-        cands[(size_t)b] = aa_hip::refine_cand{x_offsets[b], ds_offsets[b], nchans[b], nsamp[b]};
+        cands[(size_t)b] = aa_hip::refine_cand{x_offsets[b], ds_offsets[b], power_entries, nchans[b], nsamp[b]};
+        power_entries += nchans[b];
     }
     // largest windows first (longest work items start early)
     std::vector<int> order((size_t)ncand);
@@ -259,6 +263,7 @@ int aa_hip_refine_create(int device, int ncand, const int64_t* x_offsets, const 
     if (e == hipSuccess) e = hipMalloc(&p->d_cands, sizeof(aa_hip::refine_cand) * (size_t)ncand);
     if (e == hipSuccess) e = hipMalloc(&p->d_order, sizeof(int) * (size_t)ncand);
     if (e == hipSuccess) e = hipMalloc(&p->d_counter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->d_power, sizeof(double) * (size_t)power_entries);
     if (e == hipSuccess)
         e = hipMemcpy(p->d_cands, cands.data(), sizeof(aa_hip::refine_cand) * (size_t)ncand, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_order, order.data(), sizeof(int) * (size_t)ncand, hipMemcpyHostToDevice);
@@ -277,7 +282,16 @@ int aa_hip_refine_destroy(aa_hip_refine* h) {
     if (h->d_cands) (void)hipFree(h->d_cands);
     if (h->d_order) (void)hipFree(h->d_order);
     if (h->d_counter) (void)hipFree(h->d_counter);
+    if (h->d_power) (void)hipFree(h->d_power);
     delete h;
+    return AA_HIP_SUCCESS;
+}
+
+int aa_hip_refine_set_loss(aa_hip_refine* h, int loss_kind) {
+    if (!h) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "NULL handle");
+    if (loss_kind != AA_HIP_REFINE_LOSS_ENERGY && loss_kind != AA_HIP_REFINE_LOSS_EXCESS_ENERGY)
+        return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "unknown refinement loss");
+    h->loss_kind = loss_kind;
     return AA_HIP_SUCCESS;
 }
 
@@ -307,6 +321,8 @@ int aa_hip_refine_run(aa_hip_refine* h, const float* d_x, const float* d_delay_p
     a.scan_lo = scan_lo;
     a.scan_step = scan_step;
     a.counter = h->d_counter;
+    a.loss_kind = h->loss_kind;
+    a.power = h->d_power;
     return from_hip(aa_hip::launch_refine(a, h->grid, (hipStream_t)stream), "refine");
 }
 

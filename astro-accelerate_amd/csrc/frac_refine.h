@@ -10,8 +10,9 @@ constexpr int kRefineMaxSamples = 4096;    // per window (1024 threads x 4)
 constexpr int kRefineMaxChannels = 4096;   // per window (delay table in LDS)
 
 struct refine_cand {
-    int64_t x_off;    // window at x + x_off, [C][T] float32
-    int64_t ds_off;   // delay per unit DM at ds + ds_off, C entries
+    int64_t x_off;      // window at x + x_off, [C][T] float32
+    int64_t ds_off;     // delay per unit DM at ds + ds_off, C entries
+    int64_t power_off;  // channel powers at power + power_off, C entries (loss 1)
     int C, T;
 };
 
@@ -31,6 +32,8 @@ struct refine_args {
     int scan_n;                 // coarse scan points before the steps (0 = none)
     float scan_lo, scan_step;   // scan offsets relative to the starting u
     int* counter;               // queue head (reset by the launch)
+    int loss_kind;              // 0: E = sum_t y^2; 1: E minus the expected noise energy
+    double* power;              // loss 1: per-candidate channel powers (written by the launch)
 };
 
 // grid: resident work-groups (the kernel is persistent).

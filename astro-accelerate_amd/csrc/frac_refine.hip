@@ -3,25 +3,44 @@
 // x_b[c][t] (C_b channels x T_b samples) cut along the candidate's
 // dispersion track; its continuous residual DM u_b is refined by gradient
 // ascent on the energy of the dedispersed profile,
-//     L(u) = sum_t y(t)^2,  y(t) = sum_c (1 - f_c) x[c][t + i_c] + f_c x[c][t + i_c + 1],
+//     E(u) = sum_t y(t)^2,  y(t) = sum_c (1 - f_c) x[c][t + i_c] + f_c x[c][t + i_c + 1],
 //     tau_c = ds[c] * u (double), i_c = floor(tau_c), f_c = tau_c - i_c,
 // with x = 0 outside [0, T) and ds[c] the delay per unit DM in samples.
-//     dL/du = sum_c ds[c] sum_t 2 y(t) (x[c][t + i_c + 1] - x[c][t + i_c]).
+//     dE/du = sum_c ds[c] sum_t 2 y(t) (x[c][t + i_c + 1] - x[c][t + i_c]).
+//
+// Two losses (refine_args::loss_kind):
+//  0  L = E.
+//  1  L = E - N, the energy in excess of what white noise of the window's
+//     channel powers v_c = mean_t x[c][t]^2 would give at the same delays:
+//         N(u) = sum_c v_c ((1 - f_c)^2 (T - |i_c|)+ + f_c^2 (T - |i_c + 1|)+),
+//     the expected sum_t y(t)^2 of independent channels of variances v_c
+//     (each channel's two taps cover (T - |i|)+ and (T - |i + 1|)+ output
+//     samples).  Interpolation averages neighbouring noise samples, so E of
+//     pure noise is largest where every f_c is 0 (u = 0) and about a third
+//     lower in between: on weak candidates that spike outweighs the pulse.
+//     dN/du = sum_c v_c ds[c] (2 f_c (T - |i_c + 1|)+ - 2 (1 - f_c) (T - |i_c|)+).
+//     v_c is computed once per run (refine_power_kernel).
 //
 // An optional coarse scan first evaluates L (forward pass only) at scan_n
 // offsets u + scan_lo + k * scan_step and starts the steps from the best.
 //
 // One persistent launch: a work-group (1024 threads) takes candidates from a
 // queue (largest first) and runs every step of that candidate itself, so the
-// forward pass, the adjoint pass and the update are fused: pass 1 sums the
-// profile in double (each thread owns up to 4 samples, t = tid + 1024 j, and
-// keeps y in registers), pass 2 re-reads the window (recomputation: no shifted tile is
+// forward pass, the adjoint pass and the update are fused.  Thread layout: a
+// group of S lanes covers the window's samples, 4 consecutive samples per
+// lane (S = ceil(T / 4) rounded up to whole waves), and the work-group holds
+// G = 1024 / S such groups, each summing every G-th channel; so short windows
+// keep every lane busy.  A lane reads its 5-sample span x[c][4s + i_c ..
+// 4s + i_c + 4] as one 16-byte and one 4-byte load, 4 channels in flight.
+// Pass 1 sums the groups' partial profiles in double, adds them in LDS in
+// group order, pass 2 re-reads the windows (recomputation: no shifted tile is
 // stored) and accumulates the DM derivative, and thread 0 applies a
 // sign-based adaptive step (Rprop: the step grows by 1.2 while the sign of
 // dL/du holds and halves when it flips); the result is the best point
-// visited.  Windows are read straight from
-// global memory, coalesced along t: with one trial per candidate there is no
-// reuse to stage, and the passes are bound by HBM bandwidth.
+// visited.  Every sum runs in a fixed order: results do not depend on
+// scheduling.  Windows are read straight from global memory: with one trial
+// per candidate there is no reuse to stage, and the passes are bound by HBM
+// bandwidth.
 #include "frac_refine.h"
 
 #include <cmath>
@@ -31,10 +50,13 @@ namespace aa_hip {
 namespace {
 
 constexpr int kThreads = 1024;
-constexpr int kPerThread = kRefineMaxSamples / kThreads;   // 4
-constexpr int kUnroll = 4;                                  // channels in flight per thread
+constexpr int kSpt = 4;                       // consecutive samples per lane
+constexpr int kU = 4;                         // channels in flight per lane
+constexpr int kYSlots = kThreads * kSpt;      // the groups' partial profiles (G * T <= 4096)
+static_assert(kRefineMaxSamples <= kYSlots, "a window's samples fit one lane span each");
 
 __shared__ int2 r_tab[kRefineMaxChannels];   // (i_c, f_c bits) of the current u
+__shared__ double r_y[kYSlots];
 __shared__ double r_red[32];
 __shared__ float r_u;
 __shared__ long r_item;
@@ -50,110 +72,157 @@ __device__ __forceinline__ double block_sum(double v, int tid)
     return s;
 }
 
-__device__ __forceinline__ float load_or_zero(const float* __restrict__ row, int idx, int T)
+struct lane_layout {
+    int G;        // groups in the work-group
+    int g, s;     // this lane's group and position in it
+    bool active;  // g < G
+};
+
+__device__ __forceinline__ lane_layout layout_of(int T, int tid)
 {
-    return (idx >= 0 && idx < T) ? row[idx] : 0.0f;
+    const int S = ((T + kSpt - 1) / kSpt + 63) / 64 * 64;   // <= 1024 for T <= 4096
+    lane_layout L;
+    L.G = kThreads / S;
+    L.g = tid / S;
+    L.s = tid - L.g * S;
+    L.active = L.g < L.G;
+    return L;
+}
+
+// w[m] = row[j + m], m = 0..4, zero outside [0, T).
+__device__ __forceinline__ void load5(const float* __restrict__ row, int j, int T, float (&w)[5])
+{
+    if (j >= 0 && j + 4 < T) {
+        float4 q;
+        __builtin_memcpy(&q, row + j, sizeof(q));   // dword-aligned 16-byte load
+        w[0] = q.x;
+        w[1] = q.y;
+        w[2] = q.z;
+        w[3] = q.w;
+        w[4] = row[j + 4];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const int i = j + m;
+            w[m] = (i >= 0 && i < T) ? row[i] : 0.0f;
+        }
+    }
 }
 
 // (L, dL/du) of candidate `cd` at residual DM u; every thread gets both
-// (dL/du only when want_grad).
-__device__ void evaluate(const refine_args& a, const refine_cand& cd, float u, int tid, bool want_grad,
-                         double* loss, double* grad)
+// (dL/du only when want_grad).  power: the candidate's channel powers v_c
+// (loss 1) or null (loss 0).
+__device__ void evaluate(const refine_args& a, const refine_cand& cd, const double* __restrict__ power, float u,
+                         int tid, bool want_grad, double* loss, double* grad)
 {
     const float* __restrict__ x = a.x + cd.x_off;
     const float* __restrict__ ds = a.ds + cd.ds_off;
     const int C = cd.C, T = cd.T;
-    __syncthreads();  // r_tab free
+    double npart = 0.0, dnpart = 0.0;   // this thread's channels of N and dN/du
+    __syncthreads();  // r_tab and r_y free
     for (int c = tid; c < C; c += kThreads) {
-        const double tau = (double)ds[c] * (double)u;
-        const double fl = floor(tau);
-        r_tab[c] = make_int2((int)fl, __float_as_int((float)(tau - fl)));
+        const double d = (double)ds[c];
+        const double tau = d * (double)u;
+        const double fl = fmin(fmax(floor(tau), -1073741824.0), 1073741824.0);
+        const float f32 = (float)(tau - fl);
+        const int i = (int)fl;
+        r_tab[c] = make_int2(i, __float_as_int(f32));
+        if (power) {
+            const double f = (double)f32;       // the weight the passes use
+            const double n0 = (double)max(0, T - abs(i)), n1 = (double)max(0, T - abs(i + 1));
+            const double v = power[c];
+            npart = __builtin_fma(v, (1.0 - f) * (1.0 - f) * n0 + f * f * n1, npart);
+            dnpart = __builtin_fma(v * d, 2.0 * f * n1 - 2.0 * (1.0 - f) * n0, dnpart);
+        }
     }
     __syncthreads();
 
-    // The profile is accumulated in double: at an aligned pulse y reaches
-    // ~C/2 and float32 rounding of y would cost ~1e-5 of dL/du.
-    double y[kPerThread];
+    // pass 1: this group's channels of the profile at the lane's 4 samples
+    const lane_layout L = layout_of(T, tid);
+    const int t0 = kSpt * L.s;
+    double y[kSpt];
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) y[j] = 0.0;
-    int c = 0;
-    for (; c + kUnroll <= C; c += kUnroll) {
-        float x0[kUnroll][kPerThread], x1[kUnroll][kPerThread];
-        double w[kUnroll];
+    for (int m = 0; m < kSpt; ++m) y[m] = 0.0;
+    if (L.active) {
+        int c = L.g;
+        for (; c + (kU - 1) * L.G < C; c += kU * L.G) {
+            float w[kU][5];
+            double f[kU];
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
-            const int2 e = r_tab[c + k];
-            w[k] = (double)__int_as_float(e.y);
-            const float* row = x + (size_t)(c + k) * T;
-#pragma unroll
-            for (int j = 0; j < kPerThread; ++j) {
-                const int i = tid + kThreads * j + e.x;
-                x0[k][j] = load_or_zero(row, i, T);
-                x1[k][j] = load_or_zero(row, i + 1, T);
+            for (int k = 0; k < kU; ++k) {
+                const int cc = c + k * L.G;
+                const int2 e = r_tab[cc];
+                f[k] = (double)__int_as_float(e.y);
+                load5(x + (size_t)cc * T, t0 + e.x, T, w[k]);
             }
+#pragma unroll
+            for (int k = 0; k < kU; ++k)
+#pragma unroll
+                for (int m = 0; m < kSpt; ++m)
+                    y[m] = __builtin_fma(f[k], (double)w[k][m + 1], __builtin_fma(1.0 - f[k], (double)w[k][m], y[m]));
+        }
+        for (; c < C; c += L.G) {
+            float w[5];
+            const int2 e = r_tab[c];
+            const double f = (double)__int_as_float(e.y);
+            load5(x + (size_t)c * T, t0 + e.x, T, w);
+#pragma unroll
+            for (int m = 0; m < kSpt; ++m)
+                y[m] = __builtin_fma(f, (double)w[m + 1], __builtin_fma(1.0 - f, (double)w[m], y[m]));
         }
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k)
-#pragma unroll
-            for (int j = 0; j < kPerThread; ++j)
-                y[j] = __builtin_fma(w[k], (double)x1[k][j], __builtin_fma(1.0 - w[k], (double)x0[k][j], y[j]));
+        for (int m = 0; m < kSpt; ++m)
+            if (t0 + m < T) r_y[L.g * T + t0 + m] = y[m];
     }
-    for (; c < C; ++c) {
-        const int2 e = r_tab[c];
-        const double f = (double)__int_as_float(e.y);
-        const float* row = x + (size_t)c * T;
-#pragma unroll
-        for (int j = 0; j < kPerThread; ++j) {
-            const int i = tid + kThreads * j + e.x;
-            y[j] = __builtin_fma(f, (double)load_or_zero(row, i + 1, T),
-                                 __builtin_fma(1.0 - f, (double)load_or_zero(row, i, T), y[j]));
-        }
-    }
+    __syncthreads();
+    // the whole profile at the lane's samples: groups added in order
     double l2 = 0.0;
+    if (L.active) {
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        if (tid + kThreads * j >= T) y[j] = 0.0;    // outputs only for 0 <= t < T
-        l2 += y[j] * y[j];
+        for (int m = 0; m < kSpt; ++m) {
+            double s = 0.0;
+            if (t0 + m < T)
+                for (int g = 0; g < L.G; ++g) s += r_y[g * T + t0 + m];
+            y[m] = s;                                   // 0 outside 0 <= t < T
+            if (L.g == 0) l2 = __builtin_fma(s, s, l2);
+        }
     }
 
     if (!want_grad) {
-        *loss = block_sum(l2, tid);
+        *loss = block_sum(l2 - npart, tid);
         *grad = 0.0;
         return;
     }
-    // pass 2: dL/du, recomputing the window reads
-    double g = 0.0;
-    c = 0;
-    for (; c + kUnroll <= C; c += kUnroll) {
-        double d[kUnroll];
+    // pass 2: dE/du over this group's channels, recomputing the window reads
+    double gs = 0.0;
+    if (L.active) {
+        int c = L.g;
+        for (; c + (kU - 1) * L.G < C; c += kU * L.G) {
+            float w[kU][5];
 #pragma unroll
-        for (int k = 0; k < kUnroll; ++k) {
-            const int2 e = r_tab[c + k];
-            const float* row = x + (size_t)(c + k) * T;
+            for (int k = 0; k < kU; ++k) {
+                const int cc = c + k * L.G;
+                load5(x + (size_t)cc * T, t0 + r_tab[cc].x, T, w[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < kU; ++k) {
+                double s = 0.0;
+#pragma unroll
+                for (int m = 0; m < kSpt; ++m) s = __builtin_fma(y[m], (double)(w[k][m + 1] - w[k][m]), s);
+                gs = __builtin_fma((double)ds[c + k * L.G], s, gs);
+            }
+        }
+        for (; c < C; c += L.G) {
+            float w[5];
+            load5(x + (size_t)c * T, t0 + r_tab[c].x, T, w);
             double s = 0.0;
 #pragma unroll
-            for (int j = 0; j < kPerThread; ++j) {
-                const int i = tid + kThreads * j + e.x;
-                s = __builtin_fma(y[j], (double)(load_or_zero(row, i + 1, T) - load_or_zero(row, i, T)), s);
-            }
-            d[k] = s;
+            for (int m = 0; m < kSpt; ++m) s = __builtin_fma(y[m], (double)(w[m + 1] - w[m]), s);
+            gs = __builtin_fma((double)ds[c], s, gs);
         }
-#pragma unroll
-        for (int k = 0; k < kUnroll; ++k) g = __builtin_fma((double)ds[c + k], d[k], g);
     }
-    for (; c < C; ++c) {
-        const int2 e = r_tab[c];
-        const float* row = x + (size_t)c * T;
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < kPerThread; ++j) {
-            const int i = tid + kThreads * j + e.x;
-            s = __builtin_fma(y[j], (double)(load_or_zero(row, i + 1, T) - load_or_zero(row, i, T)), s);
-        }
-        g = __builtin_fma((double)ds[c], s, g);
-    }
-    *loss = block_sum(l2, tid);
-    *grad = 2.0 * block_sum(g, tid);
+    *loss = block_sum(l2 - npart, tid);
+    *grad = block_sum(2.0 * gs - dnpart, tid);
 }
 
 __global__ __launch_bounds__(kThreads) void refine_kernel(refine_args a)
@@ -167,18 +236,19 @@ __global__ __launch_bounds__(kThreads) void refine_kernel(refine_args a)
         if (item >= a.ncand) break;
         const int b = a.order[item];
         const refine_cand cd = a.cands[b];
+        const double* power = a.loss_kind == 1 ? a.power + cd.power_off : nullptr;
         if (tid == 0) r_u = a.u[b];
         __syncthreads();
         float step = a.step[b];
         double prev = a.prev_grad[b];
         double loss = 0.0, grad = 0.0;
-        if (a.scan_n > 0) {  // coarse energy scan around u; the best offset starts the steps
+        if (a.scan_n > 0) {  // coarse scan around u; the best offset starts the steps
             const float u0 = r_u;
             float best_u = u0;
-            double best_l = -1.0;
+            double best_l = -INFINITY;
             for (int k = 0; k < a.scan_n; ++k) {
                 const float uk = u0 + a.scan_lo + a.scan_step * (float)k;
-                evaluate(a, cd, uk, tid, false, &loss, &grad);
+                evaluate(a, cd, power, uk, tid, false, &loss, &grad);
                 if (loss > best_l) {
                     best_l = loss;
                     best_u = uk;
@@ -188,10 +258,10 @@ __global__ __launch_bounds__(kThreads) void refine_kernel(refine_args a)
             __syncthreads();
         }
         float best_u = r_u;
-        double best_l = -1.0, best_g = 0.0;
+        double best_l = -INFINITY, best_g = 0.0;
         for (int s = 0; s <= a.nsteps; ++s) {
             const float u = r_u;
-            evaluate(a, cd, u, tid, true, &loss, &grad);
+            evaluate(a, cd, power, u, tid, true, &loss, &grad);
             if (loss > best_l) {  // the result is the best point visited
                 best_l = loss;
                 best_u = u;
@@ -218,6 +288,26 @@ __global__ __launch_bounds__(kThreads) void refine_kernel(refine_args a)
     }
 }
 
+// v_c = mean_t x[c][t]^2 of every candidate's channels: one work-group per
+// candidate, one wave per channel at a time, a fixed reduction order.
+__global__ __launch_bounds__(256) void refine_power_kernel(refine_args a)
+{
+    const refine_cand cd = a.cands[blockIdx.x];
+    const float* __restrict__ x = a.x + cd.x_off;
+    double* __restrict__ v = a.power + cd.power_off;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = wave; c < cd.C; c += 4) {
+        const float* row = x + (size_t)c * cd.T;
+        double s = 0.0;
+        for (int t = lane; t < cd.T; t += 64) {
+            const double q = (double)row[t];
+            s = __builtin_fma(q, q, s);
+        }
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) v[c] = s / (double)cd.T;
+    }
+}
+
 __global__ void refine_reset_kernel(int* counter)
 {
     if (threadIdx.x == 0) *counter = 0;
@@ -235,7 +325,9 @@ int refine_blocks_per_cu()
 hipError_t launch_refine(const refine_args& a, int grid, hipStream_t s)
 {
     if (a.ncand <= 0) return hipSuccess;
-    if (grid <= 0) return hipErrorInvalidValue;
+    if (grid <= 0 || (a.loss_kind != 0 && a.loss_kind != 1) || (a.loss_kind == 1 && !a.power))
+        return hipErrorInvalidValue;
+    if (a.loss_kind == 1) hipLaunchKernelGGL(refine_power_kernel, dim3((unsigned)a.ncand), dim3(256), 0, s, a);
     hipLaunchKernelGGL(refine_reset_kernel, dim3(1), dim3(64), 0, s, a.counter);
     hipLaunchKernelGGL(refine_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, a);
     return hipGetLastError();

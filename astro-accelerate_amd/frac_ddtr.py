@@ -199,13 +199,23 @@ def synth_candidates(batch: int, nchans: int, nsamp: int, delay_per_dm, seed: in
 
 
 # ------------------------------------------------------------ refinement
+REFINE_LOSSES = {"energy": 0, "excess": 1}   # AA_HIP_REFINE_LOSS_ENERGY, _EXCESS_ENERGY
+
+
 class RefinePlan:
     """Batched DM refinement of candidate windows of uneven size
     (aa_hip_refine_*): windows packed in one float32 buffer, [C_b][T_b] at
-    x_offsets[b], delay tables at ds_offsets[b] of a second buffer."""
+    x_offsets[b], delay tables at ds_offsets[b] of a second buffer.
+
+    loss "excess" (default): profile energy minus the energy white noise of
+    the window's channel powers gives at the same delays; "energy": the
+    profile energy sum_t y(t)^2 alone (peaks at u = 0 on weak candidates,
+    where no channel is interpolated)."""
 
     def __init__(self, nchans: Sequence[int], nsamp: Sequence[int], x_offsets: Sequence[int],
-                 ds_offsets: Sequence[int], device: int = 0):
+                 ds_offsets: Sequence[int], device: int = 0, loss: str = "excess"):
+        if loss not in REFINE_LOSSES:
+            raise ValueError(f"loss must be one of {sorted(REFINE_LOSSES)}")
         lib = aa.load_library()
         n = len(nchans)
         self.ncand = n
@@ -216,6 +226,8 @@ class RefinePlan:
         h = ctypes.c_void_p()
         aa.check(lib.aa_hip_refine_create(int(device), n, xo, do, nc, ns, ctypes.byref(h)), "aa_hip_refine_create")
         self.handle = h
+        aa.check(lib.aa_hip_refine_set_loss(h, REFINE_LOSSES[loss]), "aa_hip_refine_set_loss")
+        self.loss = loss
         # extents the buffers must cover (checked before every launch)
         self.x_extent = max(int(o) + int(c) * int(t) for o, c, t in zip(x_offsets, nchans, nsamp))
         self.ds_extent = max(int(o) + int(c) for o, c in zip(ds_offsets, nchans))

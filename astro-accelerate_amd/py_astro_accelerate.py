@@ -187,6 +187,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_ddtr_batch_stats": ([vp, pi, ctypes.POINTER(ctypes.c_long), pi], i),
         "aa_hip_refine_create": ([i, i, ctypes.POINTER(i64), ctypes.POINTER(i64), pi, pi, ctypes.POINTER(vp)], i),
         "aa_hip_refine_destroy": ([vp], i),
+        "aa_hip_refine_set_loss": ([vp, i], i),
         "aa_hip_refine_run": ([vp, vp, vp, vp, vp, vp, vp, vp, i, f, f, i, f, f, vp], i),
         "aa_hip_frac_create": ([i, i, PFLOAT, i, PFLOAT, i, i, f, ctypes.POINTER(vp)], i),
         "aa_hip_frac_destroy": ([vp], i),

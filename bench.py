@@ -670,8 +670,8 @@ def bench_refine(args):
         f"p90 {np.percentile(err, 90):.4f}; dense grid (plain path) {t_grid:.2f} s, median {np.median(err_grid):.4f} "
         f"p90 {np.percentile(err_grid, 90):.4f}")
     # each of the nsteps + 1 evaluations reads every window twice (forward, recomputing
-    # adjoint), each scan point once
-    bytes_read = 4.0 * elems * (2 * (nsteps + 1) + SCAN[0])
+    # adjoint), each scan point once, and the channel powers of the loss once
+    bytes_read = 4.0 * elems * (2 * (nsteps + 1) + SCAN[0] + 1)
     gbs = bytes_read / (ms * 1e-3) / 1e9
     result = {
         "metric": "DM×sample/s bit-exact, % FP32 add roofline; batched vs per-beam speedup",
@@ -682,6 +682,7 @@ def bench_refine(args):
         "data": "synthetic: N(0,1) windows + Gaussian pulse (sigma 4, amplitude 0.5) at residual DM in +-1",
         "config": {"workload": "batched refinement 1024 windows 1024-4096 ch x 256-4096 samples, 20 steps (config 5)",
                    "windows": n, "window_elements": elems, "gradient_steps": nsteps, "coarse_scan": list(SCAN),
+                   "loss": "profile energy minus the white-noise energy of the window's channel powers",
                    "parallelism": "single GPU"},
         "dm_error_refined": {"median": float(np.median(err)), "p90": float(np.percentile(err, 90)),
                              "max": float(err.max())},

@@ -477,10 +477,18 @@ int aa_hip_frac_adjoint(aa_hip_frac *h, const float *d_x, const float *d_gy, int
  * counterpart).  Candidate b's window is float [nchans[b]][nsamp[b]] at
  * d_x + x_offsets[b] (elements), its delay-per-unit-DM table nchans[b]
  * floats at d_delay_per_dm + ds_offsets[b]; up to 4096 x 4096.  For its
- * residual DM u[b] the energy L = sum_t y(t)^2 of the dedispersed profile
- * (the aa_hip_frac_forward transform with one trial at offset 0) and dL/du
- * are evaluated by one fused forward + adjoint pass (the adjoint recomputes
- * the interpolation).  With scan_n > 0, L is first evaluated (forward only)
+ * residual DM u[b] the loss L and dL/du are evaluated by one fused forward +
+ * adjoint pass (the adjoint recomputes the interpolation).  The loss
+ * (aa_hip_refine_set_loss; default AA_HIP_REFINE_LOSS_ENERGY) is
+ *   ENERGY:        E = sum_t y(t)^2 of the dedispersed profile y (the
+ *                  aa_hip_frac_forward transform with one trial at offset 0);
+ *   EXCESS_ENERGY: E - N, N = sum_c v_c ((1-f_c)^2 (T-|i_c|)+ + f_c^2 (T-|i_c+1|)+)
+ *                  the energy white noise of the window's channel powers
+ *                  v_c = mean_t x[c][t]^2 would give at the same delays
+ *                  (i_c, f_c: integer and fractional delay of channel c):
+ *                  interpolation smooths noise least where every f_c is 0,
+ *                  so E alone peaks at u = 0 on weak candidates.
+ * With scan_n > 0, L is first evaluated (forward only)
  * at u + scan_lo + k * scan_step, k < scan_n, and the best point starts the
  * steps.  Then u[b] climbs L by nsteps sign-based adaptive steps:
  * step[b] grows x1.2 (up to step_max) while the sign of dL/du holds and
@@ -493,6 +501,9 @@ typedef struct aa_hip_refine aa_hip_refine;
 int aa_hip_refine_create(int device, int ncand, const int64_t *x_offsets, const int64_t *ds_offsets,
                          const int *nchans, const int *nsamp, aa_hip_refine **out);
 int aa_hip_refine_destroy(aa_hip_refine *h);
+#define AA_HIP_REFINE_LOSS_ENERGY 0
+#define AA_HIP_REFINE_LOSS_EXCESS_ENERGY 1
+int aa_hip_refine_set_loss(aa_hip_refine *h, int loss_kind);
 int aa_hip_refine_run(aa_hip_refine *h, const float *d_x, const float *d_delay_per_dm, float *d_u,
                       float *d_step, double *d_prev_grad, double *d_loss, double *d_grad, int scan_n,
                       float scan_lo, float scan_step, int nsteps, float step_max, float step_min,

@@ -78,3 +78,26 @@ def frac_ddtr_torch(x, dm, trials, delay_per_dm):
         x1 = torch.gather(xp, 2, idx + 1)
         out.append(((1.0 - f)[:, :, None] * x0 + f[:, :, None] * x1).sum(dim=1))
     return torch.stack(out, dim=1)
+
+
+def noise_energy_torch(x, u, delay_per_dm):
+    """Expected profile energy of white noise with the channel powers of x,
+    at residual DMs u [B] (one trial at offset 0):
+
+        N(u) = sum_c v_c ((1 - f_c)^2 (T - |i_c|)+ + f_c^2 (T - |i_c + 1|)+),
+        v_c = mean_t x[c][t]^2,  tau_c = delay_per_dm[c] * u,  i_c = floor(tau_c).
+
+    For independent zero-mean channels of variances v_c, E[sum_t y(t)^2]
+    equals N: the cross-channel terms vanish and channel c's two taps (weights
+    1 - f_c and f_c) cover (T - |i_c|)+ and (T - |i_c + 1|)+ output samples
+    of [0, T).  Differentiable in u (the floor carries no gradient); x [B, C, T]."""
+    import torch
+    B, C, T = x.shape
+    ds = torch.as_tensor(np.asarray(delay_per_dm), device=x.device).to(u.dtype)
+    v = (x.detach().to(u.dtype) ** 2).mean(dim=2)                     # [B, C]
+    tau = ds[None, :] * u[:, None]                                   # [B, C]
+    i0 = torch.floor(tau).detach()
+    f = tau - i0
+    n0 = torch.clamp(T - torch.abs(i0), min=0.0)
+    n1 = torch.clamp(T - torch.abs(i0 + 1.0), min=0.0)
+    return (v * ((1.0 - f) ** 2 * n0 + f ** 2 * n1)).sum(dim=1)
