@@ -63,21 +63,23 @@ __global__ __launch_bounds__(256) void corner_turn_kernel(const Tin* __restrict_
     }
 }
 
-// Wide corner turn fused with the x2 binning cascade.  A 256-thread
-// work-group turns a tile of 256 samples x 128 channels:
+// Wide corner turn fused with the x2 binning cascade.  A work-group of
+// kCtT threads turns a tile of kCtT samples x 128 channels:
 //  * load: lane (cq = lane & 7, tg = lane >> 3) reads samples 8 tg .. 8 tg + 7
 //    of channels 16 cq .. 16 cq + 15 with eight 16-byte loads (one 128-byte
 //    line of each [t][c] row per 8 lanes);
 //  * in registers: each 4 x 4 byte block (4 samples x 4 channels) is
 //    transposed with 8 v_perm, so a channel's 8 samples are two dwords;
 //  * LDS [c][t] (row stride kCtRow bytes): one 8-byte write per channel;
-//  * store: lane (tq = lane & 15, cr = lane >> 4) writes 16 samples of
-//    channels cr + 16 j (j < 8) to level 0 with 16-byte stores (16 lanes =
-//    256 contiguous bytes of a [c][t] row), and their floor averages to the
-//    next NLEV levels (8, 4, 2 bytes per lane), each level k holding
-//    L_k[t] = floor((L_{k-1}[2t] + L_{k-1}[2t+1]) / 2) for t < nsamp >> k.
+//  * store: lane (tq = lane % (kCtT / 16), cr = lane / (kCtT / 16)) writes 16
+//    samples of channels cr + 16 j (j < 8) to level 0 with 16-byte stores
+//    (kCtT contiguous bytes of a [c][t] row per row of lanes), and their
+//    floor averages to the next NLEV levels (8, 4, 2 bytes per lane), each
+//    level k holding L_k[t] = floor((L_{k-1}[2t] + L_{k-1}[2t+1]) / 2) for
+//    t < nsamp >> k.  512-sample tiles make every level's row segment twice
+//    as long as 256-sample ones (level 3: 64 bytes).
 // HBM traffic: every input byte read once, every level byte written once.
-constexpr int kCtT = 256, kCtC = 128, kCtRow = kCtT + 8;
+constexpr int kCtT = 512, kCtC = 128, kCtRow = kCtT + 8;
 
 struct ct_levels {
     uint8_t* ptr[4];
@@ -119,8 +121,8 @@ __device__ __forceinline__ uint4 ct_load16<uint16_t>(const uint16_t* p, bool& ba
 }
 
 template <typename Tin, int NLEV>
-__global__ __launch_bounds__(256) void corner_turn_wide_kernel(const Tin* __restrict__ in, long nsamp, int nchans,
-                                                               ct_levels lv, int* __restrict__ err)
+__global__ __launch_bounds__(kCtT) void corner_turn_wide_kernel(const Tin* __restrict__ in, long nsamp, int nchans,
+                                                                ct_levels lv, int* __restrict__ err)
 {
     __shared__ __align__(16) uint8_t tile[kCtC * kCtRow];
     const int tid = threadIdx.x;
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(256) void corner_turn_wide_kernel(const Tin* __rest
         }
     }
     __syncthreads();
-    const int tq = tid & 15, cr = tid >> 4;
+    const int tq = tid % (kCtT / 16), cr = tid / (kCtT / 16);
     const long t = t0 + 16 * tq;
 #pragma unroll 2
     for (int j = 0; j < 8; ++j) {
@@ -498,10 +500,10 @@ static void launch_ct_wide(const Tin* in, long nsamp, int nchans, const ct_level
 {
     const dim3 grid(ceil_div(nsamp, kCtT), ceil_div(nchans, kCtC));
     switch (nlev) {
-    case 0: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 0>), grid, dim3(256), 0, s, in, nsamp, nchans, lv, d_err); break;
-    case 1: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 1>), grid, dim3(256), 0, s, in, nsamp, nchans, lv, d_err); break;
-    case 2: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 2>), grid, dim3(256), 0, s, in, nsamp, nchans, lv, d_err); break;
-    default: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 3>), grid, dim3(256), 0, s, in, nsamp, nchans, lv, d_err); break;
+    case 0: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 0>), grid, dim3(kCtT), 0, s, in, nsamp, nchans, lv, d_err); break;
+    case 1: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 1>), grid, dim3(kCtT), 0, s, in, nsamp, nchans, lv, d_err); break;
+    case 2: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 2>), grid, dim3(kCtT), 0, s, in, nsamp, nchans, lv, d_err); break;
+    default: hipLaunchKernelGGL((corner_turn_wide_kernel<Tin, 3>), grid, dim3(kCtT), 0, s, in, nsamp, nchans, lv, d_err); break;
     }
 }
 
