@@ -288,23 +288,38 @@ __global__ __launch_bounds__(kThreads) void refine_kernel(refine_args a)
     }
 }
 
-// v_c = mean_t x[c][t]^2 of every candidate's channels: one work-group per
-// candidate, one wave per channel at a time, a fixed reduction order.
+// v_c = mean_t x[c][t]^2 of every candidate's channels: work-group (b, y)
+// of candidate b takes channels 4 y + wave + 64 k, one wave per channel at a
+// time with four 256-byte loads in flight per lane; fixed reduction order.
+constexpr int kPowerBlocksY = 16;
 __global__ __launch_bounds__(256) void refine_power_kernel(refine_args a)
 {
     const refine_cand cd = a.cands[blockIdx.x];
     const float* __restrict__ x = a.x + cd.x_off;
     double* __restrict__ v = a.power + cd.power_off;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int c = wave; c < cd.C; c += 4) {
-        const float* row = x + (size_t)c * cd.T;
-        double s = 0.0;
-        for (int t = lane; t < cd.T; t += 64) {
-            const double q = (double)row[t];
-            s = __builtin_fma(q, q, s);
+    const int T = cd.T;
+    for (int c = 4 * (int)blockIdx.y + wave; c < cd.C; c += 4 * kPowerBlocksY) {
+        const float* row = x + (size_t)c * T;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
+        int t = lane;
+        for (; t + 192 < T; t += 256) {
+            float q[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) q[m] = row[t + 64 * m];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) s[m] = __builtin_fma((double)q[m], (double)q[m], s[m]);
         }
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-        if (lane == 0) v[c] = s / (double)cd.T;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {   // fewer than 256 samples left
+            if (t + 64 * m < T) {
+                const double q = (double)row[t + 64 * m];
+                s[m] = __builtin_fma(q, q, s[m]);
+            }
+        }
+        double r = (s[0] + s[1]) + (s[2] + s[3]);
+        for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off);
+        if (lane == 0) v[c] = r / (double)T;
     }
 }
 
@@ -327,7 +342,8 @@ hipError_t launch_refine(const refine_args& a, int grid, hipStream_t s)
     if (a.ncand <= 0) return hipSuccess;
     if (grid <= 0 || (a.loss_kind != 0 && a.loss_kind != 1) || (a.loss_kind == 1 && !a.power))
         return hipErrorInvalidValue;
-    if (a.loss_kind == 1) hipLaunchKernelGGL(refine_power_kernel, dim3((unsigned)a.ncand), dim3(256), 0, s, a);
+    if (a.loss_kind == 1)
+        hipLaunchKernelGGL(refine_power_kernel, dim3((unsigned)a.ncand, kPowerBlocksY), dim3(256), 0, s, a);
     hipLaunchKernelGGL(refine_reset_kernel, dim3(1), dim3(64), 0, s, a.counter);
     hipLaunchKernelGGL(refine_kernel, dim3((unsigned)grid), dim3(kThreads), 0, s, a);
     return hipGetLastError();
