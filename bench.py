@@ -97,6 +97,9 @@ def main():
                     help="--e2e: with the pipeline's input and output DDTR normalisation options; alone: time "
                          "the input normalisation on the config's [t][c] input and the output normalisation on "
                          "its largest range's plane (HBM roofline lines, not the contract line)")
+    ap.add_argument("--spd", action="store_true",
+                    help="time the first single-pulse pass (boxcar MSD + SNR, widths 1-32) on a plane of range 0's "
+                         "size (not the contract line)")
     ap.add_argument("--zero-dm-outliers", action="store_true",
                     help="--e2e: with the pipeline's zero_dm_with_outliers option; alone: time zero-DM with "
                          "outlier rejection on the config's [t][c] input (not the contract line)")
@@ -117,6 +120,8 @@ def main():
         return bench_zero_dm(args, CONFIGS[args.config if args.config in CONFIGS else 2], outliers=True)
     if args.normalization:
         return bench_normalization(args, CONFIGS[args.config if args.config in CONFIGS else 2])
+    if args.spd:
+        return bench_spd(args, CONFIGS[2])
     if args.config == 3:
         return bench_frac(args)
     if args.config == 4:
@@ -383,6 +388,62 @@ def bench_zero_dm(args, cfg, outliers=False):
                       "ms_per_pass": ms, "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
                                                       "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS},
                       "config": {"workload": cfg["name"], "nchans": nchans, "nsamp": nsamp}}), flush=True)
+
+
+def bench_spd(args, cfg):
+    """The first single-pulse pass (aa_hip_spd_first_pass, boxcar widths
+    1..32) on a [1536][890624] float plane: the size of range 0's plane of
+    the 8-range plan, seeded N(0, 1) noise.  Timed with HIP events on its
+    stream; MSD only (one read of the plane) and MSD + SNR (the plane read
+    twice, SNR written as float and the width as u16).  Work per start sample
+    and width: one LDS read and about four VALU operations, so the pass is
+    VALU-bound; reported beside the algorithmic HBM bytes."""
+    import torch
+    import py_astro_accelerate as aa
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    ndms, nsamp, W = 1536, 890624, aa.SPD_MAX_WIDTH
+    g = torch.Generator(device=dev).manual_seed(SEED)
+    plane = torch.randn((ndms, nsamp), generator=g, device=dev, dtype=torch.float32)
+    stream = torch.cuda.Stream()
+    work = aa.spd_workspace(dev)
+    msd = torch.empty((W, 2), dtype=torch.float32, device=dev)
+    snr = torch.empty((ndms, nsamp), dtype=torch.float32, device=dev)
+    width = torch.empty((ndms, nsamp), dtype=torch.int16, device=dev)
+    torch.cuda.synchronize()
+
+    def timed(with_snr):
+        def once():
+            aa.spd_first_pass_raw(plane.data_ptr(), nsamp, ndms, nsamp, W, msd, snr if with_snr else None,
+                                  width if with_snr else None, work, stream.cuda_stream)
+        for _ in range(max(1, args.warmup)):
+            once()
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            once()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    ms_msd = timed(False)
+    ms_all = timed(True)
+    n = ndms * nsamp
+    bytes_msd, bytes_all = 4.0 * n, 4.0 * n * 2 + 4.0 * n + 2.0 * n
+    boxcars = float(n) * W                     # (start sample, width) pairs per pass
+    log(f"[spd] {ndms} x {nsamp} plane, widths 1..{W}: MSD {ms_msd:.3f} ms "
+        f"({bytes_msd / (ms_msd * 1e-3) / 1e9:.0f} GB/s), MSD + SNR {ms_all:.3f} ms "
+        f"({bytes_all / (ms_all * 1e-3) / 1e9:.0f} GB/s); {2 * boxcars / (ms_all * 1e-3) / 1e12:.2f} T boxcar "
+        f"terms/s over both passes")
+    print(json.dumps({"metric": "single-pulse first pass throughput", "value": n / (ms_all * 1e-3) / 1e9,
+                      "unit": "G start samples/s (32 widths, MSD + SNR)", "ms_msd": ms_msd, "ms_msd_snr": ms_all,
+                      "hbm_bytes_msd": bytes_msd, "hbm_bytes_msd_snr": bytes_all,
+                      "roofline": {"bound": "valu", "hbm_achieved_GBs": bytes_all / (ms_all * 1e-3) / 1e9,
+                                   "hbm_frac": bytes_all / (ms_all * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                   "boxcar_terms_per_s": 2 * boxcars / (ms_all * 1e-3)},
+                      "config": {"workload": "range 0 plane of the 8-range plan (1536 x 890624 f32), widths 1-32",
+                                 "ndms": ndms, "nsamp": nsamp, "max_width": W}}), flush=True)
 
 
 def bench_normalization(args, cfg):
