@@ -50,6 +50,10 @@ __shared__ double f_red[16];
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // v[0..19] = the five quads of a lane run starting e samples into the row.
+// Each quad stays one 16-byte read: the empty asm uses all four words, so
+// the compiler can neither narrow a read to the words one shift needs nor
+// move it into the shift's branch (words read at a 16-byte lane stride
+// would conflict on the LDS banks).
 __device__ __forceinline__ void read_window(const float4* __restrict__ row, int e, int lane, float (&v)[20])
 {
     const int e4 = e >> 2;
@@ -62,6 +66,9 @@ __device__ __forceinline__ void read_window(const float4* __restrict__ row, int 
         v[4 * m + 2] = t.z;
         v[4 * m + 3] = t.w;
     }
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+        asm volatile("" ::"v"(v[4 * m + 0]), "v"(v[4 * m + 1]), "v"(v[4 * m + 2]), "v"(v[4 * m + 3]));
 }
 
 // v[0..16] = src[start + m] (zero outside 0 <= index < n): the slow path.
@@ -140,7 +147,7 @@ __device__ __forceinline__ int ifloor(double v) { return (int)floor(v); }
 template <int KQ, bool GRAD>
 __global__ __launch_bounds__(kThreads) void frac_rows_kernel(frac_args a)
 {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = uni(tid >> 6);   // wave-uniform: DM or channel slots branch on it
     int blk = blockIdx.x;
     const int g = blk % a.n_groups;
     blk /= a.n_groups;
@@ -243,28 +250,35 @@ __global__ __launch_bounds__(kThreads) void frac_rows_kernel(frac_args a)
     };
     auto compute_chunk = [&](const float4* rows, const int2* tab, int c0) {
         const int nr = min(kCC, C - c0);
+        // this wave's (row, partner) entries, one per lane (lane r * KQ + q),
+        // read once per chunk and taken from their lanes in the loop
+        static_assert(kCC * KQ <= 64, "a wave's entries fit its lanes");
+        const int2 mine = lane < kCC * KQ ? tab[(lane / KQ) * kPartners + w + 16 * (lane % KQ)] : make_int2(0, 0);
         for (int r = 0; r < nr; ++r) {
             const int c = c0 + r;
             const float4* row = rows + r * kRQ;
 #pragma unroll
             for (int q = 0; q < KQ; ++q) {
                 if (!act[q]) continue;
-                const int2 t = tab[r * kPartners + w + 16 * q];
-                const int e = uni(t.x);
-                const float f = __int_as_float(uni(t.y));
+                const int e = __builtin_amdgcn_readlane(mine.x, r * KQ + q);
+                const float f = __int_as_float(__builtin_amdgcn_readlane(mine.y, r * KQ + q));
                 float v[20];
-                int s = e & 3;
                 if ((unsigned)e <= (unsigned)kFracMaxOffset) {  // also rejects e < 0 (non-finite dm)
                     read_window(row, e, lane, v);
+                    if constexpr (GRAD) {
+                        const float p = dot17_s(e & 3, D[q], v);
+                        accd = __builtin_fma((double)a.ds[c], (double)p, accd);
+                    } else {
+                        lerp_acc_s(e & 3, acc[q], v, 1.0f - f, f);
+                    }
                 } else {
                     read_window_global(xb + (size_t)c * T, T, kSpl * lane + e + row_start(c), v);
-                    s = 0;
-                }
-                if constexpr (GRAD) {
-                    const float p = dot17_s(s, D[q], v);
-                    accd = __builtin_fma((double)a.ds[c], (double)p, accd);
-                } else {
-                    lerp_acc_s(s, acc[q], v, 1.0f - f, f);
+                    if constexpr (GRAD) {
+                        const float p = dot17<0>(D[q], v);
+                        accd = __builtin_fma((double)a.ds[c], (double)p, accd);
+                    } else {
+                        lerp_acc<0>(acc[q], v, 1.0f - f, f);
+                    }
                 }
             }
         }
@@ -328,7 +342,7 @@ __global__ void frac_gdm_finalize(const double* __restrict__ part, int per_b, in
 template <int CQ>
 __global__ __launch_bounds__(kThreads) void frac_grad_x_kernel(frac_args a)
 {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = uni(tid >> 6);   // wave-uniform: DM or channel slots branch on it
     int blk = blockIdx.x;
     const int g = blk % a.n_groups;
     blk /= a.n_groups;
@@ -404,25 +418,27 @@ __global__ __launch_bounds__(kThreads) void frac_grad_x_kernel(frac_args a)
     };
     auto compute_chunk = [&](const float4* rows, const int2* tab, int k0) {
         const int nr = min(kCC, K - k0);
+        // this wave's (row, partner) entries, one per lane (lane r * CQ + q),
+        // read once per chunk and taken from their lanes in the loop
+        static_assert(kCC * CQ <= 64, "a wave's entries fit its lanes");
+        const int2 mine = lane < kCC * CQ ? tab[(lane / CQ) * kPartners + w + 16 * (lane % CQ)] : make_int2(0, 0);
         for (int r = 0; r < nr; ++r) {
             const int k = k0 + r;
             const float4* row = rows + r * kRQ;
 #pragma unroll
             for (int q = 0; q < CQ; ++q) {
                 if (!act[q]) continue;
-                const int2 t = tab[r * kPartners + w + 16 * q];
-                const int e = uni(t.x);
-                const float f = __int_as_float(uni(t.y));
+                const int e = __builtin_amdgcn_readlane(mine.x, r * CQ + q);
+                const float f = __int_as_float(__builtin_amdgcn_readlane(mine.y, r * CQ + q));
                 float v[20];
-                int s = e & 3;
+                // v[s + j] = gy[k][sl + j - i0 - 1] (weight f), v[s + j + 1] = gy[k][sl + j - i0] (weight 1 - f)
                 if ((unsigned)e <= (unsigned)kFracMaxOffset) {  // also rejects e < 0 (non-finite dm)
                     read_window(row, e, lane, v);
+                    lerp_acc_s(e & 3, acc[q], v, f, 1.0f - f);
                 } else {
                     read_window_global(gb + (size_t)k * T, T, kSpl * lane + e + row_start(k), v);
-                    s = 0;
+                    lerp_acc<0>(acc[q], v, f, 1.0f - f);
                 }
-                // v[s + j] = gy[k][sl + j - i0 - 1] (weight f), v[s + j + 1] = gy[k][sl + j - i0] (weight 1 - f)
-                lerp_acc_s(s, acc[q], v, f, 1.0f - f);
             }
         }
     };

@@ -285,6 +285,52 @@ def test_autograd_gradients_match_float64_finite_differences(cuda):
 
 
 @pytest.mark.gpu
+def test_gradients_at_full_window_shape_match_float64_finite_differences(cuda):
+    """The finite-difference check above at the full candidate-window shape:
+    4096 channels x 1024 samples x 64 DM offsets in +-2 (two windows).  With
+    262,144 delays per window, the DMs are chosen so that every delay lies
+    at least 5e-7 samples from an integer, and the central-difference step
+    moves no delay across one (the loss is exactly quadratic between cell
+    boundaries).  d loss / d x along a random direction.  Bar: 1e-4
+    relative."""
+    import torch
+    import frac_ddtr
+    B, C, T, K = 2, 4096, 1024, 64
+    ds = band_delays(C)
+    trials = np.linspace(-2.0, 2.0, K).astype(np.float32)
+    plan = frac_ddtr.FracPlan(ds, trials, T, max_batch=B)
+    x0, r, _ = frac_ddtr.synth_candidates(B, C, T, ds, seed=21, device=cuda)
+    dms, dist = kink_free_dms(ds, trials, B, margin=5e-7, seed=23)
+    dm0 = torch.tensor(dms, device=cuda)
+    x = x0.clone().requires_grad_(True)
+    dm = dm0.clone().requires_grad_(True)
+    loss = (frac_ddtr.frac_dedisperse(x, dm, plan) ** 2).sum()
+    loss.backward()
+    gdm = dm.grad.double().cpu().numpy()
+    x64 = x0.double()
+    dsmax = float(np.max(np.abs(ds)))
+    for b in range(B):
+        h = 0.5 * (dist[b] - 2e-7) / dsmax
+        base = float(dms[b])
+        lp = float((ref.frac_ddtr_torch(x64[b:b + 1], torch.tensor([base + h], dtype=torch.float64, device=cuda),
+                                        trials, ds) ** 2).sum())
+        lm = float((ref.frac_ddtr_torch(x64[b:b + 1], torch.tensor([base - h], dtype=torch.float64, device=cuda),
+                                        trials, ds) ** 2).sum())
+        fd = (lp - lm) / (2 * h)
+        print(f"window {b}: dm {base:+.6f} min delay distance {dist[b]:.2e} h {h:.2e}: "
+              f"d loss / d dm {gdm[b]:.9e} vs finite difference {fd:.9e}", flush=True)
+        assert abs(gdm[b] - fd) <= 1e-4 * abs(fd), (b, gdm[b], fd, h)
+    g = torch.Generator(device=cuda).manual_seed(5)
+    v = torch.randn(x0.shape, generator=g, device=cuda, dtype=torch.float64)
+    eps = 1e-3
+    lp = float((ref.frac_ddtr_torch(x64 + eps * v, dm0.double(), trials, ds) ** 2).sum())
+    lm = float((ref.frac_ddtr_torch(x64 - eps * v, dm0.double(), trials, ds) ** 2).sum())
+    fd_dir = (lp - lm) / (2 * eps)
+    got_dir = float((x.grad.double() * v).sum())
+    assert abs(got_dir - fd_dir) <= 1e-4 * abs(fd_dir), (got_dir, fd_dir)
+
+
+@pytest.mark.gpu
 def test_full_candidate_window_shape(cuda):
     """4096 channels x 1024 samples x 64 DM offsets (two candidates of the
     256-window batch): forward and both adjoints against the reference."""
