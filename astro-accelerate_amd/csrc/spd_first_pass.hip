@@ -58,6 +58,29 @@ __device__ __forceinline__ float wave_sum(float v)
     return v;
 }
 
+// v plus the value `ctrl` selects from another lane (DPP); lanes the row
+// mask leaves out, or whose source is out of range, add 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v)
+{
+    return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+// Sum over the wave's 64 lanes, in lane 63 only (other lanes hold partial
+// sums): quad sums (quad_perm 1,0,3,2 then 2,3,0,1), row sums (row_shr 4
+// then 8, lanes 12-15 of each row), then row_bcast15 and row_bcast31.  A
+// fixed order of additions, in registers (no LDS traffic).
+__device__ __forceinline__ float wave_sum_to_lane63(float v)
+{
+    v = dpp_add<0xB1, 0xf>(v);
+    v = dpp_add<0x4E, 0xf>(v);
+    v = dpp_add<0x114, 0xf>(v);
+    v = dpp_add<0x118, 0xf>(v);
+    v = dpp_add<0x142, 0xa>(v);
+    v = dpp_add<0x143, 0xc>(v);
+    return v;
+}
+
 // Chan et al.'s pairwise update: (n_a, m_a, M2_a) += (n_b, m_b, M2_b).
 __device__ __forceinline__ void chan_merge(double& na, double& ma, double& m2a, double nb, double mb, double m2b)
 {
@@ -71,9 +94,14 @@ __device__ __forceinline__ void chan_merge(double& na, double& ma, double& m2a, 
 
 // Per-workgroup moments over its tiles: partial[blk][w] = (n_w, mean of B_w,
 // sum of squared deviations of B_w).
+// WC: the width limit when it is known at compile time (32, the default),
+// else 0 and W_arg.  A runtime limit costs a scalar condition per width
+// (32 of them live across the loops, spilled to vector lanes).
+template <int WC>
 __global__ __launch_bounds__(kThreads) void spd_msd_kernel(const float* __restrict__ plane, long pitch, int ndms,
-                                                           int nsamp, int W, double* __restrict__ partial)
+                                                           int nsamp, int W_arg, double* __restrict__ partial)
 {
+    const int W = WC > 0 ? WC : W_arg;
     __shared__ float s[kTile + kMaxW];
     __shared__ float red[kThreads / 64][2 * kMaxW];
     __shared__ float wsum[kThreads / 64];
@@ -136,9 +164,9 @@ __global__ __launch_bounds__(kThreads) void spd_msd_kernel(const float* __restri
 #pragma unroll
         for (int w = 0; w < kMaxW; ++w) {
             if (w < W) {
-                const float r1 = wave_sum(a1[w]);
-                const float r2 = wave_sum(a2[w]);
-                if (lane == 0) {
+                const float r1 = wave_sum_to_lane63(a1[w]);
+                const float r2 = wave_sum_to_lane63(a2[w]);
+                if (lane == 63) {
                     red[wv][w] = r1;
                     red[wv][kMaxW + w] = r2;
                 }
@@ -195,11 +223,13 @@ __global__ void spd_msd_finalize_kernel(const double* __restrict__ partial, int 
     }
 }
 
+template <int WC>
 __global__ __launch_bounds__(kThreads) void spd_snr_kernel(const float* __restrict__ plane, long pitch, int nsamp,
-                                                           int W, const spd_aux* __restrict__ aux,
+                                                           int W_arg, const spd_aux* __restrict__ aux,
                                                            float* __restrict__ snr, long snr_pitch,
                                                            uint16_t* __restrict__ width, long width_pitch)
 {
+    const int W = WC > 0 ? WC : W_arg;
     __shared__ float s[kTile + kMaxW];
     __shared__ float off[kMaxW], rsd[kMaxW];
     const int tid = threadIdx.x;
@@ -219,6 +249,36 @@ __global__ __launch_bounds__(kThreads) void spd_snr_kernel(const float* __restri
     __syncthreads();
     float* so = snr ? snr + d * snr_pitch + t0 : nullptr;
     uint16_t* wo = width ? width + d * width_pitch + t0 : nullptr;
+    if (nload == kTile + W - 1) {   // every width fits at every start sample
+        // SNR_w = B'_w * rsd_w - off_w * rsd_w: one FMA per width
+        float p[kMaxW], r[kMaxW];
+#pragma unroll
+        for (int w = 0; w < kMaxW; ++w) {
+            r[w] = rsd[w];
+            p[w] = -off[w] * r[w];
+        }
+#pragma unroll 2
+        for (int k = 0; k < kK; ++k) {
+            const int j = tid + kThreads * k;
+            float B = s[j];
+            float best = __builtin_fmaf(B, r[0], p[0]);
+            int bw = 1;
+#pragma unroll
+            for (int w = 1; w < kMaxW; ++w) {
+                if (w < W) {
+                    B += s[j + w];
+                    const float v = __builtin_fmaf(B, r[w], p[w]);
+                    if (v > best) {
+                        best = v;
+                        bw = w + 1;
+                    }
+                }
+            }
+            if (so) so[j] = best;
+            if (wo) wo[j] = (uint16_t)bw;
+        }
+        return;
+    }
     for (int k = 0; k < kK; ++k) {
         const int j = tid + kThreads * k;
         if (j >= nout) break;
@@ -257,12 +317,21 @@ hipError_t launch_spd_first_pass(const float* plane, size_t pitch, int ndms, int
     const long per_row = ((long)nsamp + kTile - 1) / kTile;
     const long ntiles = (long)ndms * per_row;
     if (ntiles > 0x7fffffffL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(spd_msd_kernel, dim3(kBlocks), dim3(kThreads), 0, s, plane, (long)pitch, ndms, nsamp, W,
-                       partial);
+    if (W == kMaxW)
+        hipLaunchKernelGGL(spd_msd_kernel<kMaxW>, dim3(kBlocks), dim3(kThreads), 0, s, plane, (long)pitch, ndms, nsamp,
+                           W, partial);
+    else
+        hipLaunchKernelGGL(spd_msd_kernel<0>, dim3(kBlocks), dim3(kThreads), 0, s, plane, (long)pitch, ndms, nsamp, W,
+                           partial);
     hipLaunchKernelGGL(spd_msd_finalize_kernel, dim3(1), dim3(64), 0, s, (const double*)partial, kBlocks, W, msd, aux);
-    if (snr || width)
-        hipLaunchKernelGGL(spd_snr_kernel, dim3((unsigned)ntiles), dim3(kThreads), 0, s, plane, (long)pitch, nsamp,
-                           W, (const spd_aux*)aux, snr, (long)snr_pitch, width, (long)width_pitch);
+    if (snr || width) {
+        if (W == kMaxW)
+            hipLaunchKernelGGL(spd_snr_kernel<kMaxW>, dim3((unsigned)ntiles), dim3(kThreads), 0, s, plane, (long)pitch,
+                               nsamp, W, (const spd_aux*)aux, snr, (long)snr_pitch, width, (long)width_pitch);
+        else
+            hipLaunchKernelGGL(spd_snr_kernel<0>, dim3((unsigned)ntiles), dim3(kThreads), 0, s, plane, (long)pitch,
+                               nsamp, W, (const spd_aux*)aux, snr, (long)snr_pitch, width, (long)width_pitch);
+    }
     return hipGetLastError();
 }
 
