@@ -769,6 +769,13 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
                         R->ent[w][npairs + s] = (uint16_t)(split_row[s] * kPRowDw + split_e[s][w]);
                 R->nrows = nrows;
                 R->nent = npairs + nsplit;
+                // Each DM's entries in order of residue (offset & 3): the sum
+                // does not depend on their order, and the 16-byte-read kernel
+                // adds two entries with one v_add3 per dword where their
+                // residues agree.
+                for (int w = 0; w < kDms; ++w)
+                    std::stable_sort(R->ent[w], R->ent[w] + R->nent,
+                                     [](uint16_t x, uint16_t y) { return (x & 3) < (y & 3); });
             }
         }
     }

@@ -21,7 +21,10 @@ namespace {
 // (J = 4: 2016, shape 4; J = 3: 1512, shape 5; J = 2: 1008, shape 2): dword k = 252 j + 4 L + q of
 // lane L < 63 holds samples t0 + k (low half) and t0 + 252 J + k (high
 // half).  r is the same for the whole wave (one DM per wave): each entry
-// branches, on the scalar unit, to the add code of its r.
+// branches, on the scalar unit, to the add code of its r.  With J <= 3 the
+// full chunks add two entries at a time and keep the neighbour's dwords in
+// registers of their own until the flush (wide_acc2n) instead of shifting
+// them per entry.
 // ---------------------------------------------------------------------------
 
 // Lane L's x of lane L + 1 (lane 63 gets 0): DPP wave_shl:1.
@@ -133,7 +136,66 @@ __device__ __forceinline__ void wide_add1(const uint32_t* lane_q, uint32_t e, ui
     }
 }
 
-// acc += the entry whose quads are in D (r: its e & 3).
+// Two entries at once with every add on the lane's own dwords.  Dword
+// q + r - 4 of a lane's quad (q >= 4 - r) belongs to the left neighbour's
+// accumulator q; instead of shifting it over with a DPP add per entry, the
+// lane adds it into accN[3j + q - 1], which holds, for its left neighbour,
+// the not-yet-shifted part of accumulator q, and wide_take_next moves accN
+// over once per flush.  Per quad of two entries: one v_add3 per target both
+// entries add into, one add per target only one does: 4 + |RA - RB|
+// instructions for 8 dword adds (8 with a DPP add per shifted dword).  The
+// plan tables sort each DM's entries by residue, so most pairs agree.
+template <int RA, int RB, int J>
+__device__ __forceinline__ void wide_acc2n(const uint32_t (&Da)[4 * J], const uint32_t (&Db)[4 * J],
+                                           uint32_t (&acc)[4 * J], uint32_t (&accN)[3 * J])
+{
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int x = 4 * j + q, n = 3 * j + q - 1;
+            if (q + RA < 4 && q + RB < 4) {
+                acc[x] += Da[x + RA] + Db[x + RB];
+            } else if (q + RA < 4) {
+                acc[x] += Da[x + RA];
+                accN[n] += Db[x + RB - 4];
+            } else if (q + RB < 4) {
+                acc[x] += Db[x + RB];
+                accN[n] += Da[x + RA - 4];
+            } else {
+                accN[n] += Da[x + RA - 4] + Db[x + RB - 4];
+            }
+        }
+}
+
+// wide_acc2n for run-time residues (wave-uniform): ten bodies, the operands
+// swapped when ra > rb.
+template <int J>
+__device__ __forceinline__ void wide_acc2n_r(const uint32_t (&Da)[4 * J], const uint32_t (&Db)[4 * J], uint32_t ra,
+                                             uint32_t rb, uint32_t (&acc)[4 * J], uint32_t (&accN)[3 * J])
+{
+    switch ((ra << 2) | rb) {
+    case 0: wide_acc2n<0, 0, J>(Da, Db, acc, accN); break;
+    case 1: wide_acc2n<0, 1, J>(Da, Db, acc, accN); break;
+    case 2: wide_acc2n<0, 2, J>(Da, Db, acc, accN); break;
+    case 3: wide_acc2n<0, 3, J>(Da, Db, acc, accN); break;
+    case 4: wide_acc2n<0, 1, J>(Db, Da, acc, accN); break;
+    case 5: wide_acc2n<1, 1, J>(Da, Db, acc, accN); break;
+    case 6: wide_acc2n<1, 2, J>(Da, Db, acc, accN); break;
+    case 7: wide_acc2n<1, 3, J>(Da, Db, acc, accN); break;
+    case 8: wide_acc2n<0, 2, J>(Db, Da, acc, accN); break;
+    case 9: wide_acc2n<1, 2, J>(Db, Da, acc, accN); break;
+    case 10: wide_acc2n<2, 2, J>(Da, Db, acc, accN); break;
+    case 11: wide_acc2n<2, 3, J>(Da, Db, acc, accN); break;
+    case 12: wide_acc2n<0, 3, J>(Db, Da, acc, accN); break;
+    case 13: wide_acc2n<1, 3, J>(Db, Da, acc, accN); break;
+    case 14: wide_acc2n<2, 3, J>(Db, Da, acc, accN); break;
+    default: wide_acc2n<3, 3, J>(Da, Db, acc, accN); break;
+    }
+}
+
+// acc += the entry whose quads are in D (r: its e & 3), shifted dwords by
+// DPP adds.
 template <int J>
 __device__ __forceinline__ void wide_acc_r(const uint32_t (&D)[4 * J], uint32_t r, uint32_t (&acc)[4 * J])
 {
@@ -145,24 +207,44 @@ __device__ __forceinline__ void wide_acc_r(const uint32_t (&D)[4 * J], uint32_t 
     }
 }
 
-// wide_sum8 with the row build spread over the entries: after entry j's
-// adds, item kWItemAfter[j] (if any) of this wave's row of the next chunk is
-// stored (if `store`) and the same item of the row after that loaded into
-// the freed registers.  The build's VALU work and LDS writes thus fill the
-// waits on the entries' reads, and the global loads still have about a
-// chunk to arrive.  Items go after entries 1, 3, 5, 6, 7 (measured against
-// other placements: entries 0-4 4 % slower, items ahead of the adds 7 %).
-constexpr int kWItemAfter[kPairs] = {-1, 0, -1, 1, -1, 2, 3, 4};
+// Whether the kernel of J blocks per lane keeps accN: 3J more registers,
+// which at J = 4 exceed the 128 a lane has at 16 waves per CU (the kernel
+// then spills), so J = 4 keeps the DPP add per shifted dword.
+template <int J>
+constexpr bool wide_uses_accn = J <= 3;
+
+// acc[4j + q] += lane L+1's accN[3j + q - 1] (q = 1..3), then accN = 0.
+template <int J>
+__device__ __forceinline__ void wide_take_next(uint32_t (&acc)[4 * J], uint32_t (&accN)[3 * J])
+{
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+            wide_add_next(acc[4 * j + q], accN[3 * j + q - 1]);
+            accN[3 * j + q - 1] = 0;
+        }
+}
+
+// The 8 entries of a full chunk, two at a time, with the row build spread
+// over them: at slot s (two per entry pair: after the read of the pair's
+// second entry, then after the read of the next pair's first), item
+// kWItemAt[s] (if any) of this wave's row of the next chunk is stored (if
+// `store`) and the same item of the row after that loaded into the freed
+// registers.  The build's VALU work and LDS writes thus fill the waits on
+// the entries' reads, and the global loads still have about a chunk to
+// arrive.
+constexpr int kWItemAt[kPairs] = {-1, 0, -1, 1, -1, 2, 3, 4};
 template <int NI, int J>
 __device__ __forceinline__ void wide_sum8_build(const uint32_t* lane_q, const uint32_t (&e)[kPMaxEnt / 2],
-                                                uint32_t (&D0)[4 * J], uint32_t (&acc)[4 * J], pair_build<NI>& B,
-                                                bool store, uint4* r, const uint8_t* ra, const uint8_t* rb,
-                                                uint32_t mb_next, int lane)
+                                                uint32_t (&D0)[4 * J], uint32_t (&acc)[4 * J],
+                                                uint32_t (&accN)[3 * J], pair_build<NI>& B, bool store, uint4* r,
+                                                const uint8_t* ra, const uint8_t* rb, uint32_t mb_next, int lane)
 {
     const pair_sel S = pair_sel_of(B.mb);
     uint32_t D1[4 * J];
-    auto item = [&](int j) {
-        const int it = kWItemAfter[j];
+    auto item = [&](int s) {
+        const int it = kWItemAt[s];
         if (it >= 0 && it < NI) {
             if (store) pair_store_item<NI>(it, B, S, r);
             pair_load_item<NI, 2 * kWBlk * J>(it, ra, rb, lane, B);
@@ -171,10 +253,15 @@ __device__ __forceinline__ void wide_sum8_build(const uint32_t* lane_q, const ui
 #pragma unroll
     for (int i = 0; i < kPairs / 2; ++i) {
         wide_read<J>(lane_q, e[i] >> 16, D1);
-        wide_acc_r<J>(D0, e[i] & 3u, acc);
-        item(2 * i);
+        if constexpr (wide_uses_accn<J>) {
+            item(2 * i);
+            wide_acc2n_r<J>(D0, D1, e[i] & 3u, (e[i] >> 16) & 3u, acc, accN);
+        } else {
+            wide_acc_r<J>(D0, e[i] & 3u, acc);
+            item(2 * i);
+        }
         if (i + 1 < kPairs / 2) wide_read<J>(lane_q, e[i + 1] & 0xffffu, D0);
-        wide_acc_r<J>(D1, (e[i] >> 16) & 3u, acc);
+        if constexpr (!wide_uses_accn<J>) wide_acc_r<J>(D1, (e[i] >> 16) & 3u, acc);
         item(2 * i + 1);
     }
     B.mb = mb_next;
@@ -207,9 +294,12 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
     const pair_rec* recs = reinterpret_cast<const pair_rec*>(p.pair_recs) + (size_t)dt * nchunks;
     constexpr int A = 4 * J, SPLIT = kWBlk * J, T = 2 * SPLIT;
     uint32_t acc16[A];
+    uint32_t accN[3 * J];   // parts of the left neighbour's acc16 (wide_acc2n)
     uint32_t acc32[2 * A];
 #pragma unroll
     for (int j = 0; j < A; ++j) acc16[j] = 0;
+#pragma unroll
+    for (int j = 0; j < 3 * J; ++j) accN[j] = 0;
 #pragma unroll
     for (int j = 0; j < 2 * A; ++j) acc32[j] = 0;
 
@@ -234,7 +324,7 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
         const bool store = have_next;
         have_next = k + 2 < nchunks && wave < nrows_n;
         if (active && E.n == kPairs) {   // the usual full chunk: straight-line code
-            wide_sum8_build<NI, J>(base, E.e, D0, acc16, B, store, rows_next + wave * kPRQ + lane,
+            wide_sum8_build<NI, J>(base, E.e, D0, acc16, accN, B, store, rows_next + wave * kPRQ + lane,
                                 p.L + t0 + rn.offa, p.L + t0 + rn.offb + rn.sh, rn.mb, lane);
         } else {
             if (store) pair_build_store(wave, lane, B, rows_next);
@@ -242,6 +332,7 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
             if (active) wide_sum<J>(recs[k], wave, E.n, base, acc16);
         }
         if (active && ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks)) {
+            if constexpr (wide_uses_accn<J>) wide_take_next<J>(acc16, accN);
 #pragma unroll
             for (int j = 0; j < A; ++j) {
                 acc32[j] += acc16[j] & 0xFFFFu;

@@ -224,6 +224,41 @@ def test_execute_chunks_multi_range_multi_chunk(aa, oracle, cuda, kernel):
         inc += tps[0]
 
 
+@pytest.mark.parametrize("variant", [2, 4, 5])
+def test_pair_shape_forced_with_saturated_stretch(aa, oracle, cuda, variant):
+    """Each 16-byte-read pair shape (2: 1008-sample tiles, 4: 2016, 5: 1512) on every range of a
+    2-range plan over 512 channels (32 chunks: two windows of the 16-bit accumulators), on noise
+    with a stretch of all-255 samples, where every 16-bit field reaches its 65,280 bound at each
+    flush, and a pulse."""
+    import torch
+    say(f"pair shape {variant} forced on every range, saturated stretch")
+    nchans, nsamp, tsamp = 512, 40000, 64e-6
+    ranges = [(0, 100, 0.5, 1, 1), (100, 200, 1.0, 2, 2)]
+    s = oracle.strategy(ranges, nchans, nsamp, 1550.0, -0.5859, tsamp)
+    assert s.ok and s.num_tchunks == 1
+    data = oracle.gen_noise(5 + variant, nsamp, nchans)
+    data[9000:21000, :] = 255
+    oracle.inject_pulse(data, s.dmshifts, 60.0, tsamp, 4000, 6, 40)
+    want = oracle.run(s, data)
+    plan = aa.DdtrPlan(s.dmshifts, [(s.dm[r][0], s.dm[r][2], s.ndms[r], s.dm[r][3]) for r in range(s.nranges)],
+                       tsamp, s.maxshift, s.t_processed[0][0])
+    tps = [s.t_processed[r][0] for r in range(s.nranges)]
+    d_in = to_dev(data, cuda)
+    outs = [torch.empty((s.ndms[r], tps[r]), dtype=torch.float32, device=cuda) for r in range(s.nranges)]
+    plan.execute_chunk(d_in, tps, outs)   # ingest: the levels the ranges read
+    for r in range(s.nranges):
+        _, ms = plan.autotune_range(r, tps[r], outs[r], reps=1)
+        assert ms[variant] > 0, f"shape {variant} does not fit range {r}: {ms}"
+        plan.set_range_variant(r, variant)
+        outs[r].fill_(-1.0)
+    plan.execute_chunk(d_in, tps, outs)
+    for r in range(s.nranges):
+        got = outs[r].cpu().numpy()
+        np.testing.assert_array_equal(got, want[r], err_msg=f"shape {variant} range {r}")
+        ib = s.dm[r][3]   # inside the stretch every sum is 512 x 255: (sum / nchans) / inBin
+        assert np.all(got[:, 9000 // ib:(21000 - s.maxshift) // ib] == np.float32(255.0) / np.float32(ib))
+
+
 def _metadata(aa, nchans, nsamp, fch1, foff, tsamp):
     return aa.make_metadata(nchans, nsamp, fch1, foff, tsamp)
 
