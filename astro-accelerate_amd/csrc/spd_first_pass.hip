@@ -21,12 +21,14 @@
 // in double and merges them into its workgroup's running moments with
 // Chan's pairwise update (a persistent grid of kBlocks workgroups, tiles
 // assigned by stride); one finalize workgroup merges the workgroups' moments
-// in block order, so the result does not depend on scheduling.  The SNR
+// in a fixed order, so the result does not depend on scheduling.  The SNR
 // kernel centres on the plane mean c = mean_1 and divides
 // (B'_w - (mean_w - w c)) by sd_w.
 //
-// Cost per start sample and width: one LDS read and 3-4 VALU operations
-// (VALU-bound at W = 32; the plane is read twice from HBM).
+// Cost per start sample and width: 3 (MSD) or 5 (SNR) VALU operations, and
+// one LDS read in the SNR pass (the MSD pass at W = 32 reads a run of 8
+// start samples' 39 samples into registers once); VALU-bound, the plane is
+// read twice from HBM.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -97,12 +99,14 @@ __device__ __forceinline__ void chan_merge(double& na, double& ma, double& m2a, 
 // WC: the width limit when it is known at compile time (32, the default),
 // else 0 and W_arg.  A runtime limit costs a scalar condition per width
 // (32 of them live across the loops, spilled to vector lanes).
+// Four waves per SIMD (128 registers a lane): the persistent grid of
+// kBlocks work-groups is resident at once only at four work-groups per CU.
 template <int WC>
-__global__ __launch_bounds__(kThreads) void spd_msd_kernel(const float* __restrict__ plane, long pitch, int ndms,
-                                                           int nsamp, int W_arg, double* __restrict__ partial)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void spd_msd_kernel(
+    const float* __restrict__ plane, long pitch, int ndms, int nsamp, int W_arg, double* __restrict__ partial)
 {
     const int W = WC > 0 ? WC : W_arg;
-    __shared__ float s[kTile + kMaxW];
+    __shared__ __align__(16) float s[kTile + kMaxW];
     __shared__ float red[kThreads / 64][2 * kMaxW];
     __shared__ float wsum[kThreads / 64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -132,7 +136,37 @@ __global__ __launch_bounds__(kThreads) void spd_msd_kernel(const float* __restri
         float a1[kMaxW], a2[kMaxW];
 #pragma unroll
         for (int w = 0; w < kMaxW; ++w) a1[w] = a2[w] = 0.0f;
-        if (nload == kTile + W - 1) {   // every width fits at every start sample
+        if (WC == kMaxW && nload == kTile + W - 1) {
+            // All 32 widths fit at every start sample: runs of kRun
+            // consecutive start samples per thread, their kRun + 31 samples
+            // read once into registers with 16-byte reads (instead of one
+            // LDS read per start sample and width).
+            constexpr int kRun = 8, kWin = kRun + kMaxW;   // 40 registers: s[j0 .. j0 + 39]
+            static_assert(kTile % (kRun * kThreads) == 0, "whole runs per thread");
+#pragma unroll 1
+            for (int k = 0; k < kTile / (kRun * kThreads); ++k) {
+                const int j0 = kRun * (tid + kThreads * k);
+                float v[kWin];
+#pragma unroll
+                for (int m = 0; m < kWin / 4; ++m) {
+                    const float4 q = *reinterpret_cast<const float4*>(s + j0 + 4 * m);
+                    v[4 * m] = q.x;
+                    v[4 * m + 1] = q.y;
+                    v[4 * m + 2] = q.z;
+                    v[4 * m + 3] = q.w;
+                }
+#pragma unroll
+                for (int i = 0; i < kRun; ++i) {
+                    float B = 0.0f;
+#pragma unroll
+                    for (int w = 0; w < kMaxW; ++w) {
+                        B += v[i + w];
+                        a1[w] += B;
+                        a2[w] = __builtin_fmaf(B, B, a2[w]);
+                    }
+                }
+            }
+        } else if (nload == kTile + W - 1) {   // every width fits at every start sample
 #pragma unroll 2
             for (int k = 0; k < kK; ++k) {
                 const int j = tid + kThreads * k;
@@ -195,19 +229,38 @@ __global__ __launch_bounds__(kThreads) void spd_msd_kernel(const float* __restri
     }
 }
 
-// Merges the workgroups' moments in block order; writes (mean_w, sd_w) and
-// the SNR kernel's centring constants.
-__global__ void spd_msd_finalize_kernel(const double* __restrict__ partial, int nblocks, int W,
-                                        float* __restrict__ msd, spd_aux* __restrict__ aux)
+// Merges the workgroups' moments in a fixed order; writes (mean_w, sd_w)
+// and the SNR kernel's centring constants.  kFinLanes threads per width:
+// lane i merges the partials of blocks i, i + kFinLanes, ... in turn, then
+// the lanes' moments are merged pairwise in a fixed tree.
+constexpr int kFinLanes = 32;
+constexpr int kFinThreads = kMaxW * kFinLanes;
+__global__ __launch_bounds__(kFinThreads) void spd_msd_finalize_kernel(const double* __restrict__ partial,
+                                                                       int nblocks, int W, float* __restrict__ msd,
+                                                                       spd_aux* __restrict__ aux)
 {
+    __shared__ double tn[kMaxW][kFinLanes], tm[kMaxW][kFinLanes], t2[kMaxW][kFinLanes];
     __shared__ double mean[kMaxW], sd[kMaxW];
-    const int w = threadIdx.x;
-    if (w < W) {
-        double n = 0.0, m = 0.0, m2 = 0.0;
-        for (int b = 0; b < nblocks; ++b) {
+    const int w = threadIdx.x / kFinLanes, i = threadIdx.x % kFinLanes;
+    double n = 0.0, m = 0.0, m2 = 0.0;
+    if (w < W)
+        for (int b = i; b < nblocks; b += kFinLanes) {
             const double* p = partial + ((long)b * kMaxW + w) * 3;
             chan_merge(n, m, m2, p[0], p[1], p[2]);
         }
+    tn[w][i] = n;
+    tm[w][i] = m;
+    t2[w][i] = m2;
+    for (int h = kFinLanes / 2; h > 0; h >>= 1) {
+        __syncthreads();
+        if (i < h) {
+            chan_merge(n, m, m2, tn[w][i + h], tm[w][i + h], t2[w][i + h]);
+            tn[w][i] = n;
+            tm[w][i] = m;
+            t2[w][i] = m2;
+        }
+    }
+    if (w < W && i == 0) {
         const double v = n > 0.0 ? m2 / n : 0.0;
         mean[w] = m;
         sd[w] = v > 0.0 ? sqrt(v) : 0.0;
@@ -215,7 +268,7 @@ __global__ void spd_msd_finalize_kernel(const double* __restrict__ partial, int 
         msd[2 * w + 1] = (float)sd[w];
     }
     __syncthreads();
-    if (w < W) {
+    if (w < W && i == 0) {
         const double c = (double)(float)mean[0];
         if (w == 0) aux->c = (float)c;
         aux->off[w] = (float)(mean[w] - (double)(w + 1) * c);
@@ -224,13 +277,12 @@ __global__ void spd_msd_finalize_kernel(const double* __restrict__ partial, int 
 }
 
 template <int WC>
-__global__ __launch_bounds__(kThreads) void spd_snr_kernel(const float* __restrict__ plane, long pitch, int nsamp,
-                                                           int W_arg, const spd_aux* __restrict__ aux,
-                                                           float* __restrict__ snr, long snr_pitch,
-                                                           uint16_t* __restrict__ width, long width_pitch)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void spd_snr_kernel(
+    const float* __restrict__ plane, long pitch, int nsamp, int W_arg, const spd_aux* __restrict__ aux,
+    float* __restrict__ snr, long snr_pitch, uint16_t* __restrict__ width, long width_pitch)
 {
     const int W = WC > 0 ? WC : W_arg;
-    __shared__ float s[kTile + kMaxW];
+    __shared__ __align__(16) float s[kTile + kMaxW];
     __shared__ float off[kMaxW], rsd[kMaxW];
     const int tid = threadIdx.x;
     const long per_row = ((long)nsamp + kTile - 1) / kTile;
@@ -256,6 +308,43 @@ __global__ __launch_bounds__(kThreads) void spd_snr_kernel(const float* __restri
         for (int w = 0; w < kMaxW; ++w) {
             r[w] = rsd[w];
             p[w] = -off[w] * r[w];
+        }
+        if constexpr (WC == kMaxW) {
+            // runs of kRun consecutive start samples per thread, their
+            // kRun + 31 samples read into registers once
+            constexpr int kRun = 4, kWin = kRun + kMaxW;   // 36 registers: s[j0 .. j0 + 35]
+            static_assert(kTile % (kRun * kThreads) == 0, "whole runs per thread");
+#pragma unroll 1
+            for (int k = 0; k < kTile / (kRun * kThreads); ++k) {
+                const int j0 = kRun * (tid + kThreads * k);
+                float v[kWin];
+#pragma unroll
+                for (int m = 0; m < kWin / 4; ++m) {
+                    const float4 q = *reinterpret_cast<const float4*>(s + j0 + 4 * m);
+                    v[4 * m] = q.x;
+                    v[4 * m + 1] = q.y;
+                    v[4 * m + 2] = q.z;
+                    v[4 * m + 3] = q.w;
+                }
+#pragma unroll
+                for (int i = 0; i < kRun; ++i) {
+                    float B = v[i];
+                    float best = __builtin_fmaf(B, r[0], p[0]);
+                    int bw = 1;
+#pragma unroll
+                    for (int w = 1; w < kMaxW; ++w) {
+                        B += v[i + w];
+                        const float x = __builtin_fmaf(B, r[w], p[w]);
+                        if (x > best) {
+                            best = x;
+                            bw = w + 1;
+                        }
+                    }
+                    if (so) so[j0 + i] = best;
+                    if (wo) wo[j0 + i] = (uint16_t)bw;
+                }
+            }
+            return;
         }
 #pragma unroll 2
         for (int k = 0; k < kK; ++k) {
@@ -323,7 +412,7 @@ hipError_t launch_spd_first_pass(const float* plane, size_t pitch, int ndms, int
     else
         hipLaunchKernelGGL(spd_msd_kernel<0>, dim3(kBlocks), dim3(kThreads), 0, s, plane, (long)pitch, ndms, nsamp, W,
                            partial);
-    hipLaunchKernelGGL(spd_msd_finalize_kernel, dim3(1), dim3(64), 0, s, (const double*)partial, kBlocks, W, msd, aux);
+    hipLaunchKernelGGL(spd_msd_finalize_kernel, dim3(1), dim3(kFinThreads), 0, s, (const double*)partial, kBlocks, W, msd, aux);
     if (snr || width) {
         if (W == kMaxW)
             hipLaunchKernelGGL(spd_snr_kernel<kMaxW>, dim3((unsigned)ntiles), dim3(kThreads), 0, s, plane, (long)pitch,
