@@ -25,10 +25,11 @@
 // kernel centres on the plane mean c = mean_1 and divides
 // (B'_w - (mean_w - w c)) by sd_w.
 //
-// Cost per start sample and width: 3 (MSD) or 5 (SNR) VALU operations, and
-// one LDS read in the SNR pass (the MSD pass at W = 32 reads a run of 8
-// start samples' 39 samples into registers once); VALU-bound, the plane is
-// read twice from HBM.
+// Cost per start sample and width: 2 (MSD at W = 32, else 3) or 5 (SNR)
+// VALU operations.  At W = 32 both passes read a run of consecutive start
+// samples' samples into registers once (8 in the MSD pass, 4 in the SNR
+// pass), else one LDS read per start sample and width.  VALU-bound; the
+// plane is read twice from HBM.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -136,13 +137,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))
         float a1[kMaxW], a2[kMaxW];
 #pragma unroll
         for (int w = 0; w < kMaxW; ++w) a1[w] = a2[w] = 0.0f;
-        if (WC == kMaxW && nload == kTile + W - 1) {
-            // All 32 widths fit at every start sample: runs of kRun
-            // consecutive start samples per thread, their kRun + 31 samples
-            // read once into registers with 16-byte reads (instead of one
-            // LDS read per start sample and width).
+        // All 32 widths fit at every start sample (a wave-uniform case): the
+        // sums of B_w over the tile follow from the tile's sum C_0 and its
+        // edges (below), so the loop keeps only the running sum and the
+        // square sums; runs of kRun consecutive start samples per thread,
+        // their kRun + 31 samples read once into registers with 16-byte
+        // reads (instead of one LDS read per start sample and width).
+        const bool fast = WC == kMaxW && nload == kTile + W - 1;
+        if (fast) {
             constexpr int kRun = 8, kWin = kRun + kMaxW;   // 40 registers: s[j0 .. j0 + 39]
             static_assert(kTile % (kRun * kThreads) == 0, "whole runs per thread");
+            float cs = 0.0f;   // this thread's part of C_0 = sum of s[0 .. kTile)
 #pragma unroll 1
             for (int k = 0; k < kTile / (kRun * kThreads); ++k) {
                 const int j0 = kRun * (tid + kThreads * k);
@@ -157,15 +162,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))
                 }
 #pragma unroll
                 for (int i = 0; i < kRun; ++i) {
+                    cs += v[i];
                     float B = 0.0f;
 #pragma unroll
                     for (int w = 0; w < kMaxW; ++w) {
                         B += v[i + w];
-                        a1[w] += B;
                         a2[w] = __builtin_fmaf(B, B, a2[w]);
                     }
                 }
             }
+            a1[0] = cs;
         } else if (nload == kTile + W - 1) {   // every width fits at every start sample
 #pragma unroll 2
             for (int k = 0; k < kK; ++k) {
@@ -198,7 +204,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))
 #pragma unroll
         for (int w = 0; w < kMaxW; ++w) {
             if (w < W) {
-                const float r1 = wave_sum_to_lane63(a1[w]);
+                const float r1 = fast && w > 0 ? 0.0f : wave_sum_to_lane63(a1[w]);
                 const float r2 = wave_sum_to_lane63(a2[w]);
                 if (lane == 63) {
                     red[wv][w] = r1;
@@ -209,7 +215,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))
         __syncthreads();
         if (tid < W) {
             const int w = tid + 1;
-            const double S1 = (double)red[0][tid] + (double)red[1][tid] + (double)red[2][tid] + (double)red[3][tid];
+            double S1;
+            if (fast) {
+                // sum over the tile's starts j of B_w(j) = sum_{m<w} C_m, with
+                // C_m = sum_{j<kTile} s[j + m] = C_{m-1} + s[kTile - 1 + m] - s[m - 1]:
+                // S1 = w C_0 + sum_{i=1}^{w-1} (w - i) (s[kTile - 1 + i] - s[i - 1])
+                S1 = (double)w * ((double)red[0][0] + (double)red[1][0] + (double)red[2][0] + (double)red[3][0]);
+                for (int i = 1; i < w; ++i) S1 += (double)(w - i) * ((double)s[kTile - 1 + i] - (double)s[i - 1]);
+            } else {
+                S1 = (double)red[0][tid] + (double)red[1][tid] + (double)red[2][tid] + (double)red[3][tid];
+            }
             const double S2 = (double)red[0][kMaxW + tid] + (double)red[1][kMaxW + tid] +
                               (double)red[2][kMaxW + tid] + (double)red[3][kMaxW + tid];
             const long fit = (long)nload - w + 1;               // start samples whose width-w boxcar is staged
