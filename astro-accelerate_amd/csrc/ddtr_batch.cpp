@@ -81,6 +81,10 @@ ddtr_batch::~ddtr_batch() {
 
 int ddtr_batch::family_of(int variant, int row_quads) {
     if (variant == 0 || variant == 1) return 0;
+    if (variant == 6) {
+        const int ni = std::max(2, (row_quads + 63) / 64);   // as launch_dedisperse_wide2 picks
+        return row_quads <= 0 || ni > 5 ? -1 : 13 + (ni - 2);
+    }
     const int J = variant == 4 ? 4 : variant == 5 ? 3 : variant == 2 ? 2 : 0;
     const int ni = std::max(J, (row_quads + 63) / 64);   // the instantiation launch_dedisperse_wide picks
     if (J == 0 || row_quads <= 0 || ni > 5) return -1;
@@ -161,11 +165,15 @@ hipError_t ddtr_batch::launch_families(hipStream_t s) {
                                         m_fam[0].d_counters, m_grid, s);
     for (int f = 1; e == hipSuccess && f < kFamilies; ++f) {
         if (m_fam[f].jobs.empty()) continue;
-        const int J = 2 + (f - 1) / 4, ni = 2 + (f - 1) % 4;
         long longest = 0;
         for (int x = 0; x < 8; ++x) longest = std::max(longest, m_fam[f].bounds[(size_t)x + 1] - m_fam[f].bounds[(size_t)x]);
-        e = launch_dedisperse_wide_batch(m_fam[f].d_jobs, (int)m_fam[f].jobs.size(), m_fam[f].d_bounds, longest, J, ni,
-                                         s);
+        const int njobs = (int)m_fam[f].jobs.size();
+        if (f >= 13) {   // shape 6
+            e = launch_dedisperse_wide2_batch(m_fam[f].d_jobs, njobs, m_fam[f].d_bounds, longest, 2 + (f - 13), s);
+            continue;
+        }
+        const int J = 2 + (f - 1) / 4, ni = 2 + (f - 1) % 4;
+        e = launch_dedisperse_wide_batch(m_fam[f].d_jobs, njobs, m_fam[f].d_bounds, longest, J, ni, s);
     }
     for (size_t i = 0; e == hipSuccess && i < m_direct.size(); ++i) e = launch_range(m_direct[i], s);
     return e;

@@ -81,7 +81,9 @@ private:
         long *d_bounds = nullptr;  // 9 queue bounds
         int *d_counters = nullptr; // 8 queue heads (family 0)
     };
-    static constexpr int kFamilies = 1 + 3 * 4;
+    // family 0: byte rows (one persistent launch); 1..12: pair shapes by
+    // (J, NI); 13..16: shape 6 by NI
+    static constexpr int kFamilies = 1 + 4 * 4;
     static int family_of(int variant, int row_quads);   // -1: launched on its own
     family m_fam[kFamilies];
     std::vector<range_launch> m_direct;

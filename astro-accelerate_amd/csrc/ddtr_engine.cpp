@@ -73,7 +73,7 @@ int check_delays(const float *dmshifts, int nchans, int ndms, float mstartdm, fl
 
 lds_choice choose_lds(const float *h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, int variant) {
     lds_choice c{-1, -1};
-    static const int order[kLdsVariants] = {4, 5, 2, 3, 0, 1};
+    static const int order[kLdsVariants] = {4, 5, 2, 3, 0, 1, 6};
     for (int v : order) {
         if (variant >= 0 && v != variant) continue;
         const int q = lds_row_quads(v, h_dmshifts, nchans, ndms, mstartdm, mdmstep);

@@ -16,9 +16,10 @@ constexpr int kLdsRowQuadsMax = 152;
 // 2 = 16 channels pre-summed in pairs (rows of 16-bit pair sums, 1008-sample
 // tiles, window offsets up to 772 samples), 3 = 16 channels pre-summed in
 // groups of four, 4 = shape 2 on 2016-sample tiles (offsets up to 268),
-// 5 = shape 2 on 1512-sample tiles (offsets up to 520).  Shapes 2, 4 and 5
-// read rows with 16-byte LDS reads.
-constexpr int kLdsVariants = 6;
+// 5 = shape 2 on 1512-sample tiles (offsets up to 520), 6 = shape 2 with
+// 32 DM trials per tile (two per wave) sharing the chunk's rows.  Shapes 2,
+// 4, 5 and 6 read rows with 16-byte LDS reads.
+constexpr int kLdsVariants = 7;
 
 struct lds_params {
     const uint8_t* L;        // nchans rows of u8 samples
@@ -75,11 +76,11 @@ int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, 
                     size_t pitch, std::vector<uint8_t>* bytes);
 
 // Tile geometry: samples per tile of chunk shape `variant` and DM trials
-// per tile (16 in every shape).  Rows must hold the tile's samples readable
-// past the last processed sample plus the maximum delay (the engine pads by
-// 4 KiB).
+// per tile (16, or 32 in shape 6).  Rows must hold the tile's samples
+// readable past the last processed sample plus the maximum delay (the
+// engine pads by 4 KiB).
 long lds_tile_samples(int variant);
-int lds_tile_dms();
+int lds_tile_dms(int variant);
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s);
 
@@ -102,6 +103,10 @@ hipError_t launch_dedisperse_lds_batch(const lds_job* d_jobs, int njobs, const l
 // work-group takes one tile of its XCD's queue.
 hipError_t launch_dedisperse_wide_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, long max_queue,
                                         int J, int ni, hipStream_t s);
+
+// The same over shape-6 jobs (32 DM trials per tile) sharing NI = ni.
+hipError_t launch_dedisperse_wide2_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, long max_queue,
+                                         int ni, hipStream_t s);
 
 
 }  // namespace aa_hip

@@ -647,17 +647,19 @@ __global__ void batch_reset_kernel(int* counters)
 
 }  // namespace
 
+template <int NDM>
 static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
-                     int tile, std::vector<pair_rec>* recs);
+                     int tile, std::vector<pair_rec_t<NDM>>* recs);
 static int quad_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
                      std::vector<quad_rec>* recs);
 
 int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep)
 {
     if (variant < 0 || variant >= kLdsVariants) return -1;
-    if (variant == 2) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT2, nullptr);
-    if (variant == 4) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT4, nullptr);
-    if (variant == 5) return pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT3, nullptr);
+    if (variant == 2) return pair_plan<kDms>(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT2, nullptr);
+    if (variant == 4) return pair_plan<kDms>(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT4, nullptr);
+    if (variant == 5) return pair_plan<kDms>(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT3, nullptr);
+    if (variant == 6) return pair_plan<kDms2>(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, kWT2, nullptr);
     if (variant == 3) return quad_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, 0, nullptr);
     // e_max over every tile and channel: delay(c, last DM of the tile) -
     // floor16(delay(c, first DM)); a lane run reads 48 bytes starting at
@@ -687,8 +689,9 @@ int lds_row_quads(int variant, const float* h_dmshifts, int nchans, int ndms, fl
 // of every (DM tile, chunk), DM-tile-major, for rows `pitch` bytes apart.
 // Delays follow the kernels' rule (truncation of
 // dmshift * fmaf(d, mdmstep, mstartdm) in float).
+template <int NDM>
 static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstartdm, float mdmstep, size_t pitch,
-                     int tile, std::vector<pair_rec>* recs)
+                     int tile, std::vector<pair_rec_t<NDM>>* recs)
 {
     const int max_off = pair_max_offset(tile);
     auto make_row = [pitch](int a, int base, int b, int vb) {
@@ -704,32 +707,32 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
     };
     if (nchans <= 0 || ndms <= 0) return -1;
     const int nchunks = (nchans + kPairChunk - 1) / kPairChunk;
-    const int ndt = (ndms + kDms - 1) / kDms;
-    if (recs) recs->assign((size_t)ndt * nchunks, pair_rec{});
-    std::vector<int> dl((size_t)kDms * nchans), d0l((size_t)nchans);
+    const int ndt = (ndms + NDM - 1) / NDM;
+    if (recs) recs->assign((size_t)ndt * nchunks, pair_rec_t<NDM>{});
+    std::vector<int> dl((size_t)NDM * nchans), d0l((size_t)nchans);
     int e_max = 0;
     for (int dt = 0; dt < ndt; ++dt) {
-        const int d0 = dt * kDms;
+        const int d0 = dt * NDM;
         const float s0 = __builtin_fmaf((float)d0, mdmstep, mstartdm);
         for (int c = 0; c < nchans; ++c) d0l[(size_t)c] = (int)(h_dmshifts[c] * s0);
-        for (int w = 0; w < kDms; ++w) {
+        for (int w = 0; w < NDM; ++w) {
             const int d = std::min(d0 + w, ndms - 1);
             const float sd = __builtin_fmaf((float)d, mdmstep, mstartdm);
             for (int c = 0; c < nchans; ++c) dl[(size_t)w * nchans + c] = (int)(h_dmshifts[c] * sd);
         }
         auto D = [&](int w, int c) { return dl[(size_t)w * nchans + c]; };
         for (int k = 0; k < nchunks; ++k) {
-            pair_rec* R = recs ? &(*recs)[(size_t)dt * nchunks + k] : nullptr;
+            pair_rec_t<NDM>* R = recs ? &(*recs)[(size_t)dt * nchunks + k] : nullptr;
             int nrows = 0, npairs = 0, nsplit = 0;
-            int split_row[kPairs], split_e[kPairs][kDms];
+            int split_row[kPairs], split_e[kPairs][NDM];
             for (int pp = 0; pp < kPairs; ++pp) {
                 const int a = k * kPairChunk + 2 * pp;
                 if (a >= nchans) break;
                 ++npairs;
                 const bool hasb = a + 1 < nchans;
                 const int basea = d0l[(size_t)a] & ~3;
-                int ea[kDms], v[kDms], vmin = 0, vmax = 0;
-                for (int w = 0; w < kDms; ++w) {
+                int ea[NDM], v[NDM], vmin = 0, vmax = 0;
+                for (int w = 0; w < NDM; ++w) {
                     ea[w] = D(w, a) - basea;
                     if (ea[w] < 0 || ea[w] > max_off) return -1;
                     e_max = std::max(e_max, ea[w]);
@@ -743,13 +746,13 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
                     if (R) {
                         for (int i = 0; i < nv; ++i)
                             R->row[nrows + i] = make_row(a, basea, hasb ? a + 1 : -1, vmin + i);
-                        for (int w = 0; w < kDms; ++w)
+                        for (int w = 0; w < NDM; ++w)
                             R->ent[w][pp] = (uint16_t)((nrows + v[w] - vmin) * kPRowDw + ea[w]);
                     }
                     nrows += nv;
                 } else {   // split: two single-channel rows
                     const int baseb = d0l[(size_t)a + 1] & ~3;
-                    for (int w = 0; w < kDms; ++w) {
+                    for (int w = 0; w < NDM; ++w) {
                         split_e[nsplit][w] = D(w, a + 1) - baseb;
                         if (split_e[nsplit][w] < 0 || split_e[nsplit][w] > max_off) return -1;
                         e_max = std::max(e_max, split_e[nsplit][w]);
@@ -757,7 +760,7 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
                     if (R) {
                         R->row[nrows] = make_row(a, basea, -1, 0);
                         R->row[nrows + 1] = make_row(a + 1, baseb, -1, 0);
-                        for (int w = 0; w < kDms; ++w) R->ent[w][pp] = (uint16_t)(nrows * kPRowDw + ea[w]);
+                        for (int w = 0; w < NDM; ++w) R->ent[w][pp] = (uint16_t)(nrows * kPRowDw + ea[w]);
                     }
                     split_row[nsplit++] = nrows + 1;
                     nrows += 2;
@@ -765,7 +768,7 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
             }
             if (R) {
                 for (int s = 0; s < nsplit; ++s)
-                    for (int w = 0; w < kDms; ++w)
+                    for (int w = 0; w < NDM; ++w)
                         R->ent[w][npairs + s] = (uint16_t)(split_row[s] * kPRowDw + split_e[s][w]);
                 R->nrows = nrows;
                 R->nent = npairs + nsplit;
@@ -773,7 +776,7 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
                 // does not depend on their order, and the 16-byte-read kernel
                 // adds two entries with one v_add3 per dword where their
                 // residues agree.
-                for (int w = 0; w < kDms; ++w)
+                for (int w = 0; w < NDM; ++w)
                     std::stable_sort(R->ent[w], R->ent[w] + R->nent,
                                      [](uint16_t x, uint16_t y) { return (x & 3) < (y & 3); });
             }
@@ -946,11 +949,20 @@ int lds_plan_tables(int variant, const float* h_dmshifts, int nchans, int ndms, 
     }
     if (variant == 2 || variant == 4 || variant == 5) {
         std::vector<pair_rec> recs;
-        const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch,
-                                variant == 4 ? kWT4 : variant == 5 ? kWT3 : kWT2,
-                                bytes ? &recs : nullptr);
+        const int q = pair_plan<kDms>(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch,
+                                      variant == 4 ? kWT4 : variant == 5 ? kWT3 : kWT2, bytes ? &recs : nullptr);
         if (q > 0 && bytes) {
             bytes->resize(recs.size() * sizeof(pair_rec));
+            std::memcpy(bytes->data(), recs.data(), bytes->size());
+        }
+        return q;
+    }
+    if (variant == 6) {
+        std::vector<pair_rec2> recs;
+        const int q = pair_plan<kDms2>(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, kWT2,
+                                       bytes ? &recs : nullptr);
+        if (q > 0 && bytes) {
+            bytes->resize(recs.size() * sizeof(pair_rec2));
             std::memcpy(bytes->data(), recs.data(), bytes->size());
         }
         return q;
@@ -962,7 +974,7 @@ int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartd
                     std::vector<uint8_t>* bytes)
 {
     std::vector<pair_rec> recs;
-    const int q = pair_plan(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, kWT2, bytes ? &recs : nullptr);
+    const int q = pair_plan<kDms>(h_dmshifts, nchans, ndms, mstartdm, mdmstep, pitch, kWT2, bytes ? &recs : nullptr);
     if (q > 0 && bytes) {
         bytes->resize(recs.size() * sizeof(pair_rec));
         std::memcpy(bytes->data(), recs.data(), bytes->size());
@@ -972,9 +984,9 @@ int lds_pair_tables(const float* h_dmshifts, int nchans, int ndms, float mstartd
 
 long lds_tile_samples(int variant)
 {
-    return variant == 2 ? kWT2 : variant == 3 ? kQT : variant == 4 ? kWT4 : variant == 5 ? kWT3 : kT;
+    return variant == 2 || variant == 6 ? kWT2 : variant == 3 ? kQT : variant == 4 ? kWT4 : variant == 5 ? kWT3 : kT;
 }
-int lds_tile_dms() { return kDms; }
+int lds_tile_dms(int variant) { return variant == 6 ? kDms2 : kDms; }
 
 hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s)
 {
@@ -987,7 +999,7 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
     if (p.ndms <= 0 || p.t_processed <= 0) return hipSuccess;
     lds_params q = p;
     const long tile = lds_tile_samples(variant);
-    q.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
+    q.n_dm_tiles = (p.ndms + lds_tile_dms(variant) - 1) / lds_tile_dms(variant);
     q.n_t_tiles = (int)((p.t_processed + tile - 1) / tile);
     const long ntiles = (long)q.n_dm_tiles * q.n_t_tiles;
     const long grid = 8 * ((ntiles + 7) / 8);
@@ -1005,6 +1017,7 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
         // quads per lane: the row width rounded up to whole 64-quad items
         // (a row is at least half a tile wide)
         const int ni = (p.row_quads + 63) / 64;
+        if (variant == 6) return launch_dedisperse_wide2(q, ni, (unsigned)grid, s);
         return launch_dedisperse_wide(q, variant == 4 ? 4 : variant == 5 ? 3 : 2, ni, (unsigned)grid, s);
     }
     return hipGetLastError();
@@ -1012,8 +1025,8 @@ hipError_t launch_dedisperse_lds(const lds_params& p, int variant, hipStream_t s
 
 void lds_job_tiles(lds_params& p, int variant)
 {
-    const long tile = lds_tile_samples(variant);   // 16 DMs per tile in every shape
-    p.n_dm_tiles = (p.ndms + kDms - 1) / kDms;
+    const long tile = lds_tile_samples(variant);
+    p.n_dm_tiles = (p.ndms + lds_tile_dms(variant) - 1) / lds_tile_dms(variant);
     p.n_t_tiles = (int)((p.t_processed + tile - 1) / tile);
 }
 

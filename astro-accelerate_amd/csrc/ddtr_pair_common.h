@@ -80,12 +80,18 @@ struct pair_row {
     uint32_t sh, mb;        // byte shift of b's window in its dwords; ~0 pair row, 0 single row (b masked off)
     int pad0, pad1;
 };
-struct pair_rec {
-    uint16_t ent[kDms][kPMaxEnt];  // per DM slot: dword offset (row * kPRowDw + e) of each entry
+template <int NDM>
+struct pair_rec_t {
+    uint16_t ent[NDM][kPMaxEnt];   // per DM slot: dword offset (row * kPRowDw + e) of each entry
     pair_row row[kPRows];
     int nrows, nent, pad0, pad1;
 };
-static_assert(sizeof(pair_row) == 32 && sizeof(pair_rec) % 16 == 0, "records are 16-byte aligned");
+using pair_rec = pair_rec_t<kDms>;
+// Shape 6 tiles hold 32 DM trials (two per wave) over the same 16 rows.
+constexpr int kDms2 = 2 * kDms;
+using pair_rec2 = pair_rec_t<kDms2>;
+static_assert(sizeof(pair_row) == 32 && sizeof(pair_rec) % 16 == 0 && sizeof(pair_rec2) % 16 == 0,
+              "records are 16-byte aligned");
 static_assert(kPRows * kPRowDw <= 65536, "entry offsets fit 16 bits");
 
 // Two row buffers of 80 KiB (the whole LDS) in one array: buffer b starts
@@ -206,10 +212,11 @@ struct pair_ents {
     int n;
 };
 
-__device__ __forceinline__ pair_ents pair_ents_of(const pair_rec& rc, int wave)
+template <int NDM>
+__device__ __forceinline__ pair_ents pair_ents_of(const pair_rec_t<NDM>& rc, int slot)
 {
     pair_ents E;
-    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[slot][0]);
 #pragma unroll
     for (int i = 0; i < kPMaxEnt / 2; ++i) E.e[i] = ew[i];
     E.n = rc.nent;
@@ -252,5 +259,10 @@ __device__ __forceinline__ const lds_job& batch_job_of(const lds_job* __restrict
 // 3: shape 5, 2: shape 2) and NI = ni quads per lane of a row, on `grid`
 // work-groups.
 hipError_t launch_dedisperse_wide(const lds_params& q, int J, int ni, unsigned grid, hipStream_t s);
+
+// Launches shape 6: 1008-sample tiles of 32 DM trials, two per wave (DM
+// slots w and w + 16 of wave w), over the 16 rows a chunk builds; NI = ni
+// quads per lane of a row.
+hipError_t launch_dedisperse_wide2(const lds_params& q, int ni, unsigned grid, hipStream_t s);
 
 }  // namespace aa_hip

@@ -269,11 +269,11 @@ __device__ __forceinline__ void wide_sum8_build(const uint32_t* lane_q, const ui
 
 // All n entries of DM slot `wave` in chunk record rc, read from the record
 // itself (a register array indexed at run time would live in scratch memory).
-template <int J>
-__device__ __forceinline__ void wide_sum(const pair_rec& rc, int wave, int n, const uint32_t* lane_q,
+template <int J, int NDM>
+__device__ __forceinline__ void wide_sum(const pair_rec_t<NDM>& rc, int slot, int n, const uint32_t* lane_q,
                                          uint32_t (&acc)[4 * J])
 {
-    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[wave][0]);
+    const uint32_t* ew = reinterpret_cast<const uint32_t*>(&rc.ent[slot][0]);
     int i = 0;
     for (; i + 1 < n; i += 2) wide_add2<J>(lane_q, ew[i >> 1], acc);
     if (i < n) wide_add1<J>(lane_q, ew[i >> 1] & 0xffffu, acc);
@@ -399,6 +399,164 @@ __global__ __launch_bounds__(kThreads) void dedisperse_wide_batch_kernel(const l
                              (local / jb.p.n_dm_tiles) * (2L * kWBlk * J));
 }
 
+
+// ---------------------------------------------------------------------------
+// Shape 6: 1008-sample tiles (J = 2) of 32 DM trials, two per wave (DM
+// slots w and w + 16 of wave w).  A chunk still builds at most 16 rows (a
+// pair's relative delay takes at most two values over the 32 trials, else
+// the pair is split into two single-channel rows), one per wave, so the
+// build and the row stores are shared by twice the trials of shape 2, and
+// a chunk holds as many adds per wave as shape 4's.
+// ---------------------------------------------------------------------------
+
+// Entry i (< 8) of a DM's packed entries.
+__device__ __forceinline__ uint32_t wide2_entry(const uint32_t (&e)[kPMaxEnt / 2], int i)
+{
+    return (i & 1) ? (e[i >> 1] >> 16) : (e[i >> 1] & 0xffffu);
+}
+
+// The 8 entries of each of the two DMs of a full chunk, DM a's entry i
+// summed while DM b's is read and the other way round, with the row build
+// spread over them: item kW2ItemAt[s] after slot s (even: DM a's entry s/2,
+// odd: DM b's).
+constexpr int kW2ItemAt[2 * kPairs] = {-1, -1, -1, 0, -1, -1, -1, 1, -1, -1, -1, 2, -1, 3, -1, 4};
+template <int NI>
+__device__ __forceinline__ void wide2_sum8_build(const uint32_t* lane_q, const uint32_t (&ea)[kPMaxEnt / 2],
+                                                 const uint32_t (&eb)[kPMaxEnt / 2], uint32_t (&Da)[8],
+                                                 uint32_t (&acca)[8], uint32_t (&accb)[8], pair_build<NI>& B,
+                                                 bool store, uint4* r, const uint8_t* ra, const uint8_t* rb,
+                                                 uint32_t mb_next, int lane)
+{
+    constexpr int J = 2;
+    const pair_sel S = pair_sel_of(B.mb);
+    uint32_t Db[4 * J];
+    auto item = [&](int s) {
+        const int it = kW2ItemAt[s];
+        if (it >= 0 && it < NI) {
+            if (store) pair_store_item<NI>(it, B, S, r);
+            pair_load_item<NI, 2 * kWBlk * J>(it, ra, rb, lane, B);
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < kPairs; ++i) {
+        wide_read<J>(lane_q, wide2_entry(eb, i), Db);
+        wide_acc_r<J>(Da, wide2_entry(ea, i) & 3u, acca);
+        item(2 * i);
+        if (i + 1 < kPairs) wide_read<J>(lane_q, wide2_entry(ea, i + 1), Da);
+        wide_acc_r<J>(Db, wide2_entry(eb, i) & 3u, accb);
+        item(2 * i + 1);
+    }
+    B.mb = mb_next;
+}
+
+template <int NI>
+__device__ __forceinline__ void process_wide2_tile(const lds_params& p, int dt, long t0)
+{
+    constexpr int J = 2, A = 4 * J, SPLIT = kWBlk * J, T = 2 * SPLIT;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int da = dt * kDms2 + wave, db = da + kDms;
+    const bool act_a = da < p.ndms, act_b = db < p.ndms;
+    const int nchunks = (p.nchans + kPairChunk - 1) / kPairChunk;
+    const pair_rec2* recs = reinterpret_cast<const pair_rec2*>(p.pair_recs) + (size_t)dt * nchunks;
+    uint32_t acc16a[A], acc16b[A];
+    uint32_t acc32a[2 * A], acc32b[2 * A];
+#pragma unroll
+    for (int j = 0; j < A; ++j) acc16a[j] = acc16b[j] = 0;
+#pragma unroll
+    for (int j = 0; j < 2 * A; ++j) acc32a[j] = acc32b[j] = 0;
+
+    pair_build<NI> B;
+    pair_build_load<NI, T>(p, t0, recs[0].row[wave], lane, B);
+    if (wave < recs[0].nrows) pair_build_store(wave, lane, B, pair_buf(0));
+    const int k1 = min(1, nchunks - 1);
+    bool have_next = k1 == 1 && wave < recs[k1].nrows;   // B holds chunk k+1's row
+    pair_build_load<NI, T>(p, t0, recs[k1].row[wave], lane, B);
+    int kr = min(2, nchunks - 1);
+    pair_row rn = recs[kr].row[wave];                     // row record of chunk k+2
+    int nrows_n = recs[kr].nrows;
+    pair_ents Ea = pair_ents_of(recs[0], wave), Eb = pair_ents_of(recs[0], wave + kDms);
+    __syncthreads();
+    const uint32_t* lane_q = reinterpret_cast<const uint32_t*>(pair_buf(0)) + 4 * lane;
+    for (int k = 0; k < nchunks; ++k) {
+        const uint32_t* base = lane_q + ((k & 1) ? kPRows * kPRowDw : 0);   // rows of chunk k
+        uint4* rows_next = pair_buf((k + 1) & 1);
+        uint32_t Da[A];
+        wide_read<J>(base, Ea.e[0] & 0xffffu, Da);
+        const bool store = have_next;
+        have_next = k + 2 < nchunks && wave < nrows_n;
+        if (act_a && act_b && Ea.n == kPairs) {   // the usual full chunk: straight-line code
+            wide2_sum8_build<NI>(base, Ea.e, Eb.e, Da, acc16a, acc16b, B, store, rows_next + wave * kPRQ + lane,
+                                 p.L + t0 + rn.offa, p.L + t0 + rn.offb + rn.sh, rn.mb, lane);
+        } else {
+            if (store) pair_build_store(wave, lane, B, rows_next);
+            pair_build_load<NI, T>(p, t0, rn, lane, B);
+            if (act_a) wide_sum<J>(recs[k], wave, Ea.n, base, acc16a);
+            if (act_b) wide_sum<J>(recs[k], wave + kDms, Eb.n, base, acc16b);
+        }
+        if ((k + 1) % kPFlushChunks == 0 || k + 1 == nchunks) {
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                acc32a[j] += acc16a[j] & 0xFFFFu;
+                acc32a[A + j] += acc16a[j] >> 16;
+                acc16a[j] = 0;
+                acc32b[j] += acc16b[j] & 0xFFFFu;
+                acc32b[A + j] += acc16b[j] >> 16;
+                acc16b[j] = 0;
+            }
+        }
+        kr = min(k + 3, nchunks - 1);
+        rn = recs[kr].row[wave];
+        nrows_n = recs[kr].nrows;
+        const int kn = min(k + 1, nchunks - 1);
+        Ea = pair_ents_of(recs[kn], wave);
+        Eb = pair_ents_of(recs[kn], wave + kDms);
+        __syncthreads();
+    }
+
+    if (lane < 63) {
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            if (!(m == 0 ? act_a : act_b)) continue;
+            float* orow = p.out + (size_t)(m == 0 ? da : db) * p.out_pitch;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int x = 0; x < A; ++x) {
+                    const long tt = t0 + h * SPLIT + kWBlk * (x >> 2) + 4 * lane + (x & 3);
+                    if (tt < p.t_processed) orow[tt] = out_value(m == 0 ? acc32a[h * A + x] : acc32b[h * A + x], p);
+                }
+        }
+    }
+}
+
+template <int NI>
+__global__ __launch_bounds__(kThreads) void dedisperse_wide2_kernel(lds_params p)
+{
+    const int ntiles = p.n_dm_tiles * p.n_t_tiles;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int w = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (w >= ntiles) return;  // whole work-group leaves before any barrier
+    process_wide2_tile<NI>(p, w % p.n_dm_tiles, (long)(w / p.n_dm_tiles) * kWT2);
+}
+
+// Shape-6 ranges of a batch: the queues and the job lookup of
+// dedisperse_wide_batch_kernel.
+template <int NI>
+__global__ __launch_bounds__(kThreads) void dedisperse_wide2_batch_kernel(const lds_job* __restrict__ jobs, int njobs,
+                                                                          const long* __restrict__ bounds)
+{
+    typedef const __attribute__((address_space(4))) lds_job const_job;
+    const int x = (int)(blockIdx.x & 7u);
+    const long pos = (long)(blockIdx.x >> 3);
+    if (pos >= bounds[x + 1] - bounds[x]) return;   // whole work-group leaves before any barrier
+    long local;
+    const lds_job& j0 = batch_job_of(jobs, njobs, x, pos, &local);
+    const const_job& jb = ((const const_job*)jobs)[&j0 - jobs];
+    process_wide2_tile<NI>((const lds_params&)jb.p, (int)(local % jb.p.n_dm_tiles),
+                           (local / jb.p.n_dm_tiles) * (long)kWT2);
+}
+
 }  // namespace
 
 hipError_t launch_dedisperse_wide_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, long max_queue,
@@ -448,6 +606,29 @@ hipError_t launch_dedisperse_wide(const lds_params& q, int J, int ni, unsigned g
     } else {
         hipLaunchKernelGGL((dedisperse_wide_kernel<5, 2>), g, b, 0, s, q);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_dedisperse_wide2_batch(const lds_job* d_jobs, int njobs, const long* d_bounds, long max_queue,
+                                         int ni, hipStream_t s)
+{
+    if (njobs <= 0 || max_queue <= 0) return hipSuccess;
+    if (!d_jobs || !d_bounds || max_queue > 0x0fffffffL) return hipErrorInvalidValue;
+    const dim3 g((unsigned)(8 * max_queue)), b(kThreads);
+    if (ni <= 2) hipLaunchKernelGGL(dedisperse_wide2_batch_kernel<2>, g, b, 0, s, d_jobs, njobs, d_bounds);
+    else if (ni == 3) hipLaunchKernelGGL(dedisperse_wide2_batch_kernel<3>, g, b, 0, s, d_jobs, njobs, d_bounds);
+    else if (ni == 4) hipLaunchKernelGGL(dedisperse_wide2_batch_kernel<4>, g, b, 0, s, d_jobs, njobs, d_bounds);
+    else hipLaunchKernelGGL(dedisperse_wide2_batch_kernel<5>, g, b, 0, s, d_jobs, njobs, d_bounds);
+    return hipGetLastError();
+}
+
+hipError_t launch_dedisperse_wide2(const lds_params& q, int ni, unsigned grid, hipStream_t s)
+{
+    const dim3 g(grid), b(kThreads);
+    if (ni <= 2) hipLaunchKernelGGL(dedisperse_wide2_kernel<2>, g, b, 0, s, q);
+    else if (ni == 3) hipLaunchKernelGGL(dedisperse_wide2_kernel<3>, g, b, 0, s, q);
+    else if (ni == 4) hipLaunchKernelGGL(dedisperse_wide2_kernel<4>, g, b, 0, s, q);
+    else hipLaunchKernelGGL(dedisperse_wide2_kernel<5>, g, b, 0, s, q);
     return hipGetLastError();
 }
 

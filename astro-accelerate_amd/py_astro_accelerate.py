@@ -798,7 +798,8 @@ class DdtrPlan:
     def set_range_variant(self, r: int, variant: int):
         """Preferred LDS chunk shape of range r (0: 32 channels, 1: 16 channels, 2: 16 channels in
         pre-summed pairs on 1008-sample tiles, 3: 16 channels in pre-summed groups of four, 4: shape 2
-        on 2016-sample tiles, 5: shape 2 on 1512-sample tiles, -1: first that fits)."""
+        on 2016-sample tiles, 5: shape 2 on 1512-sample tiles, 6: shape 2 with 32 DM trials per tile,
+        -1: first that fits)."""
         check(_lib_or_raise().aa_hip_ddtr_set_range_variant(self.handle, int(r), int(variant)), "set_range_variant")
 
     def autotune_range(self, r: int, t_processed: int, out, reps: int = 2, stream=None):
@@ -1020,4 +1021,4 @@ def inject_pulse_u8(d_data_tc, nsamp: int, nchans: int, d_dmshifts, dm: float, t
 
 
 KERNEL_AUTO, KERNEL_DIRECT, KERNEL_LDS, KERNEL_LDS_C32, KERNEL_LDS_C16, KERNEL_LDS_PAIR, KERNEL_LDS_QUAD = range(7)
-LDS_SHAPES = 6   # LDS chunk shapes 0-5 (set_range_variant, autotune_range)
+LDS_SHAPES = 7   # LDS chunk shapes 0-6 (set_range_variant, autotune_range)

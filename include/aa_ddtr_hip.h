@@ -266,19 +266,20 @@ int aa_hip_ddtr_run_range(aa_hip_ddtr *h, int range, int t_processed, float *d_o
 
 /* Tile tuning (no reference counterpart: the reference picks its kernel by a
  * fixed LDS-size test, src/aa_dedisperse.cpp:90-131).  The LDS kernel has
- * six channel-chunk shapes: 0 = 32 channels x 2432-byte rows, 1 = 16
+ * seven channel-chunk shapes: 0 = 32 channels x 2432-byte rows, 1 = 16
  * channels x 4864-byte rows, 2 = 16 channels pre-added in channel pairs
  * (rows of 16-bit pair sums, one per distinct relative delay of the pair in
  * a 16-DM tile) on 1008-sample tiles (window offsets up to 772 samples),
  * 3 = 16 channels pre-added in groups of four, 4 = shape 2 on 2016-sample
  * tiles (offsets up to 268), 5 = shape 2 on 1512-sample tiles (offsets up
- * to 520); shapes 2, 4 and 5 read rows with 16-byte LDS reads.
+ * to 520), 6 = shape 2 with 32-DM tiles (two DM trials per wave over the
+ * same rows); shapes 2, 4, 5 and 6 read rows with 16-byte LDS reads.
  * set_range_variant records a preference for range r (-1 = first that
- * fits, in the order 4, 5, 2, 3, 0, 1), used whenever it fits.
+ * fits, in the order 4, 5, 2, 3, 0, 1, 6), used whenever it fits.
  * autotune_range times every shape that fits range r on `stream` (reps
  * launches each, writing d_out), keeps the fastest as the preference and
- * reports it: *best_variant (-1 if none fits) and variant_ms[0..5] (-1 =
- * does not fit; the array holds 6 floats).
+ * reports it: *best_variant (-1 if none fits) and variant_ms[0..6] (-1 =
+ * does not fit; the array holds 7 floats).
  * The range's level must be resident (after ingest). */
 int aa_hip_ddtr_set_range_variant(aa_hip_ddtr *h, int range, int variant);
 int aa_hip_ddtr_autotune_range(aa_hip_ddtr *h, int range, int t_processed, float *d_out,

@@ -62,6 +62,8 @@ def beams(aa, cuda):
     bs = mb.build_beams(small_specs(), cuda)
     bs[4].plan.set_range_variant(0, 1)   # 16-channel chunks where 32 also fit
     bs[1].plan.set_range_variant(1, 1)
+    bs[3].plan.set_range_variant(0, 6)   # 32 DM trials per tile (its own batched launch)
+    bs[2].plan.set_range_variant(2, 6)
     return bs
 
 
@@ -128,7 +130,7 @@ def test_autotune_keeps_results_bit_exact(aa, oracle, cuda, beams, tmp_path):
     import multibeam as mb
     say("autotune then batched launch")
     cache = mb.autotune(beams, cache_path=str(tmp_path / "tune.json"), reps=1)
-    assert cache and all(v["variant"] in (-1, 0, 1, 2, 3, 4, 5) for v in cache.values())
+    assert cache and all(v["variant"] in (-1, 0, 1, 2, 3, 4, 5, 6) for v in cache.values())
     for v in cache.values():   # the fastest shape that fits; -1 only when none fits
         fits = [ms for ms in v["ms"] if ms >= 0]
         assert (v["variant"] == -1) == (not fits)
@@ -136,7 +138,7 @@ def test_autotune_keeps_results_bit_exact(aa, oracle, cuda, beams, tmp_path):
             assert v["ms"][v["variant"]] == min(fits)
     # the full-band beam's second range does not fit the 32-channel shape
     key = repr(mb.shape_class(beams[4], 1))
-    assert cache[key]["ms"][0] == -1.0 and cache[key]["variant"] in (1, 2, 3, 4, 5)
+    assert cache[key]["ms"][0] == -1.0 and cache[key]["variant"] in (1, 2, 3, 4, 5, 6)
     mb.run_per_beam(beams)
     want = mb.output_digest(beams)
     batch = aa.DdtrBatch([b.plan for b in beams])

@@ -224,9 +224,10 @@ def test_execute_chunks_multi_range_multi_chunk(aa, oracle, cuda, kernel):
         inc += tps[0]
 
 
-@pytest.mark.parametrize("variant", [2, 4, 5])
+@pytest.mark.parametrize("variant", [2, 4, 5, 6])
 def test_pair_shape_forced_with_saturated_stretch(aa, oracle, cuda, variant):
-    """Each 16-byte-read pair shape (2: 1008-sample tiles, 4: 2016, 5: 1512) on every range of a
+    """Each 16-byte-read pair shape (2: 1008-sample tiles, 4: 2016, 5: 1512, 6: 1008 with 32 DM
+    trials per tile) on every range of a
     2-range plan over 512 channels (32 chunks: two windows of the 16-bit accumulators), on noise
     with a stretch of all-255 samples, where every 16-bit field reaches its 65,280 bound at each
     flush, and a pulse."""
