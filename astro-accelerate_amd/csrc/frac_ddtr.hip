@@ -50,21 +50,36 @@ __shared__ double f_red[16];
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // v[0..19] = the five quads of a lane run starting e samples into the row.
-// Each quad stays one 16-byte read: the empty asm uses all four words, so
-// the compiler can neither narrow a read to the words one shift needs nor
-// move it into the shift's branch (words read at a 16-byte lane stride
-// would conflict on the LDS banks).
-__device__ __forceinline__ void read_window(const float4* __restrict__ row, int e, int lane, float (&v)[20])
+// With e/4 = 4 s + P, quad m of the run sits at plane (P + m) & 3, slot
+// s + ((P + m) >> 2) + lane: one address per entry (base = row + s + lane)
+// and, per phase P, five reads at offsets fixed at compile time.  Each quad
+// stays one 16-byte read: the empty asm uses all four words, so the
+// compiler can neither narrow a read to the words one shift needs nor move
+// it into the shift's branch (words read at a 16-byte lane stride would
+// conflict on the LDS banks).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <int P>
+__device__ __forceinline__ void read_window_p(const float4* __restrict__ base, float (&v)[20])
 {
-    const int e4 = e >> 2;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const int q = e4 + m;
-        const float4 t = row[(q & 3) * kPlane + (q >> 2) + lane];
+        const f32x4 t = *reinterpret_cast<const f32x4*>(base + ((P + m) & 3) * kPlane + ((P + m) >> 2));
         v[4 * m + 0] = t.x;
         v[4 * m + 1] = t.y;
         v[4 * m + 2] = t.z;
         v[4 * m + 3] = t.w;
+    }
+}
+
+__device__ __forceinline__ void read_window(const float4* __restrict__ row, int e, int lane, float (&v)[20])
+{
+    const int e4 = e >> 2;
+    const float4* base = static_cast<const float4*>(__builtin_assume_aligned(row + (e4 >> 2) + lane, 16));
+    switch (e4 & 3) {
+    case 0: read_window_p<0>(base, v); break;
+    case 1: read_window_p<1>(base, v); break;
+    case 2: read_window_p<2>(base, v); break;
+    default: read_window_p<3>(base, v); break;
     }
 #pragma unroll
     for (int m = 0; m < 5; ++m)
