@@ -260,6 +260,32 @@ def test_pair_shape_forced_with_saturated_stretch(aa, oracle, cuda, variant):
         assert np.all(got[:, 9000 // ib:(21000 - s.maxshift) // ib] == np.float32(255.0) / np.float32(ib))
 
 
+@pytest.mark.parametrize("ndms", [70, 48, 33])
+def test_shape6_ragged_dm_tiles(aa, oracle, cuda, ndms):
+    """Shape 6 (32 DM trials per tile, two per wave) with a DM count that leaves the last tile
+    partly empty: trials past ndms are neither summed into another row nor written."""
+    import torch
+    say(f"shape 6 with {ndms} DM trials")
+    nchans, fch1, foff, tsamp, low, step, tp = 96, 1500.0, -1.0, 64e-6, 0.0, 0.5, 5000
+    ds = oracle.dmshifts(nchans, fch1, foff)
+    tsr = np.float32(tsamp)
+    ms, st = np.float32(np.float32(low) / tsr), np.float32(np.float32(step) / tsr)
+    maxd = int(np.trunc(np.float32(ds[-1]) * np.float32(np.float64(ndms - 1) * st + ms))) + 1
+    nsamp = tp + maxd
+    data = oracle.gen_noise(ndms, nsamp, nchans)
+    want = oracle.dedisperse(oracle.corner_turn(data, nsamp, nchans), ds, ndms, tp, float(ms), float(st), 1)
+    plan = aa.DdtrPlan(ds, [(low, step, ndms, 1)], tsamp, maxd, tp)
+    d_in = to_dev(data, cuda)
+    out = torch.full((ndms, tp), -1.0, dtype=torch.float32, device=cuda)
+    plan.execute_chunk(d_in, [tp], [out])
+    _, msv = plan.autotune_range(0, tp, out, reps=1)
+    assert msv[6] > 0, f"shape 6 does not fit: {msv}"
+    plan.set_range_variant(0, 6)
+    out.fill_(-1.0)
+    plan.execute_chunk(d_in, [tp], [out])
+    np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
 def _metadata(aa, nchans, nsamp, fch1, foff, tsamp):
     return aa.make_metadata(nchans, nsamp, fch1, foff, tsamp)
 
