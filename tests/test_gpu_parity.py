@@ -260,13 +260,14 @@ def test_pair_shape_forced_with_saturated_stretch(aa, oracle, cuda, variant):
         assert np.all(got[:, 9000 // ib:(21000 - s.maxshift) // ib] == np.float32(255.0) / np.float32(ib))
 
 
-@pytest.mark.parametrize("ndms", [70, 48, 33])
-def test_shape6_ragged_dm_tiles(aa, oracle, cuda, ndms):
+@pytest.mark.parametrize("ndms,nchans", [(70, 96), (48, 77), (33, 96)])
+def test_shape6_ragged_dm_tiles(aa, oracle, cuda, ndms, nchans):
     """Shape 6 (32 DM trials per tile, two per wave) with a DM count that leaves the last tile
-    partly empty: trials past ndms are neither summed into another row nor written."""
+    partly empty (trials past ndms are neither summed into another row nor written), and a
+    channel count whose last chunk holds 6 pairs and a single channel."""
     import torch
-    say(f"shape 6 with {ndms} DM trials")
-    nchans, fch1, foff, tsamp, low, step, tp = 96, 1500.0, -1.0, 64e-6, 0.0, 0.5, 5000
+    say(f"shape 6 with {ndms} DM trials, {nchans} channels")
+    fch1, foff, tsamp, low, step, tp = 1500.0, -1.0, 64e-6, 0.0, 0.5, 5000
     ds = oracle.dmshifts(nchans, fch1, foff)
     tsr = np.float32(tsamp)
     ms, st = np.float32(np.float32(low) / tsr), np.float32(np.float32(step) / tsr)
