@@ -30,8 +30,10 @@
 // group of S lanes covers the window's samples, 4 consecutive samples per
 // lane (S = ceil(T / 4) rounded up to whole waves), and the work-group holds
 // G = 1024 / S such groups, each summing every G-th channel; so short windows
-// keep every lane busy.  A lane reads its 5-sample span x[c][4s + i_c ..
-// 4s + i_c + 4] as one 16-byte and one 4-byte load, 4 channels in flight.
+// keep every lane busy.  A lane reads its 4 samples x[c][4s + i_c ..
+// 4s + i_c + 3] as one 16-byte load, 4 channels in flight; the taps that
+// reach into the next lane's samples are exchanged once per pass through
+// LDS (pass 1) or folded into per-lane weights (pass 2).
 // Pass 1 sums the groups' partial profiles in double, adds them in LDS in
 // group order, pass 2 re-reads the windows (recomputation: no shifted tile is
 // stored) and accumulates the DM derivative, and thread 0 applies a
@@ -89,24 +91,28 @@ __device__ __forceinline__ lane_layout layout_of(int T, int tid)
     return L;
 }
 
-// w[m] = row[j + m], m = 0..4, zero outside [0, T).
-__device__ __forceinline__ void load5(const float* __restrict__ row, int j, int T, float (&w)[5])
+// w[m] = row[j + m], m = 0..3, zero outside [0, T).
+__device__ __forceinline__ void load4(const float* __restrict__ row, int j, int T, float (&w)[4])
 {
-    if (j >= 0 && j + 4 < T) {
+    if (j >= 0 && j + 3 < T) {
         float4 q;
         __builtin_memcpy(&q, row + j, sizeof(q));   // dword-aligned 16-byte load
         w[0] = q.x;
         w[1] = q.y;
         w[2] = q.z;
         w[3] = q.w;
-        w[4] = row[j + 4];
     } else {
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
+        for (int m = 0; m < 4; ++m) {
             const int i = j + m;
             w[m] = (i >= 0 && i < T) ? row[i] : 0.0f;
         }
     }
+}
+
+__device__ __forceinline__ float load1(const float* __restrict__ row, int i, int T)
+{
+    return (i >= 0 && i < T) ? row[i] : 0.0f;
 }
 
 // (L, dL/du) of candidate `cd` at residual DM u; every thread gets both
@@ -137,46 +143,65 @@ __device__ void evaluate(const refine_args& a, const refine_cand& cd, const doub
     }
     __syncthreads();
 
-    // pass 1: this group's channels of the profile at the lane's 4 samples
+    // pass 1: this group's channels of the profile at the lane's 4 samples.
+    // A lane reads only its own 4 samples x[c][t0 + i_c + m]; the second tap
+    // of sample t0 - 1 (f x[c][t0 + i_c]) is its left neighbour's, kept in yn
+    // and added to the neighbour's sample in LDS after the pass.  The last
+    // lane of a group also reads x[c][4S + i_c] when its last sample, 4S - 1,
+    // is inside the window (T = 4S).
     const lane_layout L = layout_of(T, tid);
     const int t0 = kSpt * L.s;
-    double y[kSpt];
+    const int S = kThreads / L.G;
+    const bool edge = L.active && L.s == S - 1 && kSpt * S == T;
+    double y[kSpt], yn = 0.0;
 #pragma unroll
     for (int m = 0; m < kSpt; ++m) y[m] = 0.0;
     if (L.active) {
         int c = L.g;
         for (; c + (kU - 1) * L.G < C; c += kU * L.G) {
-            float w[kU][5];
+            float w[kU][4];
             double f[kU];
+            int i0[kU];
 #pragma unroll
             for (int k = 0; k < kU; ++k) {
                 const int cc = c + k * L.G;
                 const int2 e = r_tab[cc];
                 f[k] = (double)__int_as_float(e.y);
-                load5(x + (size_t)cc * T, t0 + e.x, T, w[k]);
+                i0[k] = e.x;
+                load4(x + (size_t)cc * T, t0 + e.x, T, w[k]);
             }
 #pragma unroll
-            for (int k = 0; k < kU; ++k)
+            for (int k = 0; k < kU; ++k) {
 #pragma unroll
-                for (int m = 0; m < kSpt; ++m)
-                    y[m] = __builtin_fma(f[k], (double)w[k][m + 1], __builtin_fma(1.0 - f[k], (double)w[k][m], y[m]));
+                for (int m = 0; m < kSpt; ++m) y[m] = __builtin_fma(1.0 - f[k], (double)w[k][m], y[m]);
+#pragma unroll
+                for (int m = 1; m < kSpt; ++m) y[m - 1] = __builtin_fma(f[k], (double)w[k][m], y[m - 1]);
+                yn = __builtin_fma(f[k], (double)w[k][0], yn);
+                if (edge) y[kSpt - 1] = __builtin_fma(f[k], (double)load1(x + (size_t)(c + k * L.G) * T, T + i0[k], T),
+                                                      y[kSpt - 1]);
+            }
         }
         for (; c < C; c += L.G) {
-            float w[5];
+            float w[4];
             const int2 e = r_tab[c];
             const double f = (double)__int_as_float(e.y);
-            load5(x + (size_t)c * T, t0 + e.x, T, w);
+            load4(x + (size_t)c * T, t0 + e.x, T, w);
 #pragma unroll
-            for (int m = 0; m < kSpt; ++m)
-                y[m] = __builtin_fma(f, (double)w[m + 1], __builtin_fma(1.0 - f, (double)w[m], y[m]));
+            for (int m = 0; m < kSpt; ++m) y[m] = __builtin_fma(1.0 - f, (double)w[m], y[m]);
+#pragma unroll
+            for (int m = 1; m < kSpt; ++m) y[m - 1] = __builtin_fma(f, (double)w[m], y[m - 1]);
+            yn = __builtin_fma(f, (double)w[0], yn);
+            if (edge) y[kSpt - 1] = __builtin_fma(f, (double)load1(x + (size_t)c * T, T + e.x, T), y[kSpt - 1]);
         }
 #pragma unroll
         for (int m = 0; m < kSpt; ++m)
             if (t0 + m < T) r_y[L.g * T + t0 + m] = y[m];
     }
     __syncthreads();
-    // the whole profile at the lane's samples: groups added in order
-    double l2 = 0.0;
+    if (L.active && t0 >= 1 && t0 - 1 < T) r_y[L.g * T + t0 - 1] += yn;   // one lane per slot
+    __syncthreads();
+    // the whole profile at the lane's samples and at t0 - 1: groups added in order
+    double l2 = 0.0, yp = 0.0;
     if (L.active) {
 #pragma unroll
         for (int m = 0; m < kSpt; ++m) {
@@ -186,6 +211,8 @@ __device__ void evaluate(const refine_args& a, const refine_cand& cd, const doub
             y[m] = s;                                   // 0 outside 0 <= t < T
             if (L.g == 0) l2 = __builtin_fma(s, s, l2);
         }
+        if (t0 >= 1 && t0 - 1 < T)
+            for (int g = 0; g < L.G; ++g) yp += r_y[g * T + t0 - 1];
     }
 
     if (!want_grad) {
@@ -193,31 +220,44 @@ __device__ void evaluate(const refine_args& a, const refine_cand& cd, const doub
         *grad = 0.0;
         return;
     }
-    // pass 2: dE/du over this group's channels, recomputing the window reads
+    // pass 2: dE/du over this group's channels, recomputing the window reads:
+    // sum_t y(t) (x(t + i + 1) - x(t + i)) = sum_t (y(t - 1) - y(t)) x(t + i)
+    // + y(4S - 1) x(4S + i), so each lane weighs its own 4 samples by
+    // D[m] = y(t0 + m - 1) - y(t0 + m) (and the last lane, when T = 4S, its
+    // sample 4S + i by y(T - 1)).
+    double D[kSpt];
+    D[0] = yp - y[0];
+#pragma unroll
+    for (int m = 1; m < kSpt; ++m) D[m] = y[m - 1] - y[m];
     double gs = 0.0;
     if (L.active) {
         int c = L.g;
         for (; c + (kU - 1) * L.G < C; c += kU * L.G) {
-            float w[kU][5];
+            float w[kU][4];
+            int i0[kU];
 #pragma unroll
             for (int k = 0; k < kU; ++k) {
                 const int cc = c + k * L.G;
-                load5(x + (size_t)cc * T, t0 + r_tab[cc].x, T, w[k]);
+                i0[k] = r_tab[cc].x;
+                load4(x + (size_t)cc * T, t0 + i0[k], T, w[k]);
             }
 #pragma unroll
             for (int k = 0; k < kU; ++k) {
                 double s = 0.0;
 #pragma unroll
-                for (int m = 0; m < kSpt; ++m) s = __builtin_fma(y[m], (double)(w[k][m + 1] - w[k][m]), s);
+                for (int m = 0; m < kSpt; ++m) s = __builtin_fma(D[m], (double)w[k][m], s);
+                if (edge) s = __builtin_fma(y[kSpt - 1], (double)load1(x + (size_t)(c + k * L.G) * T, T + i0[k], T), s);
                 gs = __builtin_fma((double)ds[c + k * L.G], s, gs);
             }
         }
         for (; c < C; c += L.G) {
-            float w[5];
-            load5(x + (size_t)c * T, t0 + r_tab[c].x, T, w);
+            float w[4];
+            const int i0 = r_tab[c].x;
+            load4(x + (size_t)c * T, t0 + i0, T, w);
             double s = 0.0;
 #pragma unroll
-            for (int m = 0; m < kSpt; ++m) s = __builtin_fma(y[m], (double)(w[m + 1] - w[m]), s);
+            for (int m = 0; m < kSpt; ++m) s = __builtin_fma(D[m], (double)w[m], s);
+            if (edge) s = __builtin_fma(y[kSpt - 1], (double)load1(x + (size_t)c * T, T + i0, T), s);
             gs = __builtin_fma((double)ds[c], s, gs);
         }
     }
