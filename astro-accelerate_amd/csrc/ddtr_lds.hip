@@ -721,10 +721,27 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
             for (int c = 0; c < nchans; ++c) dl[(size_t)w * nchans + c] = (int)(h_dmshifts[c] * sd);
         }
         auto D = [&](int w, int c) { return dl[(size_t)w * nchans + c]; };
+        // Values taken by the relative delay of the pair starting at channel a.
+        auto pair_values = [&](int a) {
+            if (a + 1 >= nchans) return 1;
+            int lo = D(0, a + 1) - D(0, a), hi = lo;
+            for (int w = 1; w < NDM; ++w) {
+                lo = std::min(lo, D(w, a + 1) - D(w, a));
+                hi = std::max(hi, D(w, a + 1) - D(w, a));
+            }
+            return hi - lo + 1;
+        };
         for (int k = 0; k < nchunks; ++k) {
             pair_rec_t<NDM>* R = recs ? &(*recs)[(size_t)dt * nchunks + k] : nullptr;
             int nrows = 0, npairs = 0, nsplit = 0;
             int split_row[kPairs], split_e[kPairs][NDM];
+            // Rows left over when every pair of more than two values is
+            // split: a pair of exactly three values takes one of them and
+            // is built as three pair rows (one entry per DM instead of two)
+            // while they last, in pair order.
+            int spare = kPRows;
+            for (int pp = 0; pp < kPairs && k * kPairChunk + 2 * pp < nchans; ++pp)
+                spare -= std::min(2, pair_values(k * kPairChunk + 2 * pp));
             for (int pp = 0; pp < kPairs; ++pp) {
                 const int a = k * kPairChunk + 2 * pp;
                 if (a >= nchans) break;
@@ -742,7 +759,9 @@ static int pair_plan(const float* h_dmshifts, int nchans, int ndms, float mstart
                     vmax = w ? std::max(vmax, v[w]) : v[w];
                 }
                 const int nv = vmax - vmin + 1;
-                if (nv <= 2) {
+                const bool three_rows = nv == 3 && spare > 0;
+                if (three_rows) --spare;
+                if (nv <= 2 || three_rows) {
                     if (R) {
                         for (int i = 0; i < nv; ++i)
                             R->row[nrows + i] = make_row(a, basea, hasb ? a + 1 : -1, vmin + i);

@@ -73,8 +73,9 @@ static_assert(kPRows == kDms, "one row per wave is built");
 // Plan record of one (DM tile, chunk).  A pair whose relative delay takes
 // more than two values in the tile is split into two single-channel rows
 // (each DM then adds both), so a chunk never needs more than 16 rows or a DM
-// more than 16 entries; entries 0 .. npairs-1 belong to the pairs in order,
-// the second rows of split pairs follow.
+// more than 16 entries; a pair of three values is instead built as three
+// pair rows while the chunk's 16 rows allow.  Entries 0 .. npairs-1 belong
+// to the pairs in order, the second rows of split pairs follow.
 struct pair_row {
     long long offa, offb;   // bytes from the level start to the two channels' window starts (offb: dword floor)
     uint32_t sh, mb;        // byte shift of b's window in its dwords; ~0 pair row, 0 single row (b masked off)

@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void dedisperse_wide_batch_kernel(const l
 // Shape 6: 1008-sample tiles (J = 2) of 32 DM trials, two per wave (DM
 // slots w and w + 16 of wave w).  A chunk still builds at most 16 rows (a
 // pair's relative delay takes at most two values over the 32 trials, else
-// the pair is split into two single-channel rows), one per wave, so the
+// three pair rows or two single-channel rows), one per wave, so the
 // build and the row stores are shared by twice the trials of shape 2, and
 // a chunk holds as many adds per wave as shape 4's.
 // ---------------------------------------------------------------------------

@@ -17,7 +17,8 @@ Also reported:
                (78.64 T adds/s = 256 CU x 128 packed adds/clk x 2.4 GHz);
   cpu_baseline the CPU oracle (OpenMP C) on the host cores, timed on the
                whole hot path over 1/24 of the chunk's output samples (every
-               DM row of every range; --cpu-slice 1 runs the whole plane),
+               DM row of every range; --cpu-slice 1 runs the whole plane, the
+               default on the single-range plan),
                best of 3 runs after a warm-up run; the slice is also
                compared bit for bit with the GPU output.
 
@@ -82,9 +83,9 @@ def main():
     ap.add_argument("--tune-cache", default=None, help="config 4: JSON file of autotuned chunk shapes")
     ap.add_argument("--kernel", default="auto", choices=["auto", "direct", "lds", "c32", "c16", "pair", "quad"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-slice", type=int, default=24,
+    ap.add_argument("--cpu-slice", type=int, default=None,
                     help="CPU baseline: time the oracle on 1/N of the chunk's output samples (all DM rows); "
-                         "1 runs the whole plane")
+                         "1 runs the whole plane (default: 24 on the 8-range plan, 1 on the single range)")
     ap.add_argument("--cpu-repeats", type=int, default=3,
                     help="CPU baseline: timed runs after one warm-up run; the best is reported")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -788,6 +789,8 @@ def cpu_baseline(args, cfg, strat, d_in, outs, ndms, tps, dm_x_samples):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ddtr_oracle as oracle
 
+    if args.cpu_slice is None:
+        args.cpu_slice = 24 if args.config == 2 else 1
     threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or cpu_model()[1]
     nchans = cfg["nchans"]
     nr = strat.nranges

@@ -224,6 +224,25 @@ def test_execute_chunks_multi_range_multi_chunk(aa, oracle, cuda, kernel):
         inc += tps[0]
 
 
+def _three_row_pair_chunks(oracle, s, r, ndm):
+    """(DM tile, 16-channel chunk) records of range r in which a pair's relative delay takes three
+    values over the tile's ndm trials while the chunk's rows leave room for a third pair row (two
+    rows per pair of two or more values, 16 rows per chunk): the plan builds those pairs as three
+    pair rows instead of two single-channel rows.  Delays from the oracle's table."""
+    ms, st = s.range_scalars(r)
+    delay = oracle.delay_table(s.dmshifts, s.ndms[r], ms, st).astype(np.int64)
+    nch = delay.shape[1] // 2 * 2
+    count = 0
+    for d0 in range(0, s.ndms[r], ndm):
+        rows = delay[np.minimum(np.arange(d0, d0 + ndm), s.ndms[r] - 1)]
+        v = rows[:, 1:nch:2] - rows[:, 0:nch:2]
+        nv = v.max(axis=0) - v.min(axis=0) + 1
+        for k in range(0, nv.size, 8):
+            chunk = nv[k:k + 8]
+            count += int(np.any(chunk == 3) and np.minimum(chunk, 2).sum() < 16)
+    return count
+
+
 @pytest.mark.parametrize("variant", [2, 4, 5, 6])
 def test_pair_shape_forced_with_saturated_stretch(aa, oracle, cuda, variant):
     """Each 16-byte-read pair shape (2: 1008-sample tiles, 4: 2016, 5: 1512, 6: 1008 with 32 DM
@@ -237,6 +256,8 @@ def test_pair_shape_forced_with_saturated_stretch(aa, oracle, cuda, variant):
     ranges = [(0, 100, 0.5, 1, 1), (100, 200, 1.0, 2, 2)]
     s = oracle.strategy(ranges, nchans, nsamp, 1550.0, -0.5859, tsamp)
     assert s.ok and s.num_tchunks == 1
+    # both ranges hold pairs built as three pair rows
+    assert all(_three_row_pair_chunks(oracle, s, r, 32 if variant == 6 else 16) > 0 for r in range(s.nranges))
     data = oracle.gen_noise(5 + variant, nsamp, nchans)
     data[9000:21000, :] = 255
     oracle.inject_pulse(data, s.dmshifts, 60.0, tsamp, 4000, 6, 40)
