@@ -350,13 +350,15 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
     if (active && lane < 63) {
         const int d = dt * kDms + wave;
         float* orow = p.out + (size_t)d * p.out_pitch;
+        const bool aligned = (reinterpret_cast<uintptr_t>(orow) & 15u) == 0;
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int x = 0; x < A; ++x) {
-                const long tt = t0 + h * SPLIT + kWBlk * (x >> 2) + 4 * lane + (x & 3);
-                if (tt < p.t_processed)
-                    orow[tt] = out_value(acc32[h * A + x], p);
+            for (int b = 0; b < J; ++b) {
+                const uint32_t* s = acc32 + h * A + 4 * b;
+                pair_store_out4(orow, t0 + h * SPLIT + kWBlk * b + 4 * lane, p.t_processed, aligned,
+                                f32x4_out{out_value(s[0], p), out_value(s[1], p), out_value(s[2], p),
+                                          out_value(s[3], p)});
             }
     }
 }
@@ -519,12 +521,15 @@ __device__ __forceinline__ void process_wide2_tile(const lds_params& p, int dt, 
         for (int m = 0; m < 2; ++m) {
             if (!(m == 0 ? act_a : act_b)) continue;
             float* orow = p.out + (size_t)(m == 0 ? da : db) * p.out_pitch;
+            const bool aligned = (reinterpret_cast<uintptr_t>(orow) & 15u) == 0;
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int x = 0; x < A; ++x) {
-                    const long tt = t0 + h * SPLIT + kWBlk * (x >> 2) + 4 * lane + (x & 3);
-                    if (tt < p.t_processed) orow[tt] = out_value(m == 0 ? acc32a[h * A + x] : acc32b[h * A + x], p);
+                for (int b = 0; b < J; ++b) {
+                    const uint32_t* s = (m == 0 ? acc32a : acc32b) + h * A + 4 * b;
+                    pair_store_out4(orow, t0 + h * SPLIT + kWBlk * b + 4 * lane, p.t_processed, aligned,
+                                    f32x4_out{out_value(s[0], p), out_value(s[1], p), out_value(s[2], p),
+                                              out_value(s[3], p)});
                 }
         }
     }
