@@ -46,6 +46,27 @@ __device__ __forceinline__ float out_value(uint32_t sum, const lds_params& p)
     return p.fscale != 0.0f ? (float)sum * p.fscale : __fdiv_rn(__fdiv_rn((float)sum, p.fnchans), p.fbin);
 }
 
+// Output of one lane: samples tt .. tt+3 of a DM row (tt a multiple of 4),
+// one 16-byte store where the row start is 16-byte aligned and all four lie
+// inside the plane's t_end samples, else one store per sample inside it.
+typedef float f32x4_out __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_out4(float* orow, long tt, long t_end, bool aligned, f32x4_out v)
+{
+    if (aligned && tt + 3 < t_end) {
+        *reinterpret_cast<f32x4_out*>(orow + tt) = v;
+    } else {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+            if (tt + m < t_end) orow[tt + m] = v[m];
+    }
+}
+
+// The four sums s[0..3] as output values.
+__device__ __forceinline__ f32x4_out out_values4(const uint32_t* s, const lds_params& p)
+{
+    return f32x4_out{out_value(s[0], p), out_value(s[1], p), out_value(s[2], p), out_value(s[3], p)};
+}
+
 // One range of a batched launch.  The launch has 8 queues (one per XCD);
 // queue x holds, job after job, the x-th eighth of every job's tiles (tiles
 // [x n / 8, (x+1) n / 8) of a job of n = n_dm_tiles * n_t_tiles tiles), so

@@ -345,13 +345,11 @@ __device__ __forceinline__ void process_tile(const lds_params& p, int d0, long t
     }
 
     if (active) {
-        const long tl = t.t0 + (long)kR * t.lane;
+        const long tl = t.t0 + (long)kR * t.lane;   // a multiple of 4
         float* orow = p.out + (size_t)d * p.out_pitch;
+        const bool aligned = (reinterpret_cast<uintptr_t>(orow) & 15u) == 0;
 #pragma unroll
-        for (int j = 0; j < kR; ++j) {
-            const long tt = tl + j;
-            if (tt < p.t_processed) orow[tt] = out_value(acc32[j], p);
-        }
+        for (int j = 0; j < kR; j += 4) store_out4(orow, tl + j, p.t_processed, aligned, out_values4(acc32 + j, p));
     }
 }
 

@@ -95,21 +95,6 @@ static_assert(sizeof(pair_row) == 32 && sizeof(pair_rec) % 16 == 0 && sizeof(pai
               "records are 16-byte aligned");
 static_assert(kPRows * kPRowDw <= 65536, "entry offsets fit 16 bits");
 
-// Output of one lane: samples tt .. tt+3 of a DM row (tt a multiple of 4),
-// one 16-byte store where the row start is 16-byte aligned and all four lie
-// inside the plane's t_end samples, else one store per sample inside it.
-typedef float f32x4_out __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void pair_store_out4(float* orow, long tt, long t_end, bool aligned, f32x4_out v)
-{
-    if (aligned && tt + 3 < t_end) {
-        *reinterpret_cast<f32x4_out*>(orow + tt) = v;
-    } else {
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-            if (tt + m < t_end) orow[tt + m] = v[m];
-    }
-}
-
 // Two row buffers of 80 KiB (the whole LDS) in one array: buffer b starts
 // at quad b * kPRows * kPRQ.
 __shared__ uint4 g_prow[2 * kPRows * kPRQ];

@@ -355,10 +355,8 @@ __device__ __forceinline__ void process_wide_tile(const lds_params& p, int dt, l
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int b = 0; b < J; ++b) {
-                const uint32_t* s = acc32 + h * A + 4 * b;
-                pair_store_out4(orow, t0 + h * SPLIT + kWBlk * b + 4 * lane, p.t_processed, aligned,
-                                f32x4_out{out_value(s[0], p), out_value(s[1], p), out_value(s[2], p),
-                                          out_value(s[3], p)});
+                store_out4(orow, t0 + h * SPLIT + kWBlk * b + 4 * lane, p.t_processed, aligned,
+                           out_values4(acc32 + h * A + 4 * b, p));
             }
     }
 }
@@ -526,10 +524,8 @@ __device__ __forceinline__ void process_wide2_tile(const lds_params& p, int dt, 
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int b = 0; b < J; ++b) {
-                    const uint32_t* s = (m == 0 ? acc32a : acc32b) + h * A + 4 * b;
-                    pair_store_out4(orow, t0 + h * SPLIT + kWBlk * b + 4 * lane, p.t_processed, aligned,
-                                    f32x4_out{out_value(s[0], p), out_value(s[1], p), out_value(s[2], p),
-                                              out_value(s[3], p)});
+                    store_out4(orow, t0 + h * SPLIT + kWBlk * b + 4 * lane, p.t_processed, aligned,
+                               out_values4((m == 0 ? acc32a : acc32b) + h * A + 4 * b, p));
                 }
         }
     }
