@@ -379,6 +379,13 @@ int aa_hip_ddtr_batch_stats(const aa_hip_ddtr_batch *h, int *njobs, long *ntiles
     return AA_HIP_SUCCESS;
 }
 
+int aa_hip_ddtr_batch_ct_stats(const aa_hip_ddtr_batch *h, int *batched, int *own) {
+    if (!h || !h->batch) return bad("NULL handle");
+    if (batched) *batched = h->batch->nct_batched();
+    if (own) *own = h->batch->nct_own();
+    return AA_HIP_SUCCESS;
+}
+
 int aa_hip_corner_turn_u8(const uint8_t *d_in_tc, int nsamp, int nchans, uint8_t *d_out_ct, size_t out_pitch,
                           void *stream) {
     if (!d_in_tc || !d_out_ct) return bad("NULL buffer");

@@ -76,6 +76,8 @@ ddtr_batch::~ddtr_batch() {
         if (f.d_bounds) (void)hipFree(f.d_bounds);
         if (f.d_counters) (void)hipFree(f.d_counters);
     }
+    for (auto &f : m_ct)
+        if (f.d_jobs) (void)hipFree(f.d_jobs);
     if (m_capture_stream) (void)hipStreamDestroy(m_capture_stream);
 }
 
@@ -99,19 +101,60 @@ void ddtr_batch::release_graph() {
     m_key.clear();
 }
 
-// Launches ingest and binning of every beam and resolves every range into
+// Launches the corner turns of every beam (batched where the wide turn
+// applies), then resolves every range (binning deeper levels as needed) into
 // the families' job lists (m_fam[*].jobs) and the separate launches
-// (m_direct); the caller uploads the job lists (outside any graph capture)
-// before the persistent kernels run.
+// (m_direct); the caller uploads the family job lists (outside any graph
+// capture) before the persistent kernels run.
 int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float *const *d_out,
-                        const size_t *out_pitch, hipStream_t s) {
+                        const size_t *out_pitch, hipStream_t s, bool upload) {
     std::vector<lds_job> jobs[kFamilies];
     std::vector<range_launch> direct;
+    for (auto &f : m_ct) f.jobs.clear();
+    m_ct_direct = 0;
     int flat = 0;
     for (size_t b = 0; b < m_beams.size(); ++b) {
         ddtr_engine *e = m_beams[b];
-        int st = e->ingest(inputs[b], 8, t_processed[flat], s);
+        ct_job j{};
+        bool deferred = false;
+        int st = e->ingest_job(inputs[b], t_processed[flat], &j, &deferred);
+        if (st == AA_HIP_SUCCESS && !deferred) {
+            st = e->ingest(inputs[b], 8, t_processed[flat], s);
+            ++m_ct_direct;
+        }
         if (st != AA_HIP_SUCCESS) return st;
+        if (deferred) {
+            ct_family &F = m_ct[j.nlev];
+            j.tile_end = (F.jobs.empty() ? 0 : F.jobs.back().tile_end) + corner_turn_tiles(j.nsamp, j.nchans);
+            F.jobs.push_back(j);
+        }
+        flat += e->nranges();
+    }
+    for (auto &F : m_ct) {
+        if (F.jobs.empty()) continue;
+        if (F.jobs.back().tile_end > 0x7fffffffL) return fail(AA_HIP_ERROR_UNSUPPORTED, "corner turns exceed 2^31 tiles");
+        hipError_t e = hipSuccess;
+        if (!upload && (!F.d_jobs || F.jobs.size() > F.cap)) return fail(AA_HIP_ERROR_DEVICE, "batch changed in capture");
+        if (upload) {
+            if (F.jobs.size() > F.cap) {
+                if (F.d_jobs) (void)hipFree(F.d_jobs);
+                F.d_jobs = nullptr;
+                F.cap = 0;
+                e = hipMalloc(&F.d_jobs, F.jobs.size() * sizeof(ct_job));
+                if (e != hipSuccess) return hip_fail(e, "corner-turn job table");
+                F.cap = F.jobs.size();
+            }
+            // F.jobs stays as it is until the caller's synchronisation
+            e = hipMemcpyAsync(F.d_jobs, F.jobs.data(), F.jobs.size() * sizeof(ct_job), hipMemcpyHostToDevice, s);
+        }
+        if (e == hipSuccess)
+            e = launch_corner_turn_batch(F.d_jobs, (int)F.jobs.size(), F.jobs.back().tile_end, F.jobs.back().nlev, s);
+        if (e != hipSuccess) return hip_fail(e, "batched corner turn");
+    }
+    flat = 0;
+    for (size_t b = 0; b < m_beams.size(); ++b) {
+        ddtr_engine *e = m_beams[b];
+        int st = AA_HIP_SUCCESS;
         for (int r = 0; r < e->nranges(); ++r, ++flat) {
             range_launch rl;
             st = e->prepare_range(r, t_processed[flat], d_out[flat], out_pitch[flat], s, &rl);
@@ -199,7 +242,7 @@ int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float
     release_graph();  // the eager pass rewrites the job tables a captured graph reads
 
     // Eager pass: builds and uploads the job lists, and computes this call.
-    int st = enqueue(inputs, t_processed, d_out, out_pitch, s);
+    int st = enqueue(inputs, t_processed, d_out, out_pitch, s, true);
     if (st != AA_HIP_SUCCESS) return st;
     hipError_t e = hipSuccess;
     for (int f = 0; f < kFamilies && e == hipSuccess; ++f) {
@@ -236,14 +279,16 @@ int ddtr_batch::execute(const void *const *inputs, const int *t_processed, float
         if (e != hipSuccess) return hip_fail(e, "batch capture");
         cs = m_capture_stream;
     }
-    size_t nj[kFamilies];
+    size_t nj[kFamilies], nct[kCtMaxLevels + 1];
     for (int f = 0; f < kFamilies; ++f) nj[f] = m_fam[f].jobs.size();
+    for (int k = 0; k <= kCtMaxLevels; ++k) nct[k] = m_ct[k].jobs.size();
     const size_t nd = m_direct.size();
     e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return hip_fail(e, "batch capture");
-    st = enqueue(inputs, t_processed, d_out, out_pitch, cs);
+    st = enqueue(inputs, t_processed, d_out, out_pitch, cs, false);
     bool same = m_direct.size() == nd;
     for (int f = 0; f < kFamilies; ++f) same = same && m_fam[f].jobs.size() == nj[f];
+    for (int k = 0; k <= kCtMaxLevels; ++k) same = same && m_ct[k].jobs.size() == nct[k];
     if (st == AA_HIP_SUCCESS && !same) st = fail(AA_HIP_ERROR_DEVICE, "batch changed in capture");
     if (st == AA_HIP_SUCCESS) e = launch_families(cs);
     hipGraph_t g = nullptr;

@@ -4,7 +4,9 @@
 // (include/aa_pipeline_api.hpp:113-117).
 //
 // Per call: every beam's chunk is corner-turned into its engine's level 0 and
-// binned as its ranges need (the engines' own kernels).  The (beam, range)
+// the binned levels the corner turn writes, all beams that take the wide
+// corner turn in one launch per number of binned levels; deeper levels are
+// binned as the ranges need (the engines' own kernels).  The (beam, range)
 // jobs then run in launches over shared tile queues (8, one per XCD), one
 // launch per kernel instantiation: the two byte-row chunk shapes together
 // in a persistent launch (dedisperse_lds_batch_kernel), and the pair
@@ -50,6 +52,13 @@ public:
         return n;
     }
     int ndirect() const { return (int)m_direct.size(); }
+    // corner turns of the last call: batched launches, beams turned alone
+    int nct_batched() const {
+        int n = 0;
+        for (const auto &f : m_ct) n += f.jobs.empty() ? 0 : 1;
+        return n;
+    }
+    int nct_own() const { return m_ct_direct; }
     int nlaunches() const {
         int n = 0;
         for (const auto &f : m_fam) n += f.jobs.empty() ? 0 : 1;
@@ -59,8 +68,10 @@ public:
 
 private:
     ddtr_batch() = default;
+    // upload: copy the corner-turn job tables to the device first (the eager
+    // pass; a captured pass reuses them)
     int enqueue(const void *const *inputs, const int *t_processed, float *const *d_out, const size_t *out_pitch,
-                hipStream_t s);
+                hipStream_t s, bool upload);
     hipError_t launch_families(hipStream_t s);
     void release_graph();
 
@@ -87,6 +98,14 @@ private:
     static int family_of(int variant, int row_quads);   // -1: launched on its own
     family m_fam[kFamilies];
     std::vector<range_launch> m_direct;
+    // Batched corner turns by number of binned levels (0..kCtMaxLevels).
+    struct ct_family {
+        std::vector<ct_job> jobs;
+        ct_job *d_jobs = nullptr;
+        size_t cap = 0;
+    };
+    ct_family m_ct[kCtMaxLevels + 1];
+    int m_ct_direct = 0;   // beams turned by launches of their own
     hipStream_t m_capture_stream = nullptr;  // for callers on the legacy default stream
     hipGraph_t m_graph = nullptr;
     hipGraphExec_t m_exec = nullptr;

@@ -291,6 +291,33 @@ int ddtr_engine::ingest(const void *d_input_tc, int input_bits, int t_processed0
     return AA_HIP_SUCCESS;
 }
 
+int ddtr_engine::ingest_job(const void *d_input_tc, int t_processed0, ct_job *job, bool *deferred) {
+    *deferred = false;
+    if (!d_input_tc) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "input is NULL");
+    const long len0 = (long)t_processed0 + m_maxshift;
+    if (t_processed0 <= 0 || len0 > m_levels[0].capacity)
+        return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "t_processed0 exceeds the handle's capacity");
+    if (m_sample_bytes != 1 || len0 > 0x7fffffffL) return AA_HIP_SUCCESS;   // ingest() handles these
+    const int nlev = std::min((int)m_levels.size() - 1, kCtMaxLevels);
+    ct_job j{};
+    for (int k = 0; k <= nlev; ++k) {
+        j.ptr[k] = m_levels[(size_t)k].ptr;
+        j.pitch[k] = m_levels[(size_t)k].pitch;
+        if ((size_t)(len0 >> k) > j.pitch[k]) return fail(AA_HIP_ERROR_INVALID_ARGUMENT, "level row too short");
+    }
+    if (!corner_turn_levels_wide(d_input_tc, m_nchans, j.ptr, j.pitch, nlev)) return AA_HIP_SUCCESS;
+    j.in = static_cast<const uint8_t *>(d_input_tc);
+    j.nsamp = len0;
+    j.nchans = m_nchans;
+    j.nlev = nlev;
+    *job = j;
+    for (auto &lv : m_levels) lv.len = 0;
+    for (int k = 0; k <= nlev; ++k) m_levels[(size_t)k].len = len0 >> k;
+    m_resident = nlev;
+    *deferred = true;
+    return AA_HIP_SUCCESS;
+}
+
 int ddtr_engine::take_input_overflow(hipStream_t s) {
     if (m_sample_bytes == 2) return 0;   // u16 levels hold any 16-bit value
     device_guard guard(m_device);

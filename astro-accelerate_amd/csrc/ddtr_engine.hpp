@@ -94,6 +94,12 @@ public:
     void range_scalars(int r, float *mstartdm, float *mdmstep) const;
 
     int ingest(const void *d_input_tc, int input_bits, int t_processed0, hipStream_t s);
+    // ingest() of an 8-bit chunk left to a batched launch: when the chunk
+    // and this plan's levels take the wide corner turn, fills *job with it
+    // (tile_end unset), marks its levels resident and sets *deferred; else
+    // leaves everything as it was (the caller runs ingest()).  The job must
+    // run before anything reads the levels.
+    int ingest_job(const void *d_input_tc, int t_processed0, ct_job *job, bool *deferred);
     // Synchronises `s`, returns 1 if a u16 ingest met a sample above 255
     // since the last call (and clears the flag), 0 if not, <0 on error.
     int take_input_overflow(hipStream_t s);

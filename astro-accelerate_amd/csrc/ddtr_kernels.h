@@ -41,6 +41,24 @@ constexpr int kCtMaxLevels = 3;
 hipError_t launch_corner_turn_levels(const void* in_tc, int sample_bytes, long nsamp, int nchans,
                                      uint8_t* const* ptrs, const size_t* pitches, int nlev, int* d_err,
                                      hipStream_t s, int* fused);
+// Whether launch_corner_turn_levels writes all nlev binned levels (its wide
+// path) for these buffers, and the work-groups that path launches.
+bool corner_turn_levels_wide(const void* in_tc, int nchans, uint8_t* const* ptrs, const size_t* pitches, int nlev);
+long corner_turn_tiles(long nsamp, int nchans);
+// One u8 chunk of a batched corner turn: as launch_corner_turn_levels on its
+// wide path (level 0 and nlev binned levels at ptr[k], rows pitch[k] bytes
+// apart).  tile_end: work-groups of this job and the jobs before it.
+struct ct_job {
+    const uint8_t* in;
+    long nsamp;
+    int nchans, nlev;
+    uint8_t* ptr[kCtMaxLevels + 1];
+    size_t pitch[kCtMaxLevels + 1];
+    long tile_end;
+};
+// Every job of d_jobs (device memory; all with this nlev, the wide path
+// holding for each, tile_end filled in) in one launch of ntiles work-groups.
+hipError_t launch_corner_turn_batch(const ct_job* d_jobs, int njobs, long ntiles, int nlev, hipStream_t s);
 // u16 [t][c] -> u16 [c][t] (the reference's in-place corner turn, via temp).
 hipError_t launch_transpose_u16(const uint16_t* in, int rows, int cols,
                                 uint16_t* out, hipStream_t s);

@@ -125,15 +125,17 @@ __device__ __forceinline__ uint4 ct_load16<uint16_t>(const uint16_t* p, bool& ba
                       perm_b32(b.y, b.x, 0x06040200u), perm_b32(b.w, b.z, 0x06040200u));
 }
 
+// Tile g of the turn of one chunk (nsamp x nchans): channel group
+// g % ngroups of time tile g / ngroups.
 template <typename Tin, int NLEV>
-__global__ __launch_bounds__(kCtT) void corner_turn_wide_kernel(const Tin* __restrict__ in, long nsamp, int nchans,
-                                                                ct_levels lv, int* __restrict__ err)
+__device__ __forceinline__ void ct_wide_tile(const Tin* __restrict__ in, long nsamp, int nchans, const ct_levels& lv,
+                                             int* __restrict__ err, long g)
 {
     __shared__ __align__(16) uint8_t tile[kCtC * kCtRow];
     const int tid = threadIdx.x;
     const int ngroups = (nchans + kCtC - 1) / kCtC;
-    const long t0 = (long)(blockIdx.x / ngroups) * kCtT;
-    const int c0 = (int)(blockIdx.x % ngroups) * kCtC;
+    const long t0 = (g / ngroups) * kCtT;
+    const int c0 = (int)(g % ngroups) * kCtC;
     {
         const int cq = tid & 7, tg = tid >> 3;
         const int c = c0 + 16 * cq;
@@ -209,6 +211,36 @@ __global__ __launch_bounds__(kCtT) void corner_turn_wide_kernel(const Tin* __res
             }
         }
     }
+}
+
+template <typename Tin, int NLEV>
+__global__ __launch_bounds__(kCtT) void corner_turn_wide_kernel(const Tin* __restrict__ in, long nsamp, int nchans,
+                                                                ct_levels lv, int* __restrict__ err)
+{
+    ct_wide_tile<Tin, NLEV>(in, nsamp, nchans, lv, err, blockIdx.x);
+}
+
+// The turns of many u8 chunks (one per job, all writing NLEV binned levels)
+// in one launch: work-group g turns tile g - tile_end[j - 1] of job j, the
+// first job whose tile_end exceeds g.
+template <int NLEV>
+__global__ __launch_bounds__(kCtT) void corner_turn_wide_batch_kernel(const ct_job* __restrict__ jobs, int njobs)
+{
+    const long g = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (jobs[mid].tile_end > g) hi = mid;
+        else lo = mid + 1;
+    }
+    const ct_job& J = jobs[lo];
+    ct_levels lv;
+#pragma unroll
+    for (int k = 0; k <= kCtMaxLevels; ++k) {
+        lv.ptr[k] = J.ptr[k];
+        lv.pitch[k] = J.pitch[k];
+    }
+    ct_wide_tile<uint8_t, NLEV>(J.in, J.nsamp, J.nchans, lv, nullptr, g - (lo ? jobs[lo - 1].tile_end : 0));
 }
 
 __global__ __launch_bounds__(256) void transpose_u16_kernel(const uint16_t* __restrict__ in, int rows,
@@ -519,6 +551,30 @@ static bool ct_wide_fits(const void* in, int nchans, uint8_t* const* ptrs, const
     for (int k = 0; k <= nlev; ++k)
         if ((uintptr_t)ptrs[k] % 16 != 0 || pitches[k] % 16 != 0) return false;
     return true;
+}
+
+bool corner_turn_levels_wide(const void* in_tc, int nchans, uint8_t* const* ptrs, const size_t* pitches, int nlev)
+{
+    return in_tc && nchans > 0 && nlev >= 0 && nlev <= kCtMaxLevels && ct_wide_fits(in_tc, nchans, ptrs, pitches, nlev);
+}
+
+long corner_turn_tiles(long nsamp, int nchans)
+{
+    return (long)ceil_div(nsamp, kCtT) * ceil_div(nchans, kCtC);
+}
+
+hipError_t launch_corner_turn_batch(const ct_job* d_jobs, int njobs, long ntiles, int nlev, hipStream_t s)
+{
+    if (!d_jobs || njobs <= 0 || ntiles <= 0 || ntiles > 0x7fffffffL || nlev < 0 || nlev > kCtMaxLevels)
+        return hipErrorInvalidValue;
+    const dim3 grid((unsigned)ntiles);
+    switch (nlev) {
+    case 0: hipLaunchKernelGGL((corner_turn_wide_batch_kernel<0>), grid, dim3(kCtT), 0, s, d_jobs, njobs); break;
+    case 1: hipLaunchKernelGGL((corner_turn_wide_batch_kernel<1>), grid, dim3(kCtT), 0, s, d_jobs, njobs); break;
+    case 2: hipLaunchKernelGGL((corner_turn_wide_batch_kernel<2>), grid, dim3(kCtT), 0, s, d_jobs, njobs); break;
+    default: hipLaunchKernelGGL((corner_turn_wide_batch_kernel<3>), grid, dim3(kCtT), 0, s, d_jobs, njobs); break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_corner_turn_levels(const void* in_tc, int sample_bytes, long nsamp, int nchans,

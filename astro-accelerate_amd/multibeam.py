@@ -27,7 +27,7 @@ def aa_style_ranges(dm_max: float, nranges: int):
     """AA-style DM plan to dm_max in 1..4 ranges: the DM step doubles with the
     binning factor from range to range (step / inBin constant), as in AA's
     own multi-range plans (examples/src/filterbank_dedispersion.cpp:26-33)."""
-    base = {500: 0.25, 1000: 0.4, 3000: 0.5}[int(dm_max)]
+    base = {100: 0.5, 200: 0.5, 500: 0.25, 1000: 0.4, 3000: 0.5}[int(dm_max)]
     edges = {1: [0.0, 1.0], 2: [0.0, 1 / 3, 1.0], 3: [0.0, 1 / 6, 1 / 2, 1.0],
              4: [0.0, 1 / 10, 1 / 4, 1 / 2, 1.0]}[nranges]
     out = []
@@ -39,18 +39,18 @@ def aa_style_ranges(dm_max: float, nranges: int):
 
 def multibeam_specs(nbeams: int = 48, seed: int = 20261202,
                     nchans_set=(1024, 2048, 4096), nsamp_set=(65536, 131072, 262144, 524288),
-                    dm_max_set=(500, 1000, 3000)):
-    """Seeded beam specs: nchans, nsamp (powers of 2), DM max, 1-4 AA-style
-    ranges; combinations whose dispersion sweep does not fit the beam's
-    length are drawn again (DM 3000 needs >= 262144 samples and >= 2 ranges,
-    DM 1000 >= 131072 samples)."""
+                    dm_max_set=(500, 1000, 3000), max_ranges: int = 4):
+    """Seeded beam specs: nchans, nsamp (powers of 2), DM max, 1-max_ranges
+    AA-style ranges; combinations whose dispersion sweep does not fit the
+    beam's length are drawn again (DM 3000 needs >= 262144 samples and >= 2
+    ranges, DM 1000 >= 131072 samples)."""
     rng = np.random.default_rng(seed)
     specs = []
     while len(specs) < nbeams:
         nchans = int(rng.choice(nchans_set))
         nsamp = int(rng.choice(nsamp_set))
         dm_max = int(rng.choice(dm_max_set))
-        nr = int(rng.integers(1, 5))
+        nr = int(rng.integers(1, max_ranges + 1))
         if dm_max == 3000 and (nsamp < 262144 or nr < 2):
             continue
         if dm_max == 1000 and nsamp < 131072:

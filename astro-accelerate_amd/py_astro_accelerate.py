@@ -185,6 +185,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "aa_hip_ddtr_batch_execute": ([vp, ctypes.POINTER(vp), pi, ctypes.POINTER(vp), ctypes.POINTER(sz), i, vp],
                                       i),
         "aa_hip_ddtr_batch_stats": ([vp, pi, ctypes.POINTER(ctypes.c_long), pi], i),
+        "aa_hip_ddtr_batch_ct_stats": ([vp, pi, pi], i),
         "aa_hip_refine_create": ([i, i, ctypes.POINTER(i64), ctypes.POINTER(i64), pi, pi, ctypes.POINTER(vp)], i),
         "aa_hip_refine_destroy": ([vp], i),
         "aa_hip_refine_set_loss": ([vp, i], i),
@@ -855,7 +856,11 @@ class DdtrBatch:
         nt = ctypes.c_long()
         check(_lib_or_raise().aa_hip_ddtr_batch_stats(self.handle, ctypes.byref(nj), ctypes.byref(nt),
                                                       ctypes.byref(nd)), "aa_hip_ddtr_batch_stats")
-        return {"jobs": nj.value, "tiles": nt.value, "direct_jobs": nd.value}
+        cb, co = ctypes.c_int(), ctypes.c_int()
+        check(_lib_or_raise().aa_hip_ddtr_batch_ct_stats(self.handle, ctypes.byref(cb), ctypes.byref(co)),
+              "aa_hip_ddtr_batch_ct_stats")
+        return {"jobs": nj.value, "tiles": nt.value, "direct_jobs": nd.value,
+                "corner_turn_launches": cb.value, "corner_turns_alone": co.value}
 
 
 def corner_turn_u8(d_in_tc, nsamp: int, nchans: int, d_out, out_pitch: int, stream=None):

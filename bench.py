@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--tune-cache", default=None, help="config 4: JSON file of autotuned chunk shapes")
+    ap.add_argument("--beam-set", default="standard", choices=["standard", "small"],
+                    help="config 4: the 48 beams (standard), or 256 small beams (64-256 channels, "
+                         "2^14-2^15 samples, DM max 100/200 in 1-2 ranges) whose launches do not fill the GPU")
     ap.add_argument("--kernel", default="auto", choices=["auto", "direct", "lds", "c32", "c16", "pair", "quad"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-slice", type=int, default=None,
@@ -586,14 +589,18 @@ def bench_multibeam(args):
     instantiation over all beams' ranges), against the same beams launched
     one after another (execute_chunk per beam) on the same tuned shapes
     (speedup_vs_per_beam_same_kernels, beside the 1.5x target), and on the
-    byte-row shapes.  Chunk shapes are autotuned per shape class first."""
+    byte-row shapes.  Chunk shapes are autotuned per shape class first.
+    --beam-set small: 256 beams of 64-256 channels whose launches hold a few
+    dozen tiles each, fewer than the GPU's 256 CUs."""
     import torch
     import py_astro_accelerate as aa
     import multibeam as mb
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    specs = mb.multibeam_specs(48, SEED)
+    small = args.beam_set == "small"
+    specs = (mb.multibeam_specs(256, SEED, nchans_set=(64, 128, 256), nsamp_set=(16384, 32768),
+                                dm_max_set=(100, 200), max_ranges=2) if small else mb.multibeam_specs(48, SEED))
     stream = torch.cuda.Stream()
     t0 = time.perf_counter()
     with torch.cuda.stream(stream):
@@ -671,7 +678,9 @@ def bench_multibeam(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_batched,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
         "data": "synthetic: seeded N(128,16^2) u8 noise + 2 dispersed pulses per beam, resident in HBM",
-        "config": {"workload": "multibeam 48 beams, 1024-4096 ch, 2^16-2^19 samples, 1-4 AA ranges (config 4)",
+        "config": {"workload": ("multibeam small: 256 beams, 64-256 ch, 2^14-2^15 samples, DM max 100/200, 1-2 ranges"
+                                if small else
+                                "multibeam 48 beams, 1024-4096 ch, 2^16-2^19 samples, 1-4 AA ranges (config 4)"),
                    "beams": len(beams), "ranges": nranges, "dm_x_samples": dmxs, "channel_adds": adds,
                    "parallelism": "single GPU"},
         "per_beam_ms_per_step": ms_per_beam,

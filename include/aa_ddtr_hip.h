@@ -311,6 +311,11 @@ int aa_hip_ddtr_batch_execute(aa_hip_ddtr_batch *h, const void *const *d_inputs_
 /* Jobs (ranges) in the batched launches, their tiles, and ranges launched
  * on their own (quad shape or direct kernel) in the last execute. */
 int aa_hip_ddtr_batch_stats(const aa_hip_ddtr_batch *h, int *njobs, long *ntiles, int *ndirect);
+/* Corner turns of the last execute: *batched = launches that turned many
+ * beams at once (one per number of binned levels written), *own = beams
+ * turned by a launch of their own (channel counts that are not a multiple
+ * of 16, or unaligned buffers). */
+int aa_hip_ddtr_batch_ct_stats(const aa_hip_ddtr_batch *h, int *batched, int *own);
 
 /* Replaces corner_turn_SM_inplace (src/aa_corner_turn.cpp:118-121) for the
  * 8-bit path: [t][c] -> [c][t] into a separate u8 buffer with row pitch

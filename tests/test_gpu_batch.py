@@ -22,8 +22,9 @@ def say(msg):
 
 
 def small_specs():
-    """Six small beams: ds up to ~2.6 samples/DM (100 MHz below 1760 MHz) or
-    the full 810 MHz band for the last two (large delay spreads)."""
+    """Seven small beams: ds up to ~2.6 samples/DM (100 MHz below 1760 MHz) or
+    the full 810 MHz band for two of them (large delay spreads); the last
+    has 40 channels, which the batched corner turn does not take."""
     narrow = dict(fch1=1760.0, tsamp=64e-6)
     specs = [
         dict(nchans=64, nsamp=16384, ranges=[(0, 200, 0.5, 1, 1)], bw=100.0, seed=11,
@@ -38,6 +39,9 @@ def small_specs():
              pulses=[(250.0, 0.2, 3)]),
         # full band, DM step 8: no LDS shape holds a tile's window -> direct kernel
         dict(nchans=48, nsamp=32768, ranges=[(0, 300, 8.0, 1, 1)], bw=810.0, seed=16, pulses=[(160.0, 0.3, 6)]),
+        # 40 channels (not a multiple of 16): corner-turned by a launch of its own
+        dict(nchans=40, nsamp=16384, ranges=[(0, 120, 0.5, 1, 1), (120, 240, 1.0, 2, 2)], bw=100.0, seed=17,
+             pulses=[(70.0, 0.4, 5)]),
     ]
     for s in specs:
         s.update(narrow)
@@ -98,6 +102,9 @@ def test_batched_launch_matches_oracle(aa, oracle, cuda, beams, use_graph):
     # and the pair-shape ranges; the direct-kernel range on its own
     assert st["jobs"] >= 3 and st["direct_jobs"] >= 1
     assert st["jobs"] + st["direct_jobs"] == sum(len(b.tps) for b in beams)
+    # corner turns: one batched launch per number of binned levels written
+    # (0, 1 and 2 among the six beams of 48-256 channels), the 40-channel beam alone
+    assert st["corner_turn_launches"] == 3 and st["corner_turns_alone"] == 1
     check_against_oracle(oracle, beams, f"batched graph={use_graph}")
 
 
