@@ -144,8 +144,9 @@ int ddtr_batch::enqueue(const void *const *inputs, const int *t_processed, float
                 if (e != hipSuccess) return hip_fail(e, "corner-turn job table");
                 F.cap = F.jobs.size();
             }
-            // F.jobs stays as it is until the caller's synchronisation
+            // from pageable memory: wait for it before F.jobs can change
             e = hipMemcpyAsync(F.d_jobs, F.jobs.data(), F.jobs.size() * sizeof(ct_job), hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
         }
         if (e == hipSuccess)
             e = launch_corner_turn_batch(F.d_jobs, (int)F.jobs.size(), F.jobs.back().tile_end, F.jobs.back().nlev, s);
