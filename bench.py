@@ -51,13 +51,17 @@ CONFIGS = {
 ADD_ROOFLINE_T = 78.64   # T adds/s, packed FP32 VALU adds
 # HBM bytes moved by the dedisperse launches of one step, from rocprofv3
 # FETCH_SIZE (x2.000, calibrated on a 1 GiB device copy) + WRITE_SIZE (x1.000)
-# summed over the 8 launches of chunk shape 4 (pair rows on 2016-sample
-# tiles, the automatic choice); recorded in
-# profiles/dedisperse_wide_8range_4096ch_1048576_hbm_traffic.txt.
+# summed over the launches of chunk shape 4 (pair rows on 2016-sample
+# tiles, the automatic choice): the 8 launches of the 8-range plan
+# (profiles/dedisperse_wide_8range_4096ch_1048576_hbm_traffic.txt) and the
+# one of the single-range plan (519.3 MB read + 505.5 MB written,
+# profiles/dedisperse_wide_1range_512ch_131072_kernel_trace.txt).
 # Counters cannot be read live, so the figure is tied to the kernel it was
 # taken on.
 MEASURED_TRAFFIC = {2: {"bytes": 173.5e9, "kernel": "pair",
-                        "profile": "profiles/dedisperse_wide_8range_4096ch_1048576_hbm_traffic.txt"}}
+                        "profile": "profiles/dedisperse_wide_8range_4096ch_1048576_hbm_traffic.txt"},
+                    1: {"bytes": 1.0248e9, "kernel": "pair",
+                        "profile": "profiles/dedisperse_wide_1range_512ch_131072_kernel_trace.txt"}}
 # The kernels add 8-bit samples with integer SWAR (packed 16-bit fields,
 # v_perm + v_add3_u32), not FP32 adds, so the FP32 figure above is a nominal
 # ratio.  Their own ceiling is the VALU issue rate: 256 CU x 4 SIMD x 16
@@ -77,8 +81,11 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 5; 200 on the single-range plan, whose step is ~1.3 ms)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default 2; 200 on the single-range plan, so the clocks have "
+                         "risen before the timed region)")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--tune-cache", default=None, help="config 4: JSON file of autotuned chunk shapes")
     ap.add_argument("--beam-set", default="standard", choices=["standard", "small"],
@@ -112,6 +119,11 @@ def main():
                     help="--e2e: with the pipeline's zero_dm_with_outliers option; alone: time zero-DM with "
                          "outlier rejection on the config's [t][c] input (not the contract line)")
     args = ap.parse_args()
+    short = args.config == 1 and not (args.e2e or args.zero_dm or args.zero_dm_outliers or args.normalization)
+    if args.steps is None:
+        args.steps = 200 if short else 5
+    if args.warmup is None:
+        args.warmup = 200 if short else 2
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
               "dedispersion would shard by DM range or beam with no exchange", file=sys.stderr)
