@@ -68,7 +68,7 @@ MEASURED_TRAFFIC = {2: {"bytes": 173.5e9, "kernel": "pair",
 # lanes x 2.4 GHz = 39.32 T lane-ops/s, divided by the lane-ops the kernel
 # issues per channel-add (SQ_INSTS_VALU x 64 / channel-adds, rocprofv3).
 VALU_LANE_OPS_T = 39.32
-MEASURED_ISSUE = {2: {"lane_ops_per_add": 0.407, "kernel": "pair",
+MEASURED_ISSUE = {2: {"lane_ops_per_add": 0.401, "kernel": "pair",
                       "profile": "profiles/dedisperse_wide_8range_4096ch_1048576_sq_counters.txt"}}
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
