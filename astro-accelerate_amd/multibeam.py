@@ -63,6 +63,14 @@ def multibeam_specs(nbeams: int = 48, seed: int = 20261202,
     return specs
 
 
+def small_multibeam_specs(seed: int = 20261202):
+    """256 small beams: 64-256 channels, 2^14-2^15 samples, DM max 100/200 in
+    1-2 ranges; a range holds a few dozen 16-DM tiles, fewer than the GPU's
+    CUs, so launches per range leave most of the GPU idle."""
+    return multibeam_specs(256, seed, nchans_set=(64, 128, 256), nsamp_set=(16384, 32768), dm_max_set=(100, 200),
+                           max_ranges=2)
+
+
 @dataclass
 class Beam:
     spec: dict

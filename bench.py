@@ -611,8 +611,7 @@ def bench_multibeam(args):
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     small = args.beam_set == "small"
-    specs = (mb.multibeam_specs(256, SEED, nchans_set=(64, 128, 256), nsamp_set=(16384, 32768),
-                                dm_max_set=(100, 200), max_ranges=2) if small else mb.multibeam_specs(48, SEED))
+    specs = mb.small_multibeam_specs(SEED) if small else mb.multibeam_specs(48, SEED)
     stream = torch.cuda.Stream()
     t0 = time.perf_counter()
     with torch.cuda.stream(stream):
